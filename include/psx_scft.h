@@ -1,0 +1,177 @@
+/*
+ * psx_scft.h -- C ABI of the MI355X-native SCFT engine (libpsx_scft.so).
+ *
+ * Plain C: pointers, sizes and status codes.  No C++ exception crosses this
+ * boundary; every call returns psx_status (0 = OK) and psx_last_error()
+ * returns a thread-local message for the last failing call on that thread.
+ * Device pointers ("d_") live in HBM of the current HIP device; host
+ * pointers ("h_") are ordinary host memory.  Streams are hipStream_t passed
+ * as void* (NULL = the object creates and owns a default-priority stream).
+ *
+ * Each entry point cites the reference interface it replaces
+ * (paths relative to the PolySwift++ tree polyswift/).
+ */
+#ifndef PSX_SCFT_H
+#define PSX_SCFT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int psx_status;
+enum {
+  PSX_OK = 0,
+  PSX_ERR_INVALID = 1,   /* bad argument (null pointer, bad enum, ...) */
+  PSX_ERR_SHAPE = 2,     /* grid shape / size mismatch */
+  PSX_ERR_STEPS = 3,     /* non-integral or too few block steps */
+  PSX_ERR_NAN = 4,       /* Q is NaN/Inf (reference PsBlock::calcBigQ) */
+  PSX_ERR_HIP = 5,       /* HIP runtime error */
+  PSX_ERR_NOMEM = 6,     /* device allocation failed */
+  PSX_ERR_STATE = 7      /* call out of order (e.g. Q before propagate) */
+};
+
+/* Arithmetic type of the contour sweep (FFTs, w-factors, stored slices).
+ * Q, the contour quadrature, the field update and F are always FP64. */
+typedef enum { PSX_FP64 = 0, PSX_FP32 = 1 } psx_prec;
+
+/* Copies the last error message of the calling thread into buf. */
+void psx_last_error(char* buf, size_t len);
+/* Library version string. */
+const char* psx_version(void);
+
+/* ===================================================================== */
+/* FFT pair plugin.  Replaces PsFFTBase<T,NDIM> (psbase/PsFFTBase.h:39-147) */
+/* as implemented by PsFFTW (psfft/PsFFTW.cpp:53-224).                   */
+/* ===================================================================== */
+typedef struct psx_fft psx_fft;
+
+/* Plan for an nx*ny*nz C-order real grid (PsFFTW::buildData,
+ * psfft/PsFFTW.cpp:53-88).  nz = 1 (and ny = 1) give 2D (1D) grids. */
+psx_status psx_fft_plan_create(int nx, int ny, int nz, psx_prec prec,
+                               void* stream, psx_fft** out);
+psx_status psx_fft_plan_destroy(psx_fft* plan);
+/* Number of real elements of data/kdata/res (PsFFTBase::getFFTSize,
+ * psbase/PsFFTBase.h:75; for the serial plan this is nx*ny*nz). */
+size_t psx_fft_size(const psx_fft* plan);
+/* res = Re(IFFT(kdata .* FFT(data + 0i))), both transforms unnormalised,
+ * forward sign -, backward +; kdata full grid, C order, real
+ * (PsFFTBase::scaledFFTPair, psbase/PsFFTBase.h:105-106;
+ *  psfft/PsFFTW.cpp:196-224).  Device pointers; element type per prec.
+ * data/res may alias.  Asynchronous on the plan's stream. */
+psx_status psx_fft_scaled_pair(psx_fft* plan, const void* d_data,
+                               const void* d_kdata, void* d_res);
+/* Same with host arrays (copies over PCIe; synchronous). */
+psx_status psx_fft_scaled_pair_host(psx_fft* plan, const void* h_data,
+                                    const void* h_kdata, void* h_res);
+
+/* ===================================================================== */
+/* SCFT engine for a bulk AB diblock melt: the block propagators          */
+/* (PsBlockBase / PsFlexPseudoSpec), Q, densities, Flory interaction and  */
+/* the field updaters (PsSteepDUpdater, plus Anderson mixing, which the   */
+/* reference does not have).                                              */
+/* ===================================================================== */
+typedef struct psx_grid {
+  int nx, ny, nz;        /* numCellsGlobal */
+  double dx, dy, dz;     /* cellSizes in Rg (psgrid/PsUniCartGrid.cpp:30-43) */
+} psx_grid;
+
+typedef struct psx_diblock {
+  double fA;             /* lengthfrac of block A; block B has 1 - fA */
+  double ds;             /* contour step (reference "ds"), s in [0, 1] */
+  double bA, bB;         /* bSegRatio of each block */
+  double chiN;           /* chi * scaleLength (PsFloryInteraction.cpp:160-161) */
+} psx_diblock;
+
+typedef struct psx_scft psx_scft;
+
+/* Build: block steps (psbase/PsBlockBase.cpp:252-326, error
+ * PSX_ERR_STEPS), exp(-ds b^2 k^2) tables (PsFlexPseudoSpec::build_k2,
+ * pspolymer/PsFlexPseudoSpec.cpp:188-206), device buffers for all
+ * contour slices.  Fields start at w = 0, p = 0. */
+psx_status psx_scft_create(const psx_grid* grid, const psx_diblock* chain,
+                           psx_prec prec, void* stream, psx_scft** out);
+psx_status psx_scft_destroy(psx_scft* e);
+/* Switch the sweep precision (allocates the other slice set on first use). */
+psx_status psx_scft_set_precision(psx_scft* e, psx_prec prec);
+/* Block steps of A and B. */
+psx_status psx_scft_block_steps(const psx_scft* e, int* sA, int* sB);
+
+/* Conjugate fields w_A, w_B and pressure p, FP64 device arrays of N cells
+ * (PsMonomerDens conjugate field, PsConstraint pressure field,
+ *  psphysf/PsMonomerDens.h:93-132, psphysf/PsConstraint.h:95-104). */
+psx_status psx_scft_set_fields(psx_scft* e, const double* d_wA,
+                               const double* d_wB, const double* d_p);
+psx_status psx_scft_get_fields(psx_scft* e, double* d_wA, double* d_wB,
+                               double* d_p);
+psx_status psx_scft_set_fields_host(psx_scft* e, const double* h_wA,
+                                    const double* h_wB, const double* h_p);
+psx_status psx_scft_get_fields_host(psx_scft* e, double* h_wA, double* h_wB,
+                                    double* h_p);
+
+/* w-factors and all block propagators (PsFlexPseudoSpec::reset + solveQ,
+ * pspolymer/PsFlexPseudoSpec.cpp:159-181,244-293; block-graph order of
+ * PsBlockCopolymer::update, pspolymer/PsBlockCopolymer.cpp:189-239). */
+psx_status psx_scft_propagate(psx_scft* e);
+/* Q (PsBlock::calcBigQ, pspolymer/PsBlock.cpp:205-237): PSX_ERR_NAN on NaN. */
+psx_status psx_scft_Q(psx_scft* e, double* h_Q);
+/* Densities from the contour quadrature (PsBlock::setCalcQQTIntegral +
+ * setPhysFields, pspolymer/PsBlock.cpp:154-196,256-380), FP64 device;
+ * either output may be NULL (the engine keeps its own copy). */
+psx_status psx_scft_densities(psx_scft* e, double* d_phiA, double* d_phiB);
+/* Host copies of the densities of the last densities()/iterate() call. */
+psx_status psx_scft_get_densities_host(psx_scft* e, double* h_phiA,
+                                       double* h_phiB);
+/* Copy one stored contour slice (FP64 out; slices of the forward sweep are
+ * q_A[0..sA] then q_B[1..sB]; backward: q+_B[0..sB] then q+_A[1..sA]). */
+psx_status psx_scft_get_slice_host(psx_scft* e, int backward, int t,
+                                   double* h_out);
+
+/* One steepest-descent update from the current densities
+ * (PsSteepDUpdater::update, pseffhamil/PsSteepDUpdater.cpp:159-203; with
+ * Flory calcDfD, PsConstraint::updatePres and calcFeField) -> F in the
+ * reference order (after the w update). */
+psx_status psx_scft_update_steepest(psx_scft* e, double lambda0,
+                                    double lambda1, double* h_F);
+
+/* Field-update method for psx_scft_iterate. */
+typedef struct psx_mixing {
+  int kind;              /* 0 steepest descent (lambda0, lambda1), 1 Anderson */
+  double lambda0, lambda1;
+  int depth;             /* Anderson history depth m (<= 32) */
+  double beta;           /* Anderson mixing parameter */
+  double anderson_start; /* Picard (lambda0) while residual RMS > this */
+  int f_order;           /* 0 reference order (F after update), 1 consistent */
+} psx_mixing;
+
+/* Run up to n_iters full SCFT iterations (PsDomain::update,
+ * pstrol/PsDomain.cpp:332-372), stopping early when the residual RMS
+ * sqrt(mean(dH_A^2 + dH_B^2)) < tol (tol <= 0: never).  Histories (host,
+ * length n_iters, may be NULL) receive F, Q and the residual RMS of each
+ * iteration.  *h_done = iterations run.  use_graph != 0 captures one
+ * iteration as a hipGraph and replays it. */
+psx_status psx_scft_iterate(psx_scft* e, const psx_mixing* mix, int n_iters,
+                            double tol, int use_graph, double* h_F,
+                            double* h_Q, double* h_err, int* h_done);
+/* Clear the Anderson history (e.g. after a precision switch). */
+psx_status psx_scft_reset_history(psx_scft* e);
+
+/* Free energy of the current fields and densities (reference order). */
+psx_status psx_scft_free_energy(psx_scft* e, double* h_F);
+
+/* dF/dL_x,y,z at fixed fields from the stored propagators (adjoint of the
+ * split-step sweep); no reference counterpart (the reference has no
+ * analytic stress).  Requires a preceding propagate. */
+psx_status psx_scft_box_stress(psx_scft* e, double* h_dFdL);
+
+/* Average device time (ms) of one contour step of the dual sweep and of
+ * each of its line passes (z, y, x), timed with hipEvents on the engine's
+ * stream over `reps` repetitions. */
+psx_status psx_scft_time_step(psx_scft* e, int reps, double* h_ms_step,
+                              double* h_ms_pass3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

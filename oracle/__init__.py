@@ -1,0 +1,240 @@
+"""CPU oracle for the SCFT hot path -- TEST INFRASTRUCTURE ONLY.
+
+ctypes front end of ``liboracle_scft.so`` (built from ``psx_oracle.cpp`` by
+``oracle/Makefile``).  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this package, and only as the
+checker.  The engine (``polyswift_plusplus_amd``) never imports it.
+
+Each function names the reference lines it restates; the restatement itself
+is in ``psx_oracle.cpp``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle_scft.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
+        ("ds", ctypes.c_double), ("fA", ctypes.c_double), ("chiN", ctypes.c_double),
+        ("bA", ctypes.c_double), ("bB", ctypes.c_double),
+        ("mix", ctypes.c_int),
+        ("lambda0", ctypes.c_double), ("lambda1", ctypes.c_double),
+        ("depth", ctypes.c_int), ("beta", ctypes.c_double),
+        ("anderson_start", ctypes.c_double),
+        ("f_order", ctypes.c_int),
+    ]
+
+
+@dataclass
+class Params:
+    """One bulk AB diblock melt.  Lengths in Rg, s in [0, 1]."""
+    nx: int
+    ny: int
+    nz: int
+    dx: float
+    dy: float
+    dz: float
+    ds: float
+    fA: float
+    chiN: float
+    bA: float = 1.0
+    bB: float = 1.0
+    mix: int = 0            # 0 steepest descent, 1 Anderson
+    lambda0: float = 0.1
+    lambda1: float = 0.0
+    depth: int = 0
+    beta: float = 0.1
+    anderson_start: float = 1e300
+    f_order: int = 0        # 0 reference (F after update), 1 consistent
+
+    @property
+    def N(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    @property
+    def shape(self):
+        return (self.nx, self.ny, self.nz)
+
+    def c(self) -> OrcParams:
+        return OrcParams(self.nx, self.ny, self.nz, self.dx, self.dy, self.dz,
+                         self.ds, self.fA, self.chiN, self.bA, self.bB, self.mix,
+                         self.lambda0, self.lambda1, self.depth, self.beta,
+                         self.anderson_start, self.f_order)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {_LIB_PATH} (run make -C oracle)")
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_get_threads.restype = ctypes.c_int
+        L.orc_block_steps.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.orc_block_steps.restype = ctypes.c_int
+        L.orc_kfac.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 5 + [_dp]
+        L.orc_fft3.argtypes = [ctypes.c_int] * 3 + [_dp, _dp, ctypes.c_int]
+        L.orc_dft3_brute.argtypes = [ctypes.c_int] * 3 + [_dp] * 4 + [ctypes.c_int]
+        L.orc_scaled_fft_pair.argtypes = [ctypes.c_int] * 3 + [_dp] * 3
+        L.orc_solve_q.argtypes = [ctypes.c_int] * 3 + [_dp] * 3 + [ctypes.c_int, _dp]
+        L.orc_quadrature.argtypes = [ctypes.c_long, ctypes.c_int, ctypes.c_double,
+                                     _dp, _dp, ctypes.c_double, _dp]
+        L.orc_diblock_densities.argtypes = [ctypes.POINTER(OrcParams)] + [_dp] * 5
+        L.orc_sd_update.argtypes = [ctypes.POINTER(OrcParams), ctypes.c_long] + [_dp] * 5 + [ctypes.c_double]
+        L.orc_sd_update.restype = ctypes.c_double
+        L.orc_free_energy.argtypes = [ctypes.POINTER(OrcParams), ctypes.c_long] + [_dp] * 4 + [ctypes.c_double]
+        L.orc_free_energy.restype = ctypes.c_double
+        L.orc_scft_run.argtypes = ([ctypes.POINTER(OrcParams)] + [_dp] * 3 +
+                                   [ctypes.c_int, ctypes.c_double] + [_dp] * 5)
+        L.orc_box_stress.argtypes = [ctypes.POINTER(OrcParams), _dp, _dp, _dp]
+        L.orc_time_steps.argtypes = [ctypes.c_int] * 4
+        L.orc_time_steps.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def set_threads(n: int) -> None:
+    lib().orc_set_threads(int(n))
+
+
+def block_steps(length_frac: float, ds: float) -> int:
+    """psbase/PsBlockBase.cpp:252-326; raises on a non-integral count."""
+    r = lib().orc_block_steps(length_frac, ds)
+    if r < 0:
+        raise ValueError(f"non-integral or too few block steps: {length_frac}/{ds}")
+    return r
+
+
+def kfac(shape, cell, ds, b=1.0) -> np.ndarray:
+    """exp(-ds b^2 k^2), full grid (psbase/PsGridField.cpp:66-104)."""
+    out = np.empty(shape, dtype=np.float64)
+    lib().orc_kfac(*shape, *cell, ds, b, _p(out))
+    return out
+
+
+def fft3(a: np.ndarray, sign: int = -1) -> np.ndarray:
+    re, im = _f64(a.real).copy(), _f64(np.imag(a)).copy()
+    if lib().orc_fft3(*a.shape, _p(re), _p(im), sign):
+        raise ValueError("bad shape")
+    return re + 1j * im
+
+
+def dft3_brute(a: np.ndarray, sign: int = -1) -> np.ndarray:
+    re, im = _f64(a.real), _f64(np.imag(a))
+    ore, oim = np.empty_like(re), np.empty_like(im)
+    lib().orc_dft3_brute(*a.shape, _p(re), _p(im), _p(ore), _p(oim), sign)
+    return ore + 1j * oim
+
+
+def scaled_fft_pair(data: np.ndarray, kdata: np.ndarray) -> np.ndarray:
+    """psfft/PsFFTW.cpp:196-224 (unnormalised, real part kept)."""
+    d, k = _f64(data), _f64(kdata)
+    out = np.empty_like(d)
+    lib().orc_scaled_fft_pair(*d.shape, _p(d), _p(k), _p(out))
+    return out
+
+
+def solve_q(kf: np.ndarray, wfac: np.ndarray, q0: np.ndarray, steps: int) -> np.ndarray:
+    """pspolymer/PsFlexPseudoSpec.cpp:244-293: slices[0..steps]."""
+    kf, wfac, q0 = _f64(kf), _f64(wfac), _f64(q0)
+    out = np.empty((steps + 1,) + kf.shape)
+    lib().orc_solve_q(*kf.shape, _p(kf), _p(wfac), _p(q0), steps, _p(out))
+    return out
+
+
+def quadrature(qs: np.ndarray, qts: np.ndarray, ds: float, Q: float) -> np.ndarray:
+    """pspolymer/PsBlock.cpp:256-380 (0.33333 Simpson + trapezoid tail)."""
+    qs, qts = _f64(qs), _f64(qts)
+    steps = qs.shape[0] - 1
+    out = np.empty(qs.shape[1:])
+    lib().orc_quadrature(out.size, steps, ds, _p(qs), _p(qts), Q, _p(out))
+    return out
+
+
+def diblock_densities(P: Params, wA, wB, want_slices: bool = False):
+    """Densities, Q (and optionally all slices qA, qB, qtB, qtA)."""
+    wA, wB = _f64(wA), _f64(wB)
+    phiA, phiB = np.empty_like(wA), np.empty_like(wB)
+    Q = ctypes.c_double()
+    cp = P.c()
+    sl = None
+    slp = None
+    if want_slices:
+        sA, sB = block_steps(P.fA, P.ds), block_steps(1.0 - P.fA, P.ds)
+        sl = np.empty((2 * (sA + 1) + 2 * (sB + 1),) + wA.shape)
+        slp = _p(sl)
+    rc = lib().orc_diblock_densities(ctypes.byref(cp), _p(wA), _p(wB), _p(phiA),
+                                     _p(phiB), ctypes.cast(ctypes.byref(Q), _dp), slp)
+    if rc:
+        raise RuntimeError(f"oracle densities failed rc={rc}")
+    if want_slices:
+        sA, sB = block_steps(P.fA, P.ds), block_steps(1.0 - P.fA, P.ds)
+        i0 = 0
+        qA = sl[i0:i0 + sA + 1]; i0 += sA + 1
+        qB = sl[i0:i0 + sB + 1]; i0 += sB + 1
+        qtB = sl[i0:i0 + sB + 1]; i0 += sB + 1
+        qtA = sl[i0:i0 + sA + 1]
+        return phiA, phiB, Q.value, (qA, qB, qtB, qtA)
+    return phiA, phiB, Q.value
+
+
+def sd_update(P: Params, wA, wB, p, phiA, phiB, Q):
+    """In-place steepest-descent update; returns F (reference order)."""
+    cp = P.c()
+    return lib().orc_sd_update(ctypes.byref(cp), wA.size, _p(wA), _p(wB), _p(p),
+                               _p(_f64(phiA)), _p(_f64(phiB)), Q)
+
+
+def free_energy(P: Params, wA, wB, phiA, phiB, Q) -> float:
+    cp = P.c()
+    return lib().orc_free_energy(ctypes.byref(cp), wA.size, _p(_f64(wA)), _p(_f64(wB)),
+                                 _p(_f64(phiA)), _p(_f64(phiB)), Q)
+
+
+def scft_run(P: Params, wA, wB, p=None, n_iters=1, tol=0.0):
+    """Run iterations in place on copies; returns dict of results."""
+    wA, wB = _f64(wA).copy(), _f64(wB).copy()
+    p = np.zeros_like(wA) if p is None else _f64(p).copy()
+    F = np.zeros(n_iters); Qh = np.zeros(n_iters); err = np.zeros(n_iters)
+    phiA, phiB = np.empty_like(wA), np.empty_like(wA)
+    cp = P.c()
+    done = lib().orc_scft_run(ctypes.byref(cp), _p(wA), _p(wB), _p(p), n_iters, tol,
+                              _p(F), _p(Qh), _p(err), _p(phiA), _p(phiB))
+    if done < 0:
+        raise RuntimeError("oracle scft_run failed")
+    return dict(wA=wA, wB=wB, p=p, F=F[:done], Q=Qh[:done], err=err[:done],
+                phiA=phiA, phiB=phiB, iters=done)
+
+
+def box_stress(P: Params, wA, wB) -> np.ndarray:
+    """dF/dL_d at fixed fields (no reference counterpart)."""
+    out = np.zeros(3)
+    cp = P.c()
+    if lib().orc_box_stress(ctypes.byref(cp), _p(_f64(wA)), _p(_f64(wB)), _p(out)):
+        raise RuntimeError("oracle stress failed")
+    return out
+
+
+def time_steps(shape, steps: int) -> float:
+    return lib().orc_time_steps(*shape, steps)

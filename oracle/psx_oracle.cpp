@@ -1,0 +1,620 @@
+/*
+ * psx_oracle.cpp -- CPU oracle for the SCFT hot path (FP64).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP engine.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ *
+ * "Faithful" pass structure: every element-wise operation is its own loop and
+ * the FFT pair is a full-grid complex-to-complex transform, as the reference
+ * does it (psfft/PsFFTW.cpp:196-224, pspolymer/PsFlexPseudoSpec.cpp:267-289).
+ * OpenMP threads (orc_set_threads) parallelise the FFT lines and the loops;
+ * with one thread it is the single-rank reference pass structure.
+ */
+#include "psx_oracle.h"
+
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef std::complex<double> cd;
+
+static int g_threads = 1;
+
+extern "C" void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
+extern "C" int orc_get_threads(void) { return g_threads; }
+
+/* ------------------------------------------------------------------------ */
+/* 1D mixed-radix FFT: recursive decimation in time, generic radix combine. */
+/* ------------------------------------------------------------------------ */
+namespace {
+
+struct Plan1D {
+  int n = 0;
+  std::vector<int> fac;   // radices, product = n
+  std::vector<cd> tw;     // tw[j] = exp(-2 pi i j / n)
+};
+
+std::vector<int> factorize(int n) {
+  std::vector<int> f;
+  while (n % 4 == 0) { f.push_back(4); n /= 4; }
+  while (n % 2 == 0) { f.push_back(2); n /= 2; }
+  for (int p = 3; n > 1; p += 2) {
+    while (n % p == 0) { f.push_back(p); n /= p; }
+    if (p * p > n && n > 1) { f.push_back(n); break; }
+  }
+  return f;
+}
+
+const Plan1D& get_plan(int n) {
+  static std::mutex mu;
+  static std::map<int, std::unique_ptr<Plan1D>> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(n);
+  if (it != cache.end()) return *it->second;
+  auto p = std::make_unique<Plan1D>();
+  p->n = n;
+  p->fac = factorize(n);
+  p->tw.resize(n);
+  const long double twopi = 6.283185307179586476925286766559L;
+  for (int j = 0; j < n; ++j) {
+    long double a = twopi * (long double)j / (long double)n;
+    p->tw[j] = cd((double)cosl(a), (double)-sinl(a));
+  }
+  const Plan1D& ref = *p;
+  cache[n] = std::move(p);
+  return ref;
+}
+
+// out[0..n) = DFT of in[j*is], j < n.  fs = stride into the size-N0 twiddles.
+void fft_work(cd* out, const cd* in, long is, int fs, const int* fac, int n,
+              const Plan1D& P, bool inverse) {
+  const int p = fac[0];
+  const int m = n / p;
+  if (m == 1) {
+    for (int r = 0; r < p; ++r) out[r] = in[r * is];
+  } else {
+    for (int r = 0; r < p; ++r)
+      fft_work(out + r * m, in + r * is, is * p, fs * p, fac + 1, m, P, inverse);
+  }
+  // out[r*m + k] holds Y_r[k]; X[k + q m] = sum_r W_n^{r(k + q m)} Y_r[k]
+  const int N0 = P.n;
+  cd t[64];
+  std::vector<cd> tbig;
+  cd* tp = t;
+  if (p > 64) { tbig.resize(p); tp = tbig.data(); }
+  for (int k = 0; k < m; ++k) {
+    for (int r = 0; r < p; ++r) {
+      cd w = P.tw[((long)fs * r * k) % N0];
+      if (inverse) w = std::conj(w);
+      tp[r] = out[r * m + k] * w;
+    }
+    for (int q = 0; q < p; ++q) {
+      cd acc = 0.0;
+      for (int r = 0; r < p; ++r) {
+        cd w = P.tw[((long)fs * m * ((r * q) % p)) % N0];
+        if (inverse) w = std::conj(w);
+        acc += tp[r] * w;
+      }
+      out[k + q * m] = acc;
+    }
+  }
+}
+
+void fft1d(cd* out, const cd* in, int n, bool inverse) {
+  if (n == 1) { out[0] = in[0]; return; }
+  const Plan1D& P = get_plan(n);
+  fft_work(out, in, 1, 1, P.fac.data(), n, P, inverse);
+}
+
+// Transform every line along `axis` of a C-order complex nx*ny*nz array.
+void fft_axis(cd* a, int nx, int ny, int nz, int axis, bool inverse) {
+  const int dims[3] = {nx, ny, nz};
+  const int n = dims[axis];
+  if (n == 1) return;
+  long inner = 1;
+  for (int d = axis + 1; d < 3; ++d) inner *= dims[d];
+  long outer = 1;
+  for (int d = 0; d < axis; ++d) outer *= dims[d];
+  const long nlines = outer * inner;
+  get_plan(n);  // build outside the parallel region
+#pragma omp parallel num_threads(g_threads)
+  {
+    std::vector<cd> buf(n), res(n);
+#pragma omp for schedule(static)
+    for (long l = 0; l < nlines; ++l) {
+      const long o = l / inner, i = l % inner;
+      cd* base = a + o * (long)n * inner + i;
+      for (int k = 0; k < n; ++k) buf[k] = base[(long)k * inner];
+      fft1d(res.data(), buf.data(), n, inverse);
+      for (int k = 0; k < n; ++k) base[(long)k * inner] = res[k];
+    }
+  }
+}
+
+void fft3(cd* a, int nx, int ny, int nz, bool inverse) {
+  fft_axis(a, nx, ny, nz, 2, inverse);
+  fft_axis(a, nx, ny, nz, 1, inverse);
+  fft_axis(a, nx, ny, nz, 0, inverse);
+}
+
+long ncells(const orc_params* P) { return (long)P->nx * P->ny * P->nz; }
+
+}  // namespace
+
+/* ------------------------------------------------------------------------ */
+extern "C" int orc_block_steps(double bs_len, double ds) {
+  // psbase/PsBlockBase.cpp:252-326 (forceBlockSteps = false, minBlockSteps 4)
+  const double bs = bs_len / ds;
+  const double intPart = (double)(long)bs;
+  const double dx = bs - intPart;
+  if (dx > 0.0001 && dx < 0.9999) return -1;
+  const long c = (long)std::ceil(bs), f = (long)std::floor(bs);
+  long r;
+  if (c == f) r = c;
+  else if (std::fabs(bs - (double)c) < std::fabs(bs - (double)f)) r = c;
+  else r = f;
+  if (r < 4) return -1;
+  return (int)r;
+}
+
+extern "C" void orc_kfac(int nx, int ny, int nz, double dx, double dy,
+                         double dz, double ds, double b, double* kfac) {
+  // psbase/PsGridField.cpp:66-104; pspolymer/PsFlexPseudoSpec.cpp:188-206
+  const double twopi = 2.0 * std::acos(-1.0);
+  const int g[3] = {nx, ny, nz};
+  const double h[3] = {dx, dy, dz};
+  double n2[3];
+  for (int d = 0; d < 3; ++d) n2[d] = (double)g[d] / 2.0;
+  long n = 0;
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j)
+      for (int k = 0; k < nz; ++k) {
+        const int idx[3] = {i, j, k};
+        double k2 = 0.0;
+        for (int d = 0; d < 3; ++d) {
+          double nk = n2[d] - std::fabs((double)idx[d] - n2[d]);
+          double kv = twopi * nk / (double)g[d];
+          kv = kv / h[d];
+          k2 += kv * kv;
+        }
+        kfac[n++] = std::exp(-1.0 * ds * k2 * b * b);
+      }
+}
+
+extern "C" int orc_fft3(int nx, int ny, int nz, double* re, double* im,
+                        int sign) {
+  if (nx < 1 || ny < 1 || nz < 1) return -1;
+  const long N = (long)nx * ny * nz;
+  std::vector<cd> a(N);
+  for (long i = 0; i < N; ++i) a[i] = cd(re[i], im[i]);
+  fft3(a.data(), nx, ny, nz, sign > 0);
+  for (long i = 0; i < N; ++i) { re[i] = a[i].real(); im[i] = a[i].imag(); }
+  return 0;
+}
+
+extern "C" int orc_dft3_brute(int nx, int ny, int nz, const double* re,
+                              const double* im, double* ore, double* oim,
+                              int sign) {
+  if (nx < 1 || ny < 1 || nz < 1) return -1;
+  const long double twopi = 6.283185307179586476925286766559L;
+  for (int a = 0; a < nx; ++a)
+    for (int b = 0; b < ny; ++b)
+      for (int c = 0; c < nz; ++c) {
+        long double sr = 0, si = 0;
+        for (int i = 0; i < nx; ++i)
+          for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+              // exact phase from integer residues
+              long double ph = twopi * (((long)a * i % nx) / (long double)nx +
+                                        ((long)b * j % ny) / (long double)ny +
+                                        ((long)c * k % nz) / (long double)nz);
+              long double cr = cosl(ph), ci = (sign > 0 ? 1 : -1) * sinl(ph);
+              long idx = ((long)i * ny + j) * nz + k;
+              sr += re[idx] * cr - im[idx] * ci;
+              si += re[idx] * ci + im[idx] * cr;
+            }
+        long o = ((long)a * ny + b) * nz + c;
+        ore[o] = (double)sr;
+        oim[o] = (double)si;
+      }
+  return 0;
+}
+
+extern "C" int orc_scaled_fft_pair(int nx, int ny, int nz, const double* data,
+                                   const double* kdata, double* res) {
+  // psfft/PsFFTW.cpp:196-224: pack, forward, scale Re/Im by kdata, backward,
+  // keep the real part; no 1/N.
+  if (nx < 1 || ny < 1 || nz < 1) return -1;
+  const long N = (long)nx * ny * nz;
+  std::vector<cd> in(N);
+#pragma omp parallel for num_threads(g_threads)
+  for (long n = 0; n < N; ++n) in[n] = cd(data[n], 0.0);
+  fft3(in.data(), nx, ny, nz, false);
+#pragma omp parallel for num_threads(g_threads)
+  for (long n = 0; n < N; ++n)
+    in[n] = cd(in[n].real() * kdata[n], in[n].imag() * kdata[n]);
+  fft3(in.data(), nx, ny, nz, true);
+#pragma omp parallel for num_threads(g_threads)
+  for (long n = 0; n < N; ++n) res[n] = in[n].real();
+  return 0;
+}
+
+extern "C" int orc_solve_q(int nx, int ny, int nz, const double* kfac,
+                           const double* wfac, const double* q0, int steps,
+                           double* slices) {
+  // pspolymer/PsFlexPseudoSpec.cpp:244-293
+  const long N = (long)nx * ny * nz;
+  const double scaleFFT = 1.0 / (double)N;
+  std::vector<double> qX(q0, q0 + N), qw(N), res(N);
+  std::memcpy(slices, qX.data(), N * sizeof(double));
+  for (int ss = 1; ss <= steps; ++ss) {
+#pragma omp parallel for num_threads(g_threads)
+    for (long n = 0; n < N; ++n) qw[n] = qX[n] * wfac[n];
+    if (orc_scaled_fft_pair(nx, ny, nz, qw.data(), kfac, res.data())) return -1;
+#pragma omp parallel for num_threads(g_threads)
+    for (long n = 0; n < N; ++n) qX[n] = res[n] * wfac[n];
+#pragma omp parallel for num_threads(g_threads)
+    for (long n = 0; n < N; ++n) qX[n] = qX[n] * scaleFFT;
+    std::memcpy(slices + (long)ss * N, qX.data(), N * sizeof(double));
+  }
+  return 0;
+}
+
+extern "C" void orc_quadrature(long N, int steps, double ds, const double* qs,
+                               const double* qts, double Q, double* out) {
+  // pspolymer/PsBlock.cpp:256-380.  M = steps + 1 points; Simpson with the
+  // literal 0.33333 over the first M (odd) or M-1 (even) points, then a
+  // trapezoid on the last interval when M is even, then the 1/Q scale.
+  const long M = steps + 1;
+  const bool even = (M % 2) == 0;
+  const long numStail = even ? M - 4 : M - 3;
+  for (long r = 0; r < N; ++r) out[r] = 0.0;
+  for (long n = 0; n <= numStail; n += 2) {
+    const double* q1 = qs + n * N;
+    const double* q2 = qs + (n + 1) * N;
+    const double* q3 = qs + (n + 2) * N;
+    const double* t1 = qts + (M - n - 1) * N;
+    const double* t2 = qts + (M - n - 2) * N;
+    const double* t3 = qts + (M - n - 3) * N;
+    for (long r = 0; r < N; ++r)
+      out[r] += q1[r] * t1[r] + 4.0 * (q2[r] * t2[r]) + q3[r] * t3[r];
+  }
+  const double simp = 0.33333 * ds;
+  for (long r = 0; r < N; ++r) out[r] *= simp;
+  if (even) {
+    const double* q1 = qs + (M - 2) * N;
+    const double* q2 = qs + (M - 1) * N;
+    const double* t1 = qts + 1 * N;
+    const double* t2 = qts + 0 * N;
+    for (long r = 0; r < N; ++r)
+      out[r] += (q1[r] * t1[r] + q2[r] * t2[r]) * (ds * 0.5);
+  }
+  const double iq = 1.0 / Q;
+  for (long r = 0; r < N; ++r) out[r] *= iq;
+}
+
+extern "C" int orc_diblock_densities(const orc_params* P, const double* wA,
+                                     const double* wB, double* phiA,
+                                     double* phiB, double* Qout,
+                                     double* slices) {
+  const int nx = P->nx, ny = P->ny, nz = P->nz;
+  const long N = ncells(P);
+  const int sA = orc_block_steps(P->fA, P->ds);
+  const int sB = orc_block_steps(1.0 - P->fA, P->ds);
+  if (sA < 0 || sB < 0) return -1;
+  // exp(-ds b^2 k^2) per block (pspolymer/PsFlexPseudoSpec.cpp:188-206)
+  std::vector<double> kA(N), kB(N), fa(N), fb(N), ones(N, 1.0);
+  orc_kfac(nx, ny, nz, P->dx, P->dy, P->dz, P->ds, P->bA, kA.data());
+  orc_kfac(nx, ny, nz, P->dx, P->dy, P->dz, P->ds, P->bB, kB.data());
+  // w-factors (pspolymer/PsFlexPseudoSpec.cpp:159-181)
+  const double ds2 = -0.5 * P->ds;
+  for (long n = 0; n < N; ++n) { fa[n] = std::exp(ds2 * wA[n]); fb[n] = std::exp(ds2 * wB[n]); }
+  std::vector<double> own;
+  double* S = slices;
+  const long total = (long)(sA + 1) * 2 + (long)(sB + 1) * 2;
+  if (!S) { own.resize(total * N); S = own.data(); }
+  double* qA = S;
+  double* qB = qA + (long)(sA + 1) * N;
+  double* qtB = qB + (long)(sB + 1) * N;
+  double* qtA = qtB + (long)(sB + 1) * N;
+  // diblock order A-fwd, B-fwd, B-bwd, A-bwd (pspolymer/PsBlockCopolymer.cpp:189-272)
+  if (orc_solve_q(nx, ny, nz, kA.data(), fa.data(), ones.data(), sA, qA)) return -1;
+  if (orc_solve_q(nx, ny, nz, kB.data(), fb.data(), qA + (long)sA * N, sB, qB)) return -1;
+  if (orc_solve_q(nx, ny, nz, kB.data(), fb.data(), ones.data(), sB, qtB)) return -1;
+  if (orc_solve_q(nx, ny, nz, kA.data(), fa.data(), qtB + (long)sB * N, sA, qtA)) return -1;
+  // Q from block 0 head: qHeadInitial (= 1) * qHeadFinal (pspolymer/PsBlock.cpp:205-237)
+  double sum = 0.0;
+  const double* qhf = qtA + (long)sA * N;
+  for (long n = 0; n < N; ++n) sum += 1.0 * qhf[n];
+  const double Q = sum / (double)N;
+  if (Q != Q) return -2;
+  *Qout = Q;
+  // densities: vf = 1, lengthRatio = 1, densWt = 1 (pspolymer/PsBlock.cpp:154-196)
+  orc_quadrature(N, sA, P->ds, qA, qtA, Q, phiA);
+  orc_quadrature(N, sB, P->ds, qB, qtB, Q, phiB);
+  return 0;
+}
+
+extern "C" double orc_free_energy(const orc_params* P, long N,
+                                  const double* wA, const double* wB,
+                                  const double* phiA, const double* phiB,
+                                  double Q) {
+  // pseffhamil/PsPolymerUpdater.cpp:173-226, calcUpdateFieldProd 303-341,
+  // pseffhamil/PsFloryInteraction.cpp:275-321 (calcDisorder = true), bulk.
+  const double fA = P->fA, fB = 1.0 - P->fA;
+  const double vol = (double)N;
+  double prod = 0.0, inter = 0.0;
+  for (long r = 0; r < N; ++r) {
+    prod += (-(phiA[r] * wA[r]) - (phiB[r] * wB[r])) / vol;
+    inter += (P->chiN * phiA[r] * phiB[r] - P->chiN * fA * fB) / vol;
+  }
+  return -(1.0 * std::log(Q)) / 1.0 + (prod + inter);
+}
+
+extern "C" double orc_sd_update(const orc_params* P, long N, double* wA,
+                                double* wB, double* p, const double* phiA,
+                                const double* phiB, double Q) {
+  // pseffhamil/PsSteepDUpdater.cpp:159-203 with update_set2Fields 250-333
+  const double fA = P->fA, fB = 1.0 - P->fA;
+  std::vector<double> dHA(N), dHB(N);
+  for (long r = 0; r < N; ++r) {
+    dHA[r] = p[r] - wA[r] + P->chiN * (phiB[r] - fB);
+    dHB[r] = p[r] - wB[r] + P->chiN * (phiA[r] - fA);
+  }
+  std::vector<double> wA0, wB0;
+  if (P->f_order == 1) { wA0.assign(wA, wA + N); wB0.assign(wB, wB + N); }
+  for (long r = 0; r < N; ++r) {
+    const double xa = (dHA[r] + (-1.0 * P->lambda1) * dHB[r]) * P->lambda0;
+    const double xb = (dHB[r] + (-1.0 * P->lambda1) * dHA[r]) * P->lambda0;
+    wA[r] += xa;
+    wB[r] += xb;
+  }
+  // pressure: p = 0.5 (wA + wB), then minus its mean (psphysf/PsConstraint.cpp:184-262)
+  double sum = 0.0;
+  for (long r = 0; r < N; ++r) { p[r] = (wA[r] + wB[r]) * 0.5; sum += p[r]; }
+  const double avg = sum / (double)N;
+  for (long r = 0; r < N; ++r) p[r] -= avg;
+  if (P->f_order == 1) return orc_free_energy(P, N, wA0.data(), wB0.data(), phiA, phiB, Q);
+  return orc_free_energy(P, N, wA, wB, phiA, phiB, Q);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Anderson mixing (no reference counterpart).  Type-II Anderson acceleration
+ * (Walker & Ni, SIAM J. Numer. Anal. 49 (2011) 1715; the SCFT form of
+ * Matsen, Eur. Phys. J. E 30 (2009) 361):
+ *   U_ij = <f_k - f_{k-i}, f_k - f_{k-j}>,  V_i = <f_k - f_{k-i}, f_k>,
+ *   c = U^{-1} V,  xbar = x_k + sum_i c_i (x_{k-i} - x_k),
+ *   fbar = f_k + sum_i c_i (f_{k-i} - f_k),  x_{k+1} = xbar + beta fbar,
+ * with x = (wA, wB), f = (dH_A, dH_B) and <a,b> = sum_r (a_A b_A + a_B b_B).
+ * Depth 0 (or residual RMS above anderson_start) is the Picard step
+ * x_{k+1} = x_k + lambda0 f_k, identical to steepest descent [lambda0, 0]. */
+/* ------------------------------------------------------------------------ */
+namespace {
+
+// Solve the small dense system A x = b (n <= 32) by Gaussian elimination
+// with partial pivoting; returns false if singular.
+// A pivot at or below 1e-13 times the largest |A_ii| counts as singular.
+bool solve_dense(int n, std::vector<double> A, std::vector<double> b,
+                 std::vector<double>& x) {
+  double dmax = 0.0;
+  for (int i = 0; i < n; ++i) dmax = std::max(dmax, std::fabs(A[i * n + i]));
+  const double thr = 1e-13 * dmax;
+  for (int c = 0; c < n; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < n; ++r)
+      if (std::fabs(A[r * n + c]) > std::fabs(A[piv * n + c])) piv = r;
+    if (!(std::fabs(A[piv * n + c]) > thr)) return false;
+    if (piv != c) {
+      for (int k = 0; k < n; ++k) std::swap(A[c * n + k], A[piv * n + k]);
+      std::swap(b[c], b[piv]);
+    }
+    for (int r = c + 1; r < n; ++r) {
+      double f = A[r * n + c] / A[c * n + c];
+      for (int k = c; k < n; ++k) A[r * n + k] -= f * A[c * n + k];
+      b[r] -= f * b[c];
+    }
+  }
+  x.assign(n, 0.0);
+  for (int r = n - 1; r >= 0; --r) {
+    double s = b[r];
+    for (int k = r + 1; k < n; ++k) s -= A[r * n + k] * x[k];
+    x[r] = s / A[r * n + r];
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" int orc_scft_run(const orc_params* P, double* wA, double* wB,
+                            double* p, int n_iters, double tol,
+                            double* F_hist, double* Q_hist, double* err_hist,
+                            double* phiA_out, double* phiB_out) {
+  const long N = ncells(P);
+  std::vector<double> phiA(N), phiB(N);
+  const double fA = P->fA, fB = 1.0 - P->fA;
+  const int m = P->mix == 1 ? P->depth : 0;
+  // history ring (x, f) of the last m iterates
+  std::vector<std::vector<double>> hx, hf;
+  int done = 0;
+  for (int it = 0; it < n_iters; ++it) {
+    double Q;
+    // pstrol/PsDomain.cpp:332-372: densities reset, propagate, Q, densities
+    int rc = orc_diblock_densities(P, wA, wB, phiA.data(), phiB.data(), &Q, nullptr);
+    if (rc) return -1;
+    // residual f = dH (pseffhamil/PsSteepDUpdater.cpp:174-184, Flory calcDfD)
+    std::vector<double> f(2 * N), x(2 * N);
+    double ff = 0.0;
+    for (long r = 0; r < N; ++r) {
+      f[r] = p[r] - wA[r] + P->chiN * (phiB[r] - fB);
+      f[N + r] = p[r] - wB[r] + P->chiN * (phiA[r] - fA);
+      x[r] = wA[r];
+      x[N + r] = wB[r];
+      ff += f[r] * f[r] + f[N + r] * f[N + r];
+    }
+    const double err = std::sqrt(ff / (double)N);
+    double F;
+    if (P->mix == 0) {
+      F = orc_sd_update(P, N, wA, wB, p, phiA.data(), phiB.data(), Q);
+    } else {
+      std::vector<double> xn(2 * N);
+      const int k = (int)hx.size();
+      const int mk = (err > P->anderson_start) ? 0 : std::min(m, k);
+      std::vector<double> c;
+      bool ok = false;
+      if (mk > 0) {
+        std::vector<double> U(mk * mk), V(mk);
+        std::vector<std::vector<double>> D(mk, std::vector<double>(2 * N));
+        for (int i = 0; i < mk; ++i) {
+          const std::vector<double>& fo = hf[k - 1 - i];
+          for (long r = 0; r < 2 * N; ++r) D[i][r] = f[r] - fo[r];
+        }
+        for (int i = 0; i < mk; ++i) {
+          double v = 0.0;
+          for (long r = 0; r < 2 * N; ++r) v += D[i][r] * f[r];
+          V[i] = v;
+          for (int j = 0; j <= i; ++j) {
+            double u = 0.0;
+            for (long r = 0; r < 2 * N; ++r) u += D[i][r] * D[j][r];
+            U[i * mk + j] = U[j * mk + i] = u;
+          }
+        }
+        ok = solve_dense(mk, U, V, c);
+      }
+      if (ok) {
+        for (long r = 0; r < 2 * N; ++r) {
+          double xb = x[r], fbv = f[r];
+          for (int i = 0; i < mk; ++i) {
+            xb += c[i] * (hx[k - 1 - i][r] - x[r]);
+            fbv += c[i] * (hf[k - 1 - i][r] - f[r]);
+          }
+          xn[r] = xb + P->beta * fbv;
+        }
+      } else {
+        for (long r = 0; r < 2 * N; ++r) xn[r] = x[r] + P->lambda0 * f[r];
+      }
+      if (m > 0) {
+        hx.push_back(x);
+        hf.push_back(f);
+        if ((int)hx.size() > m) { hx.erase(hx.begin()); hf.erase(hf.begin()); }
+      }
+      std::vector<double> wA0(wA, wA + N), wB0(wB, wB + N);
+      double sum = 0.0;
+      for (long r = 0; r < N; ++r) {
+        wA[r] = xn[r];
+        wB[r] = xn[N + r];
+        p[r] = (wA[r] + wB[r]) * 0.5;
+        sum += p[r];
+      }
+      const double avg = sum / (double)N;
+      for (long r = 0; r < N; ++r) p[r] -= avg;
+      if (P->f_order == 1)
+        F = orc_free_energy(P, N, wA0.data(), wB0.data(), phiA.data(), phiB.data(), Q);
+      else
+        F = orc_free_energy(P, N, wA, wB, phiA.data(), phiB.data(), Q);
+    }
+    if (F_hist) F_hist[it] = F;
+    if (Q_hist) Q_hist[it] = Q;
+    if (err_hist) err_hist[it] = err;
+    done = it + 1;
+    if (tol > 0.0 && err < tol) break;
+  }
+  if (phiA_out) std::memcpy(phiA_out, phiA.data(), N * sizeof(double));
+  if (phiB_out) std::memcpy(phiB_out, phiB.data(), N * sizeof(double));
+  return done;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Box stress (no reference counterpart; the reference has no analytic
+ * stress).  With T_t = W_t F^-1 K_t F W_t the step operator of contour step t
+ * and K = exp(-ds b^2 k^2), k_d = 2 pi n_d / L_d, the partition function
+ * N Q = <q+_{Ns-1-t}, T_t q_t> for every t, so
+ *   dQ/dL_d = (1/N) sum_t <q+_{Ns-1-t}, W_t F^-1 (dK/dL_d) F W_t q_t>,
+ *   dK/dL_d = K * ds b^2 * 2 k_d^2 / L_d,
+ * and at fixed fields dF/dL_d = -(1/Q) dQ/dL_d. */
+/* ------------------------------------------------------------------------ */
+extern "C" int orc_box_stress(const orc_params* P, const double* wA,
+                              const double* wB, double* dFdL) {
+  const int nx = P->nx, ny = P->ny, nz = P->nz;
+  const long N = ncells(P);
+  const int sA = orc_block_steps(P->fA, P->ds);
+  const int sB = orc_block_steps(1.0 - P->fA, P->ds);
+  if (sA < 0 || sB < 0) return -1;
+  const long total = (long)(sA + 1) * 2 + (long)(sB + 1) * 2;
+  std::vector<double> S(total * N), phiA(N), phiB(N);
+  double Q;
+  if (orc_diblock_densities(P, wA, wB, phiA.data(), phiB.data(), &Q, S.data())) return -1;
+  const double* qA = S.data();
+  const double* qB = qA + (long)(sA + 1) * N;
+  const double* qtB = qB + (long)(sB + 1) * N;
+  const double* qtA = qtB + (long)(sB + 1) * N;
+  const int Ns = sA + sB;
+  const double twopi = 2.0 * std::acos(-1.0);
+  const int g[3] = {nx, ny, nz};
+  const double h[3] = {P->dx, P->dy, P->dz};
+  // per-mode k_d^2 and the two kfac tables
+  std::vector<double> kd2[3];
+  for (int d = 0; d < 3; ++d) kd2[d].resize(N);
+  std::vector<double> KA(N), KB(N);
+  orc_kfac(nx, ny, nz, P->dx, P->dy, P->dz, P->ds, P->bA, KA.data());
+  orc_kfac(nx, ny, nz, P->dx, P->dy, P->dz, P->ds, P->bB, KB.data());
+  long n = 0;
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j)
+      for (int k = 0; k < nz; ++k, ++n) {
+        const int idx[3] = {i, j, k};
+        for (int d = 0; d < 3; ++d) {
+          double n2 = g[d] / 2.0;
+          double nk = n2 - std::fabs((double)idx[d] - n2);
+          double kv = twopi * nk / (double)g[d] / h[d];
+          kd2[d][n] = kv * kv;
+        }
+      }
+  std::vector<double> wfA(N), wfB(N);
+  for (long r = 0; r < N; ++r) { wfA[r] = std::exp(-0.5 * P->ds * wA[r]); wfB[r] = std::exp(-0.5 * P->ds * wB[r]); }
+  double acc[3] = {0, 0, 0};
+  std::vector<cd> a(N), b(N);
+  for (int t = 0; t < Ns; ++t) {
+    // forward slice q_t along the whole chain, backward slice q+_{Ns-1-t}
+    const bool inA = t < sA;
+    const double* qt = inA ? qA + (long)t * N : qB + (long)(t - sA) * N;
+    const int u = Ns - 1 - t;  // backward steps taken (from the B tail)
+    const double* qb = (u < sB) ? qtB + (long)u * N : qtA + (long)(u - sB) * N;
+    const double* wf = inA ? wfA.data() : wfB.data();
+    const double* K = inA ? KA.data() : KB.data();
+    const double bb = inA ? P->bA * P->bA : P->bB * P->bB;
+    for (long r = 0; r < N; ++r) { a[r] = cd(wf[r] * qt[r], 0.0); b[r] = cd(wf[r] * qb[r], 0.0); }
+    fft3(a.data(), nx, ny, nz, false);
+    fft3(b.data(), nx, ny, nz, false);
+    for (long r = 0; r < N; ++r) {
+      const double s = (std::conj(b[r]) * a[r]).real() * K[r] * P->ds * bb * 2.0;
+      for (int d = 0; d < 3; ++d) acc[d] += s * kd2[d][r];
+    }
+  }
+  for (int d = 0; d < 3; ++d) {
+    const double L = g[d] * h[d];
+    // <x, F^-1 y> = (1/N) <F x, y>, then Q = (N Q) / N
+    const double dQ = acc[d] / L / (double)N / (double)N;
+    dFdL[d] = g[d] > 1 ? -dQ / Q : 0.0;
+  }
+  return 0;
+}
+
+extern "C" double orc_time_steps(int nx, int ny, int nz, int steps) {
+  const long N = (long)nx * ny * nz;
+  std::vector<double> k(N), w(N), q0(N, 1.0), sl((long)(steps + 1) * N);
+  orc_kfac(nx, ny, nz, 0.0625, 0.0625, 0.0625, 0.01, 1.0, k.data());
+  for (long r = 0; r < N; ++r) w[r] = std::exp(-0.005 * std::sin(0.001 * r));
+  auto t0 = std::chrono::steady_clock::now();
+  orc_solve_q(nx, ny, nz, k.data(), w.data(), q0.data(), steps, sl.data());
+  auto t1 = std::chrono::steady_clock::now();
+  return std::chrono::duration<double>(t1 - t0).count();
+}

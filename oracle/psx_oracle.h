@@ -1,0 +1,120 @@
+/*
+ * psx_oracle.h -- CPU oracle for the SCFT hot path (FP64).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP engine
+ * is compared against; only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product path never calls it.
+ *
+ * It restates, in plain single-file C++, the reference algorithm of
+ * PolySwift++ for one bulk block-copolymer SCFT iteration:
+ *   - k^2 table and exp(-ds k^2 b^2)      psbase/PsGridField.cpp:66-104,
+ *                                          pspolymer/PsFlexPseudoSpec.cpp:188-206
+ *   - scaledFFTPair (unnormalised pair)   psfft/PsFFTW.cpp:196-224
+ *   - solveQ split-step propagator        pspolymer/PsFlexPseudoSpec.cpp:244-293
+ *   - w-factor                            pspolymer/PsFlexPseudoSpec.cpp:159-181
+ *   - block graph for a linear diblock    pspolymer/PsBlockCopolymer.cpp:189-313
+ *   - Q                                   pspolymer/PsBlock.cpp:205-237
+ *   - Simpson(0.33333)+trapezoid quadrature pspolymer/PsBlock.cpp:256-380
+ *   - densities                           pspolymer/PsBlock.cpp:154-196
+ *   - block steps                         psbase/PsBlockBase.cpp:252-326
+ *   - steepest-descent field update       pseffhamil/PsSteepDUpdater.cpp:159-333
+ *   - Flory dF/dphi and F terms           pseffhamil/PsFloryInteraction.cpp:204-321
+ *   - pressure shift                      psphysf/PsConstraint.cpp:184-262
+ *   - free energy                         pseffhamil/PsPolymerUpdater.cpp:173-341
+ *   - iteration order                     pstrol/PsDomain.cpp:332-372
+ * plus two components the reference does not have (Anderson mixing and the
+ * analytic box stress), restated from their published definitions.
+ *
+ * The FFT is this file's own recursive mixed-radix Cooley-Tukey transform
+ * (FFTW 2.x is not available); it shares no code with the GPU Stockham FFT.
+ */
+#ifndef PSX_ORACLE_H
+#define PSX_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Run parameters of one AB diblock melt (bulk, no walls, no solvent). */
+typedef struct orc_params {
+  int nx, ny, nz;        /* grid cells; nz = 1 (and ny = 1) for 2D (1D) */
+  double dx, dy, dz;     /* cell sizes in Rg (reference "cellSizes") */
+  double ds;             /* contour step (both blocks) */
+  double fA;             /* lengthfrac of block A; block B has 1 - fA */
+  double chiN;           /* Flory chi times N */
+  double bA, bB;         /* segment-length ratios (reference bSegRatio) */
+  int mix;               /* 0 steepest descent (lambda0, lambda1); 1 Anderson */
+  double lambda0, lambda1;
+  int depth;             /* Anderson history depth m */
+  double beta;           /* Anderson mixing parameter */
+  double anderson_start; /* use Picard while residual RMS > this */
+  int f_order;           /* 0: F after the w update (reference); 1: consistent */
+} orc_params;
+
+/* Thread count used by the FFT line loops and element-wise loops (OpenMP). */
+void orc_set_threads(int n);
+int orc_get_threads(void);
+
+/* Block steps per psbase/PsBlockBase.cpp:252-326; returns -1 on error. */
+int orc_block_steps(double length_frac, double ds);
+
+/* kfac[i,j,k] = exp(-ds*b^2*k2), full grid, C order (k fastest). */
+void orc_kfac(int nx, int ny, int nz, double dx, double dy, double dz,
+              double ds, double b, double* kfac);
+
+/* In-place unnormalised 3D complex DFT; sign -1 forward, +1 backward. */
+int orc_fft3(int nx, int ny, int nz, double* re, double* im, int sign);
+/* O(N^2) direct 3D DFT (exact-index twiddles), for tiny grids only. */
+int orc_dft3_brute(int nx, int ny, int nz, const double* re, const double* im,
+                   double* ore, double* oim, int sign);
+
+/* res = Re(IFFT_u(kdata .* FFT_u(data + 0i))), no 1/N (psfft/PsFFTW.cpp:196-224). */
+int orc_scaled_fft_pair(int nx, int ny, int nz, const double* data,
+                        const double* kdata, double* res);
+
+/* One block sweep: slices[0] = q0, slices[s] = split step of slices[s-1],
+ * s = 1..steps, with wfac = exp(-ds w/2) (pspolymer/PsFlexPseudoSpec.cpp:244-293). */
+int orc_solve_q(int nx, int ny, int nz, const double* kfac, const double* wfac,
+                const double* q0, int steps, double* slices);
+
+/* qqt = (1/Q) * sum_j c_j qs[j] qts[M-1-j], Simpson 0.33333 + trapezoid tail
+ * (pspolymer/PsBlock.cpp:256-380).  qs, qts hold M = steps+1 slices. */
+void orc_quadrature(long n, int steps, double ds, const double* qs,
+                    const double* qts, double Q, double* out);
+
+/* Full propagation of a diblock: fills densities and Q.  If slices != NULL it
+ * receives qA[0..sA], qB[0..sB], qtB[0..sB], qtA[0..sA] in that order. */
+int orc_diblock_densities(const orc_params* P, const double* wA,
+                          const double* wB, double* phiA, double* phiB,
+                          double* Q, double* slices);
+
+/* One steepest-descent update + pressure shift + F, in reference order
+ * (pseffhamil/PsSteepDUpdater.cpp:159-203).  Updates wA, wB, p in place. */
+double orc_sd_update(const orc_params* P, long n, double* wA, double* wB,
+                     double* p, const double* phiA, const double* phiB,
+                     double Q);
+
+/* Free energy F for given fields (pseffhamil/PsPolymerUpdater.cpp:173-226). */
+double orc_free_energy(const orc_params* P, long n, const double* wA,
+                       const double* wB, const double* phiA,
+                       const double* phiB, double Q);
+
+/* Run SCFT iterations until n_iters or residual RMS < tol (tol <= 0: never).
+ * Histories (length n_iters, may be NULL): F, Q, residual RMS.
+ * Returns the number of iterations done, or -1 on error. */
+int orc_scft_run(const orc_params* P, double* wA, double* wB, double* p,
+                 int n_iters, double tol, double* F_hist, double* Q_hist,
+                 double* err_hist, double* phiA, double* phiB);
+
+/* Analytic box stress dF/dL_d (d = x,y,z) at fixed w for the diblock: the
+ * derivative of -ln Q through k^2 only (fields held fixed in cell units). */
+int orc_box_stress(const orc_params* P, const double* wA, const double* wB,
+                   double* dFdL);
+
+/* Time `steps` faithful contour steps on an nx*ny*nz grid (seconds). */
+double orc_time_steps(int nx, int ny, int nz, int steps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,551 @@
+// psx_kernels.hpp -- CDNA4 kernels of the SCFT iteration.
+//
+// Layout: every field is a C-order nx*ny*nz array (z fastest), one element
+// per cell, FP64 or FP32 (T).  A "line" along axis d is the n_d elements
+// with all other indices fixed; with inner = prod(n_{d+1..}) its element k
+// sits at ((l / inner) * n_d + k) * inner + l % inner for line number l.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "psx_fft_lds.hpp"
+
+namespace psx {
+
+constexpr int kThreads = 256;
+constexpr int kMaxDepth = 32;
+
+// ------------------------------------------------------------------------
+// Geometry of one line pass.
+struct LineGeom {
+  FftPlanDev pl;      // n and radices
+  int inner;          // element stride of a line
+  int nlines;         // N / n
+  int L;              // real lines per tile (even) / complex lines per tile
+  int ls;             // LDS line stride in complex words (n + 1)
+  const void* tw;     // n twiddles exp(-2 pi i j/n), type Cx<T>
+};
+
+// One field of a pair round-trip pass.
+struct PairIO {
+  const void* in;     // T[N]
+  void* out;          // T[N]  (may equal in)
+  const void* pre;    // T[N] multiplied into the input, or null
+  const void* post;   // T[N] multiplied into the output, or null
+  const void* emul;   // T[n] spectral multiplier (includes 1/n)
+};
+
+__device__ __forceinline__ int line_addr(int l, int k, int n, int inner) {
+  return ((l / inner) * n + k) * inner + (l - (l / inner) * inner);
+}
+
+// Real round trip along one axis for a tile of L real lines:
+//   out = post * IFFT_n( emul * FFT_n( pre * in ) )
+// Two real lines share one complex line (line 2p -> real part, 2p+1 ->
+// imaginary part); the operator is real and even in k, so the two results
+// come back in the real and imaginary parts unchanged.  blockIdx.y selects
+// one of two independent fields (the two contour sweeps run in one launch).
+template <typename T, bool CONTIG>
+__global__ void __launch_bounds__(kThreads)
+k_pair_roundtrip(LineGeom g, PairIO f0, PairIO f1) {
+  typedef typename Cx<T>::type C;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const bool second = blockIdx.y != 0;
+  const T* __restrict__ in = (const T*)(second ? f1.in : f0.in);
+  T* __restrict__ out = (T*)(second ? f1.out : f0.out);
+  const T* __restrict__ pre = (const T*)(second ? f1.pre : f0.pre);
+  const T* __restrict__ post = (const T*)(second ? f1.post : f0.post);
+  const T* __restrict__ emul = (const T*)(second ? f1.emul : f0.emul);
+  const C* __restrict__ tw = (const C*)g.tw;
+
+  const int n = g.pl.n, L = g.L, P = L >> 1, ls = g.ls;
+  C* A = (C*)smem_raw;
+  C* B = A + P * ls;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int l0 = blockIdx.x * L;
+  const int tot = L * n;
+
+  T* Ar = (T*)A;
+  for (int e = tid; e < tot; e += nt) {
+    int li, k;
+    if (CONTIG) { li = e / n; k = e - li * n; }
+    else { k = e / L; li = e - k * L; }
+    const int l = l0 + li;
+    T v = (T)0;
+    if (l < g.nlines) {
+      const int addr = CONTIG ? l * n + k : line_addr(l, k, n, g.inner);
+      v = in[addr];
+      if (pre) v *= pre[addr];
+    }
+    Ar[2 * ((li >> 1) * ls + k) + (li & 1)] = v;
+  }
+  __syncthreads();
+
+  C* res = fft_lines<-1, T>(A, B, g.pl, P, ls, tw, tid, nt);
+  for (int e = tid; e < P * n; e += nt) {
+    const int line = e / n, k = e - line * n;
+    C v = res[line * ls + k];
+    const T m = emul[k];
+    v.x *= m; v.y *= m;
+    res[line * ls + k] = v;
+  }
+  __syncthreads();
+  C* other = (res == A) ? B : A;
+  res = fft_lines<+1, T>(res, other, g.pl, P, ls, tw, tid, nt);
+
+  const T* Rr = (const T*)res;
+  for (int e = tid; e < tot; e += nt) {
+    int li, k;
+    if (CONTIG) { li = e / n; k = e - li * n; }
+    else { k = e / L; li = e - k * L; }
+    const int l = l0 + li;
+    if (l < g.nlines) {
+      const int addr = CONTIG ? l * n + k : line_addr(l, k, n, g.inner);
+      T v = Rr[2 * ((li >> 1) * ls + k) + (li & 1)];
+      if (post) v *= post[addr];
+      out[addr] = v;
+    }
+  }
+}
+
+// Complex line pass of the general FFT pair (any real kdata, full grid).
+struct CplxIO {
+  const void* in;     // complex (interleaved) T[2N], or real T[N] if in_real
+  void* out;          // complex T[2N], or real part only T[N] if out_real
+  const void* kmul;   // T[N] full-grid multiplier between fwd and inv, or null
+  int in_real, out_real, do_fwd, do_inv;
+};
+
+template <typename T, bool CONTIG>
+__global__ void __launch_bounds__(kThreads)
+k_cplx_lines(LineGeom g, CplxIO f) {
+  typedef typename Cx<T>::type C;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const C* __restrict__ tw = (const C*)g.tw;
+  const int n = g.pl.n, P = g.L, ls = g.ls;
+  C* A = (C*)smem_raw;
+  C* B = A + P * ls;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int l0 = blockIdx.x * P;
+  const int tot = P * n;
+  for (int e = tid; e < tot; e += nt) {
+    int li, k;
+    if (CONTIG) { li = e / n; k = e - li * n; }
+    else { k = e / P; li = e - k * P; }
+    const int l = l0 + li;
+    C v; v.x = (T)0; v.y = (T)0;
+    if (l < g.nlines) {
+      const int addr = CONTIG ? l * n + k : line_addr(l, k, n, g.inner);
+      if (f.in_real) v.x = ((const T*)f.in)[addr];
+      else v = ((const C*)f.in)[addr];
+    }
+    A[li * ls + k] = v;
+  }
+  __syncthreads();
+  C* res = A;
+  C* other = B;
+  if (f.do_fwd) {
+    res = fft_lines<-1, T>(A, B, g.pl, P, ls, tw, tid, nt);
+    other = (res == A) ? B : A;
+  }
+  if (f.kmul) {
+    const T* km = (const T*)f.kmul;
+    for (int e = tid; e < tot; e += nt) {
+      const int li = e / n, k = e - li * n;
+      const int l = l0 + li;
+      if (l < g.nlines) {
+        const int addr = CONTIG ? l * n + k : line_addr(l, k, n, g.inner);
+        const T m = km[addr];
+        C v = res[li * ls + k];
+        v.x *= m; v.y *= m;
+        res[li * ls + k] = v;
+      }
+    }
+    __syncthreads();
+  }
+  if (f.do_inv) {
+    res = fft_lines<+1, T>(res, other, g.pl, P, ls, tw, tid, nt);
+  }
+  for (int e = tid; e < tot; e += nt) {
+    int li, k;
+    if (CONTIG) { li = e / n; k = e - li * n; }
+    else { k = e / P; li = e - k * P; }
+    const int l = l0 + li;
+    if (l < g.nlines) {
+      const int addr = CONTIG ? l * n + k : line_addr(l, k, n, g.inner);
+      const C v = res[li * ls + k];
+      if (f.out_real) ((T*)f.out)[addr] = v.x;
+      else ((C*)f.out)[addr] = v;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// Device-resident iteration state (one per engine).
+struct DevState {
+  double Q;           // partition function of the last sweep
+  double pbar;        // mean of p_raw; the pressure is p = p_raw - pbar
+  double err;         // residual RMS of the current iteration
+  double F;           // free energy of the last completed iteration
+  long long iter;     // iterations since creation (Anderson ring index)
+  long long since_reset;  // iterations since the last history reset
+  long long call_iter;    // iterations in the current iterate() call
+  int nanflag;
+  int mk;             // Anderson depth used in the last mixing step
+  double beta_used;
+  double coef[kMaxDepth];
+  double gram[(kMaxDepth + 1) * (kMaxDepth + 1)];
+};
+
+// Block-wide sum of NV doubles; the result is valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV]) {
+  __shared__ double sh[NV][kThreads / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double x = v[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    v[i] = x;
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) sh[i][wid] = v[i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      double s = 0.0;
+      for (int w = 0; w < nw; ++w) s += sh[i][w];
+      v[i] = s;
+    }
+  }
+}
+
+// part[block] = sum of x over a grid-stride range, in FP64.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_partial_sum(const T* __restrict__ x, long long N, double* __restrict__ part) {
+  double v[1] = {0.0};
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long long)gridDim.x * blockDim.x)
+    v[0] += (double)x[i];
+  block_sum<1>(v);
+  if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+// Q = sum(part) / N  (PsBlock::calcBigQ with qHeadInitial = 1).
+__global__ void k_finish_Q(const double* __restrict__ part, int nb, double invN,
+                           DevState* st) {
+  double v[1] = {0.0};
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) v[0] += part[i];
+  block_sum<1>(v);
+  if (threadIdx.x == 0) {
+    const double Q = v[0] * invN;
+    st->Q = Q;
+    if (!(Q == Q) || Q == 1.0 / 0.0 || Q == -1.0 / 0.0) st->nanflag = 1;
+  }
+}
+
+template <typename T>
+__global__ void k_wfac(const double* __restrict__ w, long long N, double halfds,
+                       T* __restrict__ out) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long long)gridDim.x * blockDim.x)
+    out[i] = (T)exp(-halfds * w[i]);
+}
+
+template <typename T>
+__global__ void k_fill(T* __restrict__ x, long long N, T v) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long long)gridDim.x * blockDim.x)
+    x[i] = v;
+}
+
+template <typename T>
+__global__ void k_to_f64(const T* __restrict__ x, long long N, double* __restrict__ y) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long long)gridDim.x * blockDim.x)
+    y[i] = (double)x[i];
+}
+
+// p = p_raw - pbar
+__global__ void k_pressure(const double* __restrict__ praw, const DevState* st,
+                           long long N, double* __restrict__ p) {
+  const double pb = st->pbar;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = praw[i] - pb;
+}
+
+// ------------------------------------------------------------------------
+// Contour quadrature, densities and the field update, one pass over cells.
+enum QuadMode {
+  QM_DENS = 0,      // phi only
+  QM_SD = 1,        // phi + steepest-descent update + w-factors + partials
+  QM_RESID = 2,     // phi + residual into the Anderson ring + err partial
+  QM_SD_PHI = 3,    // steepest-descent update from stored phi
+  QM_FE = 4         // free-energy partials of stored phi and current w
+};
+constexpr int kNPart = 4;   // sum p_raw, sum(-phi w), sum(chiN phiA phiB), sum dH^2
+
+struct QuadArgs {
+  const void* F;       // forward slices, (Ns+1)*N of T
+  const void* G;       // backward slices, (Ns+1)*N of T
+  long long N;
+  int Ns, sA;
+  const double* cw;    // quadrature weights: A at [0..sA], B at [sA+1..Ns+1]
+  double* phiA; double* phiB;
+  double* wA; double* wB; double* praw;
+  void* wfA; void* wfB;          // T
+  double* ringx; double* ringf;  // Anderson ring: slot s at s*2N (A then B)
+  int m1;                        // ring slots (depth + 1)
+  double chiN, fA, fB, lam0, lam1, halfds, dens;
+  DevState* st;
+  double* part;                  // [gridDim.x][kNPart]
+  int mode, f_order;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_quad_update(QuadArgs a) {
+  const T* __restrict__ F = (const T*)a.F;
+  const T* __restrict__ G = (const T*)a.G;
+  const long long N = a.N;
+  const int Ns = a.Ns, sA = a.sA;
+  const double* __restrict__ cw = a.cw;
+  const double invQ = 1.0 / a.st->Q;
+  const double pbar = a.st->pbar;
+  const long long slot = (a.mode == QM_RESID) ? (a.st->iter % a.m1) : 0;
+  double acc[kNPart] = {0.0, 0.0, 0.0, 0.0};
+  for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < N;
+       r += (long long)gridDim.x * blockDim.x) {
+    double phA, phB;
+    if (a.mode == QM_SD_PHI || a.mode == QM_FE) {
+      phA = a.phiA[r];
+      phB = a.phiB[r];
+    } else {
+      double qa = 0.0, qb = 0.0;
+      for (int t = 0; t < sA; ++t)
+        qa += cw[t] * ((double)F[(long long)t * N + r] * (double)G[(long long)(Ns - t) * N + r]);
+      {
+        const double j = (double)F[(long long)sA * N + r] * (double)G[(long long)(Ns - sA) * N + r];
+        qa += cw[sA] * j;
+        qb += cw[sA + 1] * j;
+      }
+      for (int t = sA + 1; t <= Ns; ++t)
+        qb += cw[t + 1] * ((double)F[(long long)t * N + r] * (double)G[(long long)(Ns - t) * N + r]);
+      phA = qa * invQ * a.dens;
+      phB = qb * invQ * a.dens;
+      a.phiA[r] = phA;
+      a.phiB[r] = phB;
+    }
+    if (a.mode == QM_DENS) continue;
+    const double wA = a.wA[r], wB = a.wB[r];
+    if (a.mode == QM_FE) {
+      acc[1] += -(phA * wA) - (phB * wB);
+      acc[2] += a.chiN * phA * phB;
+      continue;
+    }
+    const double p = a.praw[r] - pbar;
+    const double dHA = p - wA + a.chiN * (phB - a.fB);
+    const double dHB = p - wB + a.chiN * (phA - a.fA);
+    acc[3] += dHA * dHA + dHB * dHB;
+    if (a.mode == QM_RESID) {
+      double* fx = a.ringx + slot * 2 * N;
+      double* ff = a.ringf + slot * 2 * N;
+      fx[r] = wA; fx[N + r] = wB;
+      ff[r] = dHA; ff[N + r] = dHB;
+      continue;
+    }
+    const double nA = wA + (dHA + (-a.lam1) * dHB) * a.lam0;
+    const double nB = wB + (dHB + (-a.lam1) * dHA) * a.lam0;
+    const double pr = (nA + nB) * 0.5;
+    a.wA[r] = nA; a.wB[r] = nB; a.praw[r] = pr;
+    ((T*)a.wfA)[r] = (T)exp(-a.halfds * nA);
+    ((T*)a.wfB)[r] = (T)exp(-a.halfds * nB);
+    acc[0] += pr;
+    if (a.f_order == 0) acc[1] += -(phA * nA) - (phB * nB);
+    else acc[1] += -(phA * wA) - (phB * wB);
+    acc[2] += a.chiN * phA * phB;
+  }
+  block_sum<kNPart>(acc);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < kNPart; ++i) a.part[blockIdx.x * kNPart + i] = acc[i];
+  }
+}
+
+// Reduce the partials of k_quad_update / k_mix and close the iteration.
+enum FinishMode { FIN_ITER_SD = 0, FIN_ERR = 1, FIN_FE = 2, FIN_ITER_MIX = 3 };
+__global__ void k_finish(const double* __restrict__ part, int nb, double invN,
+                         double chiN, double fA, double fB, int mode,
+                         DevState* st, double* hist) {
+  double v[kNPart] = {0.0, 0.0, 0.0, 0.0};
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+#pragma unroll
+    for (int j = 0; j < kNPart; ++j) v[j] += part[i * kNPart + j];
+  }
+  block_sum<kNPart>(v);
+  if (threadIdx.x != 0) return;
+  if (mode == FIN_ERR) { st->err = sqrt(v[3] * invN); return; }
+  const double F = -log(st->Q) + (v[1] * invN + v[2] * invN) - chiN * fA * fB;
+  st->F = F;
+  if (mode == FIN_FE) return;
+  if (mode == FIN_ITER_SD) st->err = sqrt(v[3] * invN);
+  st->pbar = v[0] * invN;
+  if (hist) {
+    double* h = hist + 3 * st->call_iter;
+    h[0] = F; h[1] = st->Q; h[2] = st->err;
+  }
+  st->iter += 1;
+  st->since_reset += 1;
+  st->call_iter += 1;
+}
+
+// Anderson: per-block partial dots <f_cur, f_slot> for all ring slots.
+__global__ void __launch_bounds__(kThreads)
+k_gram(const double* __restrict__ ringf, long long N, int m1, const DevState* st,
+       double* __restrict__ part) {
+  const long long cur = st->iter % m1;
+  const double* fc = ringf + cur * 2 * N;
+  double acc[kMaxDepth + 1];
+  for (int j = 0; j < m1; ++j) acc[j] = 0.0;
+  for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < 2 * N;
+       r += (long long)gridDim.x * blockDim.x) {
+    const double x = fc[r];
+    for (int j = 0; j < m1; ++j) acc[j] += x * ringf[(long long)j * 2 * N + r];
+  }
+  // reduce each slot's partial over the block
+  __shared__ double sh[kThreads / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int j = 0; j < m1; ++j) {
+    double x = acc[j];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if (lane == 0) sh[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double s = 0.0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sh[w];
+      part[(long long)blockIdx.x * m1 + j] = s;
+    }
+    __syncthreads();
+  }
+}
+
+// Anderson: finish the Gram row, then solve U c = V for the mixing
+// coefficients (one block; the solve itself is one thread, m <= 32).
+__global__ void k_anderson_solve(const double* __restrict__ part, int nb, int m,
+                                 double beta, double lam0, double start,
+                                 DevState* st) {
+  const int m1 = m + 1;
+  const long long cur = st->iter % m1;
+  for (int j = threadIdx.x; j < m1; j += blockDim.x) {
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += part[(long long)b * m1 + j];
+    st->gram[cur * m1 + j] = s;
+    st->gram[j * m1 + cur] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  long long avail = st->since_reset < m ? st->since_reset : m;
+  int mk = (int)avail;
+  if (st->err > start) mk = 0;
+  double U[kMaxDepth * kMaxDepth], V[kMaxDepth];
+  int slot[kMaxDepth];
+  const double* Gm = st->gram;
+  const double gcc = Gm[cur * m1 + cur];
+  for (int i = 0; i < mk; ++i) slot[i] = (int)((cur - 1 - i + m1) % m1);
+  for (int i = 0; i < mk; ++i) {
+    const int si = slot[i];
+    V[i] = gcc - Gm[si * m1 + cur];
+    for (int j = 0; j < mk; ++j) {
+      const int sj = slot[j];
+      U[i * mk + j] = gcc - Gm[cur * m1 + sj] - Gm[si * m1 + cur] + Gm[si * m1 + sj];
+    }
+  }
+  bool ok = mk > 0;
+  if (ok) {
+    double dmax = 0.0;
+    for (int i = 0; i < mk; ++i) dmax = fmax(dmax, fabs(U[i * mk + i]));
+    const double thr = 1e-13 * dmax;
+    for (int c = 0; c < mk && ok; ++c) {
+      int piv = c;
+      for (int r = c + 1; r < mk; ++r)
+        if (fabs(U[r * mk + c]) > fabs(U[piv * mk + c])) piv = r;
+      if (!(fabs(U[piv * mk + c]) > thr)) { ok = false; break; }
+      if (piv != c) {
+        for (int k = 0; k < mk; ++k) { double t = U[c * mk + k]; U[c * mk + k] = U[piv * mk + k]; U[piv * mk + k] = t; }
+        double t = V[c]; V[c] = V[piv]; V[piv] = t;
+      }
+      for (int r = c + 1; r < mk; ++r) {
+        const double f = U[r * mk + c] / U[c * mk + c];
+        for (int k = c; k < mk; ++k) U[r * mk + k] -= f * U[c * mk + k];
+        V[r] -= f * V[c];
+      }
+    }
+    if (ok) {
+      for (int r = mk - 1; r >= 0; --r) {
+        double s = V[r];
+        for (int k = r + 1; k < mk; ++k) s -= U[r * mk + k] * st->coef[k];
+        st->coef[r] = s / U[r * mk + r];
+      }
+    }
+  }
+  if (ok) { st->mk = mk; st->beta_used = beta; }
+  else { st->mk = 0; st->beta_used = lam0; }
+}
+
+// Anderson: x_new = x + sum c_i (x_i - x) + beta (f + sum c_i (f_i - f)),
+// then p_raw, w-factors and the free-energy partials.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_mix(QuadArgs a) {
+  const long long N = a.N;
+  const int m1 = a.m1;
+  const DevState* st = a.st;
+  const long long cur = st->iter % m1;
+  const int mk = st->mk;
+  const double beta = st->beta_used;
+  double c[kMaxDepth];
+  long long sl[kMaxDepth];
+  for (int i = 0; i < mk; ++i) { c[i] = st->coef[i]; sl[i] = (cur - 1 - i + m1) % m1; }
+  const double* fc = a.ringf + cur * 2 * N;
+  double acc[kNPart] = {0.0, 0.0, 0.0, 0.0};
+  for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < N;
+       r += (long long)gridDim.x * blockDim.x) {
+    double nw[2], old[2];
+    for (int h = 0; h < 2; ++h) {
+      const long long o = h * N + r;
+      const double x = h == 0 ? a.wA[r] : a.wB[r];
+      const double f = fc[o];
+      double xb = x, fb = f;
+      for (int i = 0; i < mk; ++i) {
+        xb += c[i] * (a.ringx[sl[i] * 2 * N + o] - x);
+        fb += c[i] * (a.ringf[sl[i] * 2 * N + o] - f);
+      }
+      old[h] = x;
+      nw[h] = xb + beta * fb;
+    }
+    const double pr = (nw[0] + nw[1]) * 0.5;
+    a.wA[r] = nw[0]; a.wB[r] = nw[1]; a.praw[r] = pr;
+    ((T*)a.wfA)[r] = (T)exp(-a.halfds * nw[0]);
+    ((T*)a.wfB)[r] = (T)exp(-a.halfds * nw[1]);
+    const double phA = a.phiA[r], phB = a.phiB[r];
+    acc[0] += pr;
+    if (a.f_order == 0) acc[1] += -(phA * nw[0]) - (phB * nw[1]);
+    else acc[1] += -(phA * old[0]) - (phB * old[1]);
+    acc[2] += a.chiN * phA * phB;
+  }
+  block_sum<kNPart>(acc);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < kNPart; ++i) a.part[blockIdx.x * kNPart + i] = acc[i];
+  }
+}
+
+__global__ void k_state_set_call(DevState* st) { st->call_iter = 0; }
+__global__ void k_state_reset_hist(DevState* st) { st->since_reset = 0; }
+
+}  // namespace psx

@@ -1,0 +1,1042 @@
+// psx_scft.hip -- host side of the C ABI declared in include/psx_scft.h.
+//
+// One engine = one bulk AB diblock on one HIP device and one stream.  All
+// fields stay resident in HBM; a contour step is one launch per active grid
+// axis, each a batched round-trip line FFT (psx_kernels.hpp) with the
+// exp(-ds w / 2) multiplies fused into the first and last launch.  The
+// forward (q) and backward (q+) sweeps advance together in the same launches.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/psx_scft.h"
+#include "psx_kernels.hpp"
+
+using namespace psx;
+
+namespace {
+
+thread_local std::string g_err;
+
+psx_status fail(psx_status code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define PSX_HIP(call)                                                         \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      return fail(PSX_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define PSX_TRY(call)                                                         \
+  do {                                                                        \
+    psx_status s_ = (call);                                                   \
+    if (s_ != PSX_OK) return s_;                                              \
+  } while (0)
+
+size_t prec_bytes(int prec) { return prec == PSX_FP64 ? 8 : 4; }
+
+// Radix plan: 8s first, then one 4 and one 2, then odd radices <= 13.
+bool radix_plan(int n, FftPlanDev* pl) {
+  pl->n = n;
+  pl->nst = 0;
+  int m = n;
+  auto push = [&](int r) {
+    if (pl->nst >= 8) return false;
+    pl->rad[pl->nst++] = r;
+    m /= r;
+    return true;
+  };
+  while (m % 8 == 0) if (!push(8)) return false;
+  if (m % 4 == 0) if (!push(4)) return false;
+  if (m % 2 == 0) if (!push(2)) return false;
+  for (int p : {3, 5, 7, 11, 13})
+    while (m % p == 0) if (!push(p)) return false;
+  return m == 1;
+}
+
+// Twiddles exp(-2 pi i j / n) in long double, rounded once.
+std::vector<double> twiddles(int n) {
+  std::vector<double> t(2 * (size_t)n);
+  const long double twopi = 6.283185307179586476925286766559L;
+  for (int j = 0; j < n; ++j) {
+    long double a = twopi * (long double)j / (long double)n;
+    t[2 * j] = (double)cosl(a);
+    t[2 * j + 1] = (double)-sinl(a);
+  }
+  return t;
+}
+
+template <typename T>
+std::vector<T> cast_vec(const std::vector<double>& v) {
+  return std::vector<T>(v.begin(), v.end());
+}
+
+psx_status upload(void** dptr, const void* h, size_t bytes) {
+  if (hipMalloc(dptr, bytes) != hipSuccess) return fail(PSX_ERR_NOMEM, "hipMalloc failed");
+  PSX_HIP(hipMemcpy(*dptr, h, bytes, hipMemcpyHostToDevice));
+  return PSX_OK;
+}
+
+psx_status dalloc(void** dptr, size_t bytes) {
+  *dptr = nullptr;
+  if (bytes == 0) bytes = 16;
+  if (hipMalloc(dptr, bytes) != hipSuccess) {
+    *dptr = nullptr;
+    return fail(PSX_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+  }
+  return PSX_OK;
+}
+
+void dfree(void*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+// One line pass along one grid axis.
+struct Axis {
+  int axis = 2;       // 0 = x, 1 = y, 2 = z
+  int n = 1;
+  int inner = 1;
+  int nlines = 1;
+  bool contig = true;
+  FftPlanDev pl{};
+  void* tw[2] = {nullptr, nullptr};  // per precision
+};
+
+psx_status make_axes(const int dims[3], std::vector<Axis>* axes) {
+  axes->clear();
+  const long long N = (long long)dims[0] * dims[1] * dims[2];
+  for (int d = 2; d >= 0; --d) {
+    if (dims[d] == 1) continue;
+    Axis a;
+    a.axis = d;
+    a.n = dims[d];
+    long long inner = 1;
+    for (int e = d + 1; e < 3; ++e) inner *= dims[e];
+    a.inner = (int)inner;
+    a.nlines = (int)(N / dims[d]);
+    a.contig = inner == 1;
+    if (!radix_plan(a.n, &a.pl))
+      return fail(PSX_ERR_SHAPE, "grid size " + std::to_string(a.n) +
+                                     " has a prime factor above 13 (unsupported)");
+    axes->push_back(a);
+  }
+  if (axes->empty()) {  // 1x1x1: one identity pass of length 1
+    Axis a;
+    a.axis = 2; a.n = 1; a.inner = 1; a.nlines = 1; a.contig = true;
+    radix_plan(1, &a.pl);
+    axes->push_back(a);
+  }
+  return PSX_OK;
+}
+
+psx_status axis_tw(Axis& a, int prec) {
+  if (a.tw[prec]) return PSX_OK;
+  std::vector<double> t = twiddles(a.n);
+  if (prec == PSX_FP64) return upload(&a.tw[prec], t.data(), t.size() * sizeof(double));
+  std::vector<float> f = cast_vec<float>(t);
+  return upload(&a.tw[prec], f.data(), f.size() * sizeof(float));
+}
+
+// Tile: real lines per block of the pair kernel (even).
+int pair_tile(const Axis& a, int prec) {
+  const int target = prec == PSX_FP64 ? 2048 : 4096;
+  int L = target / a.n;
+  if (L < 2) L = 2;
+  if (L > 64) L = 64;
+  L &= ~1;
+  int cap = a.nlines + (a.nlines & 1);
+  if (L > cap) L = cap < 2 ? 2 : cap;
+  return L;
+}
+
+int cplx_tile(const Axis& a) {
+  int P = 1024 / a.n;
+  if (P < 1) P = 1;
+  if (P > 32) P = 32;
+  if (P > a.nlines) P = a.nlines;
+  return P;
+}
+
+LineGeom geom(const Axis& a, int prec, int L) {
+  LineGeom g;
+  g.pl = a.pl;
+  g.inner = a.inner;
+  g.nlines = a.nlines;
+  g.L = L;
+  g.ls = a.n + 1;
+  g.tw = a.tw[prec];
+  return g;
+}
+
+template <typename T>
+psx_status launch_pair(const Axis& a, int prec, const PairIO& f0, const PairIO& f1,
+                       int nfields, hipStream_t s) {
+  const int L = pair_tile(a, prec);
+  LineGeom g = geom(a, prec, L);
+  const int P = L / 2;
+  const size_t smem = 2 * (size_t)P * g.ls * 2 * sizeof(T);
+  dim3 grid((a.nlines + L - 1) / L, nfields);
+  if (a.contig)
+    hipLaunchKernelGGL((k_pair_roundtrip<T, true>), grid, dim3(kThreads), smem, s, g, f0, f1);
+  else
+    hipLaunchKernelGGL((k_pair_roundtrip<T, false>), grid, dim3(kThreads), smem, s, g, f0, f1);
+  PSX_HIP(hipGetLastError());
+  return PSX_OK;
+}
+
+template <typename T>
+psx_status launch_cplx(const Axis& a, int prec, const CplxIO& f, hipStream_t s) {
+  const int P = cplx_tile(a);
+  LineGeom g = geom(a, prec, P);
+  const size_t smem = 2 * (size_t)P * g.ls * 2 * sizeof(T);
+  dim3 grid((a.nlines + P - 1) / P);
+  if (a.contig)
+    hipLaunchKernelGGL((k_cplx_lines<T, true>), grid, dim3(kThreads), smem, s, g, f);
+  else
+    hipLaunchKernelGGL((k_cplx_lines<T, false>), grid, dim3(kThreads), smem, s, g, f);
+  PSX_HIP(hipGetLastError());
+  return PSX_OK;
+}
+
+int grid_1d(long long N) {
+  long long b = (N + kThreads - 1) / kThreads;
+  if (b > 2048) b = 2048;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+psx_status make_stream(void* user, hipStream_t* s, bool* own) {
+  if (user) { *s = (hipStream_t)user; *own = false; return PSX_OK; }
+  PSX_HIP(hipStreamCreateWithFlags(s, hipStreamNonBlocking));
+  *own = true;
+  return PSX_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// FFT pair plugin
+// ===========================================================================
+struct psx_fft {
+  int dims[3];
+  long long N;
+  int prec;
+  hipStream_t stream;
+  bool own_stream;
+  std::vector<Axis> axes;
+  void* work = nullptr;   // N complex of T
+  void* hbuf[3] = {nullptr, nullptr, nullptr};  // device staging for the host API
+};
+
+extern "C" void psx_last_error(char* buf, size_t len) {
+  if (!buf || len == 0) return;
+  std::snprintf(buf, len, "%s", g_err.c_str());
+}
+
+extern "C" const char* psx_version(void) { return "psx-scft 0.1 (gfx950)"; }
+
+extern "C" psx_status psx_fft_plan_create(int nx, int ny, int nz, psx_prec prec,
+                                          void* stream, psx_fft** out) {
+  if (!out) return fail(PSX_ERR_INVALID, "null output handle");
+  *out = nullptr;
+  if (nx < 1 || ny < 1 || nz < 1) return fail(PSX_ERR_SHAPE, "grid dims must be >= 1");
+  if (prec != PSX_FP64 && prec != PSX_FP32) return fail(PSX_ERR_INVALID, "bad precision");
+  if ((long long)nx * ny * nz > (1LL << 30)) return fail(PSX_ERR_SHAPE, "grid too large");
+  psx_fft* p = new psx_fft();
+  p->dims[0] = nx; p->dims[1] = ny; p->dims[2] = nz;
+  p->N = (long long)nx * ny * nz;
+  p->prec = prec;
+  psx_status st = make_axes(p->dims, &p->axes);
+  if (st == PSX_OK) {
+    for (auto& a : p->axes) {
+      st = axis_tw(a, prec);
+      if (st != PSX_OK) break;
+    }
+  }
+  if (st == PSX_OK) st = dalloc(&p->work, 2 * p->N * prec_bytes(prec));
+  if (st == PSX_OK) st = make_stream(stream, &p->stream, &p->own_stream);
+  if (st != PSX_OK) { psx_fft_plan_destroy(p); return st; }
+  *out = p;
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_fft_plan_destroy(psx_fft* p) {
+  if (!p) return PSX_OK;
+  for (auto& a : p->axes) { dfree(a.tw[0]); dfree(a.tw[1]); }
+  dfree(p->work);
+  for (auto& h : p->hbuf) dfree(h);
+  if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
+  delete p;
+  return PSX_OK;
+}
+
+extern "C" size_t psx_fft_size(const psx_fft* p) { return p ? (size_t)p->N : 0; }
+
+template <typename T>
+static psx_status fft_pair_impl(psx_fft* p, const void* data, const void* kdata, void* res) {
+  const int na = (int)p->axes.size();
+  // forward along z, y (in order), then x with the k multiply and its inverse,
+  // then the inverses back out; the last axis keeps only the real part.
+  for (int i = 0; i < na; ++i) {
+    CplxIO f{};
+    f.in = (i == 0) ? data : p->work;
+    f.in_real = (i == 0);
+    f.do_fwd = 1;
+    const bool last = (i == na - 1);
+    f.kmul = last ? kdata : nullptr;
+    f.do_inv = last ? 1 : 0;
+    f.out = (last && na == 1) ? res : p->work;
+    f.out_real = (last && na == 1);
+    PSX_TRY(launch_cplx<T>(p->axes[i], p->prec, f, p->stream));
+  }
+  for (int i = na - 2; i >= 0; --i) {
+    CplxIO f{};
+    f.in = p->work;
+    f.in_real = 0;
+    f.do_fwd = 0;
+    f.do_inv = 1;
+    f.kmul = nullptr;
+    f.out = (i == 0) ? res : p->work;
+    f.out_real = (i == 0);
+    PSX_TRY(launch_cplx<T>(p->axes[i], p->prec, f, p->stream));
+  }
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_fft_scaled_pair(psx_fft* p, const void* d_data,
+                                          const void* d_kdata, void* d_res) {
+  if (!p || !d_data || !d_kdata || !d_res) return fail(PSX_ERR_INVALID, "null argument");
+  if (p->prec == PSX_FP64) return fft_pair_impl<double>(p, d_data, d_kdata, d_res);
+  return fft_pair_impl<float>(p, d_data, d_kdata, d_res);
+}
+
+extern "C" psx_status psx_fft_scaled_pair_host(psx_fft* p, const void* h_data,
+                                               const void* h_kdata, void* h_res) {
+  if (!p || !h_data || !h_kdata || !h_res) return fail(PSX_ERR_INVALID, "null argument");
+  const size_t bytes = p->N * prec_bytes(p->prec);
+  for (auto& h : p->hbuf)
+    if (!h) PSX_TRY(dalloc(&h, bytes));
+  PSX_HIP(hipMemcpyAsync(p->hbuf[0], h_data, bytes, hipMemcpyHostToDevice, p->stream));
+  PSX_HIP(hipMemcpyAsync(p->hbuf[1], h_kdata, bytes, hipMemcpyHostToDevice, p->stream));
+  PSX_TRY(psx_fft_scaled_pair(p, p->hbuf[0], p->hbuf[1], p->hbuf[2]));
+  PSX_HIP(hipMemcpyAsync(h_res, p->hbuf[2], bytes, hipMemcpyDeviceToHost, p->stream));
+  PSX_HIP(hipStreamSynchronize(p->stream));
+  return PSX_OK;
+}
+
+// ===========================================================================
+// SCFT engine
+// ===========================================================================
+namespace {
+
+struct SweepBufs {
+  bool ready = false;
+  void* F = nullptr;     // (Ns+1) slices, forward (head of A -> tail of B)
+  void* G = nullptr;     // (Ns+1) slices, backward (tail of B -> head of A)
+  void* tmpF = nullptr;
+  void* tmpG = nullptr;
+  void* wfA = nullptr;
+  void* wfB = nullptr;
+  void* emul[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};  // [block][axis idx]
+  bool wfac_valid = false;
+};
+
+struct GraphKey {
+  int prec, kind, depth, f_order;
+  double lam0, lam1, beta, start;
+  bool operator<(const GraphKey& o) const {
+    return std::memcmp(this, &o, sizeof(GraphKey)) < 0;
+  }
+};
+
+}  // namespace
+
+struct psx_scft {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int dims[3];
+  long long N;
+  double cell[3];
+  double fA, fB, ds, bA, bB, chiN;
+  int sA, sB, Ns;
+  int prec;
+  std::vector<Axis> axes;
+  SweepBufs sw[2];
+  double *wA = nullptr, *wB = nullptr, *praw = nullptr, *phiA = nullptr, *phiB = nullptr;
+  double* cw = nullptr;          // quadrature weights (Ns + 2)
+  double* part = nullptr;        // reduction partials
+  int nb = 1;                    // blocks of the 1D kernels
+  int part_cap = 0;              // doubles in part
+  DevState* st = nullptr;
+  double* hist = nullptr;
+  int hist_cap = 0;
+  double* ringx = nullptr;
+  double* ringf = nullptr;
+  int ring_m1 = 0;
+  bool swept = false;            // slices valid for the current fields
+  bool dens_valid = false;       // phi valid for the current slices
+  std::map<GraphKey, hipGraphExec_t> graphs;
+  // cached host-visible scratch for the host API
+  void* hstage = nullptr;
+};
+
+namespace {
+
+void drop_graphs(psx_scft* e) {
+  for (auto& kv : e->graphs) (void)hipGraphExecDestroy(kv.second);
+  e->graphs.clear();
+}
+
+// exp(-ds b^2 k_d^2) / n_d for every axis and block; the product over axes is
+// the reference kfac = exp(-ds b^2 k^2) (pspolymer/PsFlexPseudoSpec.cpp:188-206)
+// because k^2 = kx^2 + ky^2 + kz^2 (psbase/PsGridField.cpp:66-104).
+std::vector<double> axis_emul(const Axis& a, double cell, double ds, double b) {
+  std::vector<double> e(a.n);
+  const double twopi = 2.0 * std::acos(-1.0);
+  const double n2 = (double)a.n / 2.0;
+  for (int k = 0; k < a.n; ++k) {
+    const double nk = n2 - std::fabs((double)k - n2);
+    double kv = twopi * nk / (double)a.n;
+    kv = kv / cell;
+    e[k] = std::exp(-1.0 * ds * kv * kv * b * b) / (double)a.n;
+  }
+  return e;
+}
+
+psx_status ensure_sweep(psx_scft* e, int prec) {
+  SweepBufs& s = e->sw[prec];
+  if (s.ready) return PSX_OK;
+  const size_t tb = prec_bytes(prec);
+  const size_t slices = (size_t)(e->Ns + 1) * e->N * tb;
+  PSX_TRY(dalloc(&s.F, slices));
+  PSX_TRY(dalloc(&s.G, slices));
+  PSX_TRY(dalloc(&s.tmpF, e->N * tb));
+  PSX_TRY(dalloc(&s.tmpG, e->N * tb));
+  PSX_TRY(dalloc(&s.wfA, e->N * tb));
+  PSX_TRY(dalloc(&s.wfB, e->N * tb));
+  for (size_t i = 0; i < e->axes.size(); ++i) {
+    Axis& a = e->axes[i];
+    PSX_TRY(axis_tw(a, prec));
+    for (int blk = 0; blk < 2; ++blk) {
+      const double b = blk == 0 ? e->bA : e->bB;
+      const double h = a.n == 1 ? 1.0 : e->cell[a.axis];
+      std::vector<double> em = axis_emul(a, h, e->ds, b);
+      if (prec == PSX_FP64) PSX_TRY(upload(&s.emul[blk][i], em.data(), em.size() * 8));
+      else {
+        std::vector<float> f = cast_vec<float>(em);
+        PSX_TRY(upload(&s.emul[blk][i], f.data(), f.size() * 4));
+      }
+    }
+  }
+  // free ends: q(head of A) = q+(tail of B) = 1 (pspolymer/PsBlock.cpp:146-147)
+  const int g = grid_1d(e->N);
+  if (prec == PSX_FP64) {
+    hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kThreads), 0, e->stream, (double*)s.F, e->N, 1.0);
+    hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kThreads), 0, e->stream, (double*)s.G, e->N, 1.0);
+  } else {
+    hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kThreads), 0, e->stream, (float*)s.F, e->N, 1.0f);
+    hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kThreads), 0, e->stream, (float*)s.G, e->N, 1.0f);
+  }
+  PSX_HIP(hipGetLastError());
+  s.ready = true;
+  s.wfac_valid = false;
+  return PSX_OK;
+}
+
+void free_sweep(SweepBufs& s) {
+  dfree(s.F); dfree(s.G); dfree(s.tmpF); dfree(s.tmpG); dfree(s.wfA); dfree(s.wfB);
+  for (auto& b : s.emul) for (auto& p : b) dfree(p);
+  s.ready = false;
+}
+
+psx_status compute_wfac(psx_scft* e) {
+  SweepBufs& s = e->sw[e->prec];
+  const int g = grid_1d(e->N);
+  const double h = 0.5 * e->ds;
+  if (e->prec == PSX_FP64) {
+    hipLaunchKernelGGL(k_wfac<double>, dim3(g), dim3(kThreads), 0, e->stream, e->wA, e->N, h, (double*)s.wfA);
+    hipLaunchKernelGGL(k_wfac<double>, dim3(g), dim3(kThreads), 0, e->stream, e->wB, e->N, h, (double*)s.wfB);
+  } else {
+    hipLaunchKernelGGL(k_wfac<float>, dim3(g), dim3(kThreads), 0, e->stream, e->wA, e->N, h, (float*)s.wfA);
+    hipLaunchKernelGGL(k_wfac<float>, dim3(g), dim3(kThreads), 0, e->stream, e->wB, e->N, h, (float*)s.wfB);
+  }
+  PSX_HIP(hipGetLastError());
+  s.wfac_valid = true;
+  return PSX_OK;
+}
+
+// One contour step of both sweeps: F[t] -> F[t+1], G[t] -> G[t+1].
+template <typename T>
+psx_status dual_step(psx_scft* e, int t, int only_pass = -1) {
+  SweepBufs& s = e->sw[e->prec];
+  const size_t tb = sizeof(T);
+  const char* F = (const char*)s.F;
+  const char* G = (const char*)s.G;
+  const bool fa = t < e->sA;   // forward step t lies in block A
+  const bool gb = t < e->sB;   // backward step t lies in block B
+  const int na = (int)e->axes.size();
+  for (int i = 0; i < na; ++i) {
+    if (only_pass >= 0 && i != only_pass) continue;
+    const bool first = i == 0, last = i == na - 1;
+    PairIO f0, f1;
+    f0.in = first ? (const void*)(F + (size_t)t * e->N * tb) : s.tmpF;
+    f0.out = last ? (void*)(F + (size_t)(t + 1) * e->N * tb) : s.tmpF;
+    f0.pre = first ? (fa ? s.wfA : s.wfB) : nullptr;
+    f0.post = last ? (fa ? s.wfA : s.wfB) : nullptr;
+    f0.emul = s.emul[fa ? 0 : 1][i];
+    f1.in = first ? (const void*)(G + (size_t)t * e->N * tb) : s.tmpG;
+    f1.out = last ? (void*)(G + (size_t)(t + 1) * e->N * tb) : s.tmpG;
+    f1.pre = first ? (gb ? s.wfB : s.wfA) : nullptr;
+    f1.post = last ? (gb ? s.wfB : s.wfA) : nullptr;
+    f1.emul = s.emul[gb ? 1 : 0][i];
+    PSX_TRY(launch_pair<T>(e->axes[i], e->prec, f0, f1, 2, e->stream));
+  }
+  return PSX_OK;
+}
+
+psx_status sweep(psx_scft* e) {
+  for (int t = 0; t < e->Ns; ++t) {
+    if (e->prec == PSX_FP64) PSX_TRY(dual_step<double>(e, t));
+    else PSX_TRY(dual_step<float>(e, t));
+  }
+  return PSX_OK;
+}
+
+psx_status launch_Q(psx_scft* e) {
+  SweepBufs& s = e->sw[e->prec];
+  const int g = e->nb;
+  const size_t off = (size_t)e->Ns * e->N * prec_bytes(e->prec);
+  const void* last = (const char*)s.G + off;
+  if (e->prec == PSX_FP64)
+    hipLaunchKernelGGL(k_partial_sum<double>, dim3(g), dim3(kThreads), 0, e->stream, (const double*)last, e->N, e->part);
+  else
+    hipLaunchKernelGGL(k_partial_sum<float>, dim3(g), dim3(kThreads), 0, e->stream, (const float*)last, e->N, e->part);
+  hipLaunchKernelGGL(k_finish_Q, dim3(1), dim3(kThreads), 0, e->stream, (const double*)e->part, g,
+                     1.0 / (double)e->N, e->st);
+  PSX_HIP(hipGetLastError());
+  return PSX_OK;
+}
+
+QuadArgs quad_args(psx_scft* e, int mode, const psx_mixing* mix) {
+  SweepBufs& s = e->sw[e->prec];
+  QuadArgs a;
+  a.F = s.F; a.G = s.G; a.N = e->N; a.Ns = e->Ns; a.sA = e->sA;
+  a.cw = e->cw;
+  a.phiA = e->phiA; a.phiB = e->phiB;
+  a.wA = e->wA; a.wB = e->wB; a.praw = e->praw;
+  a.wfA = s.wfA; a.wfB = s.wfB;
+  a.ringx = e->ringx; a.ringf = e->ringf;
+  a.m1 = (mix && mix->kind == 1) ? mix->depth + 1 : 1;
+  a.chiN = e->chiN; a.fA = e->fA; a.fB = e->fB;
+  a.lam0 = mix ? mix->lambda0 : 0.0;
+  a.lam1 = mix ? mix->lambda1 : 0.0;
+  a.halfds = 0.5 * e->ds;
+  a.dens = 1.0;   // volfrac / lengthRatio (pspolymer/PsBlock.cpp:170-173)
+  a.st = e->st;
+  a.part = e->part;
+  a.mode = mode;
+  a.f_order = mix ? mix->f_order : 0;
+  return a;
+}
+
+psx_status launch_quad(psx_scft* e, int mode, const psx_mixing* mix) {
+  QuadArgs a = quad_args(e, mode, mix);
+  if (e->prec == PSX_FP64)
+    hipLaunchKernelGGL(k_quad_update<double>, dim3(e->nb), dim3(kThreads), 0, e->stream, a);
+  else
+    hipLaunchKernelGGL(k_quad_update<float>, dim3(e->nb), dim3(kThreads), 0, e->stream, a);
+  PSX_HIP(hipGetLastError());
+  return PSX_OK;
+}
+
+psx_status launch_finish(psx_scft* e, int mode, double* hist) {
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(kThreads), 0, e->stream, (const double*)e->part, e->nb,
+                     1.0 / (double)e->N, e->chiN, e->fA, e->fB, mode, e->st, hist);
+  PSX_HIP(hipGetLastError());
+  return PSX_OK;
+}
+
+psx_status ensure_ring(psx_scft* e, int depth) {
+  const int m1 = depth + 1;
+  if (e->ring_m1 >= m1 && e->ringx) return PSX_OK;
+  drop_graphs(e);
+  dfree((void*&)e->ringx);
+  dfree((void*&)e->ringf);
+  const size_t bytes = (size_t)m1 * 2 * e->N * sizeof(double);
+  PSX_TRY(dalloc((void**)&e->ringx, bytes));
+  PSX_TRY(dalloc((void**)&e->ringf, bytes));
+  PSX_HIP(hipMemset(e->ringx, 0, bytes));
+  PSX_HIP(hipMemset(e->ringf, 0, bytes));
+  e->ring_m1 = m1;
+  const int need = e->nb * (m1 > kNPart ? m1 : kNPart);
+  if (need > e->part_cap) {
+    dfree((void*&)e->part);
+    PSX_TRY(dalloc((void**)&e->part, need * sizeof(double)));
+    e->part_cap = need;
+  }
+  hipLaunchKernelGGL(k_state_reset_hist, dim3(1), dim3(1), 0, e->stream, e->st);
+  PSX_HIP(hipGetLastError());
+  return PSX_OK;
+}
+
+// Enqueue one full SCFT iteration on e->stream (capturable).
+psx_status enqueue_iteration(psx_scft* e, const psx_mixing* mix) {
+  PSX_TRY(sweep(e));
+  PSX_TRY(launch_Q(e));
+  if (mix->kind == 0) {
+    PSX_TRY(launch_quad(e, QM_SD, mix));
+    PSX_TRY(launch_finish(e, FIN_ITER_SD, e->hist));
+    return PSX_OK;
+  }
+  const int m = mix->depth;
+  const int m1 = m + 1;
+  PSX_TRY(launch_quad(e, QM_RESID, mix));
+  PSX_TRY(launch_finish(e, FIN_ERR, nullptr));
+  hipLaunchKernelGGL(k_gram, dim3(e->nb), dim3(kThreads), 0, e->stream, (const double*)e->ringf, e->N, m1,
+                     (const DevState*)e->st, e->part);
+  hipLaunchKernelGGL(k_anderson_solve, dim3(1), dim3(64), 0, e->stream, (const double*)e->part, e->nb, m,
+                     mix->beta, mix->lambda0, mix->anderson_start, e->st);
+  QuadArgs a = quad_args(e, QM_SD, mix);
+  if (e->prec == PSX_FP64)
+    hipLaunchKernelGGL(k_mix<double>, dim3(e->nb), dim3(kThreads), 0, e->stream, a);
+  else
+    hipLaunchKernelGGL(k_mix<float>, dim3(e->nb), dim3(kThreads), 0, e->stream, a);
+  PSX_HIP(hipGetLastError());
+  PSX_TRY(launch_finish(e, FIN_ITER_MIX, e->hist));
+  return PSX_OK;
+}
+
+psx_status read_state(psx_scft* e, DevState* h) {
+  PSX_HIP(hipMemcpyAsync(h, e->st, sizeof(DevState), hipMemcpyDeviceToHost, e->stream));
+  PSX_HIP(hipStreamSynchronize(e->stream));
+  return PSX_OK;
+}
+
+}  // namespace
+
+extern "C" psx_status psx_scft_destroy(psx_scft* e) {
+  if (!e) return PSX_OK;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  drop_graphs(e);
+  for (auto& s : e->sw) free_sweep(s);
+  for (auto& a : e->axes) { dfree(a.tw[0]); dfree(a.tw[1]); }
+  dfree((void*&)e->wA); dfree((void*&)e->wB); dfree((void*&)e->praw);
+  dfree((void*&)e->phiA); dfree((void*&)e->phiB); dfree((void*&)e->cw);
+  dfree((void*&)e->part); dfree((void*&)e->st); dfree((void*&)e->hist);
+  dfree((void*&)e->ringx); dfree((void*&)e->ringf); dfree(e->hstage);
+  if (e->own_stream && e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+  return PSX_OK;
+}
+
+// Block steps per psbase/PsBlockBase.cpp:252-326 (forceBlockSteps = false).
+static int block_steps(double len, double ds) {
+  const double bs = len / ds;
+  const double ip = (double)(long)bs;
+  const double dx = bs - ip;
+  if (dx > 0.0001 && dx < 0.9999) return -1;
+  const long c = (long)std::ceil(bs), f = (long)std::floor(bs);
+  long r;
+  if (c == f) r = c;
+  else if (std::fabs(bs - (double)c) < std::fabs(bs - (double)f)) r = c;
+  else r = f;
+  if (r < 4) return -2;
+  return (int)r;
+}
+
+// Simpson (literal 0.33333) with a trapezoid tail for an even point count,
+// pspolymer/PsBlock.cpp:256-380, as per-point weights over M = steps+1.
+static void quad_weights(int steps, double ds, double* w) {
+  const int M = steps + 1;
+  const bool even = (M % 2) == 0;
+  const int last = even ? M - 2 : M - 1;   // Simpson covers points 0..last
+  for (int j = 0; j < M; ++j) w[j] = 0.0;
+  for (int n = 0; n + 2 <= last; n += 2) {
+    w[n] += 1.0; w[n + 1] += 4.0; w[n + 2] += 1.0;
+  }
+  for (int j = 0; j <= last; ++j) w[j] *= 0.33333 * ds;
+  if (even) { w[M - 2] += 0.5 * ds; w[M - 1] += 0.5 * ds; }
+}
+
+extern "C" psx_status psx_scft_create(const psx_grid* g, const psx_diblock* c,
+                                      psx_prec prec, void* stream, psx_scft** out) {
+  if (!out || !g || !c) return fail(PSX_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (g->nx < 1 || g->ny < 1 || g->nz < 1) return fail(PSX_ERR_SHAPE, "grid dims must be >= 1");
+  if (!(g->dx > 0 && g->dy > 0 && g->dz > 0)) return fail(PSX_ERR_INVALID, "cell sizes must be > 0");
+  if ((long long)g->nx * g->ny * g->nz > (1LL << 30)) return fail(PSX_ERR_SHAPE, "grid too large");
+  if (prec != PSX_FP64 && prec != PSX_FP32) return fail(PSX_ERR_INVALID, "bad precision");
+  if (!(c->ds > 0) || !(c->fA > 0 && c->fA < 1)) return fail(PSX_ERR_INVALID, "need ds > 0 and 0 < fA < 1");
+  const int sA = block_steps(c->fA, c->ds);
+  const int sB = block_steps(1.0 - c->fA, c->ds);
+  if (sA < 0 || sB < 0)
+    return fail(PSX_ERR_STEPS, "non-integral or too few block steps (fA/ds = " +
+                                   std::to_string(c->fA / c->ds) + ", (1-fA)/ds = " +
+                                   std::to_string((1.0 - c->fA) / c->ds) + ")");
+  psx_scft* e = new psx_scft();
+  e->dims[0] = g->nx; e->dims[1] = g->ny; e->dims[2] = g->nz;
+  e->N = (long long)g->nx * g->ny * g->nz;
+  e->cell[0] = g->dx; e->cell[1] = g->dy; e->cell[2] = g->dz;
+  e->fA = c->fA; e->fB = 1.0 - c->fA; e->ds = c->ds;
+  e->bA = c->bA; e->bB = c->bB; e->chiN = c->chiN;
+  e->sA = sA; e->sB = sB; e->Ns = sA + sB;
+  e->prec = prec;
+  psx_status st = PSX_OK;
+#define E_TRY(x) do { if (st == PSX_OK) st = (x); } while (0)
+  E_TRY((hipGetDevice(&e->dev) == hipSuccess) ? PSX_OK : fail(PSX_ERR_HIP, "hipGetDevice failed"));
+  E_TRY(make_stream(stream, &e->stream, &e->own_stream));
+  E_TRY(make_axes(e->dims, &e->axes));
+  const size_t fb = e->N * sizeof(double);
+  E_TRY(dalloc((void**)&e->wA, fb));
+  E_TRY(dalloc((void**)&e->wB, fb));
+  E_TRY(dalloc((void**)&e->praw, fb));
+  E_TRY(dalloc((void**)&e->phiA, fb));
+  E_TRY(dalloc((void**)&e->phiB, fb));
+  e->nb = grid_1d(e->N);
+  e->part_cap = e->nb * kNPart;
+  E_TRY(dalloc((void**)&e->part, e->part_cap * sizeof(double)));
+  E_TRY(dalloc((void**)&e->st, sizeof(DevState)));
+  std::vector<double> w(e->Ns + 2);
+  quad_weights(sA, e->ds, w.data());
+  quad_weights(sB, e->ds, w.data() + sA + 1);
+  E_TRY(upload((void**)&e->cw, w.data(), w.size() * sizeof(double)));
+  if (st == PSX_OK) {
+    DevState h;
+    std::memset(&h, 0, sizeof(h));
+    h.Q = 1.0;
+    st = (hipMemcpy(e->st, &h, sizeof(h), hipMemcpyHostToDevice) == hipSuccess)
+             ? PSX_OK : fail(PSX_ERR_HIP, "state upload failed");
+  }
+  if (st == PSX_OK) {
+    const bool ok = hipMemset(e->wA, 0, fb) == hipSuccess && hipMemset(e->wB, 0, fb) == hipSuccess &&
+                    hipMemset(e->praw, 0, fb) == hipSuccess && hipMemset(e->phiA, 0, fb) == hipSuccess &&
+                    hipMemset(e->phiB, 0, fb) == hipSuccess;
+    if (!ok) st = fail(PSX_ERR_HIP, "field init failed");
+  }
+  E_TRY(ensure_sweep(e, prec));
+#undef E_TRY
+  if (st != PSX_OK) { psx_scft_destroy(e); return st; }
+  *out = e;
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_scft_set_precision(psx_scft* e, psx_prec prec) {
+  if (!e) return fail(PSX_ERR_INVALID, "null engine");
+  if (prec != PSX_FP64 && prec != PSX_FP32) return fail(PSX_ERR_INVALID, "bad precision");
+  if (e->prec == (int)prec) return PSX_OK;
+  PSX_TRY(ensure_sweep(e, prec));
+  e->prec = prec;
+  e->sw[prec].wfac_valid = false;
+  e->swept = false;
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_scft_block_steps(const psx_scft* e, int* sA, int* sB) {
+  if (!e || !sA || !sB) return fail(PSX_ERR_INVALID, "null argument");
+  *sA = e->sA; *sB = e->sB;
+  return PSX_OK;
+}
+
+static void fields_changed(psx_scft* e) {
+  e->sw[0].wfac_valid = false;
+  e->sw[1].wfac_valid = false;
+  e->swept = false;
+  e->dens_valid = false;
+}
+
+// p is stored as p_raw with pbar = 0.
+static psx_status set_pbar_zero(psx_scft* e) {
+  DevState h;
+  PSX_TRY(read_state(e, &h));
+  h.pbar = 0.0;
+  PSX_HIP(hipMemcpyAsync(e->st, &h, sizeof(h), hipMemcpyHostToDevice, e->stream));
+  PSX_HIP(hipStreamSynchronize(e->stream));
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_scft_set_fields(psx_scft* e, const double* wA, const double* wB,
+                                          const double* p) {
+  if (!e || !wA || !wB) return fail(PSX_ERR_INVALID, "null argument");
+  const size_t fb = e->N * sizeof(double);
+  PSX_HIP(hipMemcpyAsync(e->wA, wA, fb, hipMemcpyDeviceToDevice, e->stream));
+  PSX_HIP(hipMemcpyAsync(e->wB, wB, fb, hipMemcpyDeviceToDevice, e->stream));
+  if (p) PSX_HIP(hipMemcpyAsync(e->praw, p, fb, hipMemcpyDeviceToDevice, e->stream));
+  else PSX_HIP(hipMemsetAsync(e->praw, 0, fb, e->stream));
+  PSX_TRY(set_pbar_zero(e));
+  fields_changed(e);
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_scft_set_fields_host(psx_scft* e, const double* wA, const double* wB,
+                                               const double* p) {
+  if (!e || !wA || !wB) return fail(PSX_ERR_INVALID, "null argument");
+  const size_t fb = e->N * sizeof(double);
+  PSX_HIP(hipMemcpyAsync(e->wA, wA, fb, hipMemcpyHostToDevice, e->stream));
+  PSX_HIP(hipMemcpyAsync(e->wB, wB, fb, hipMemcpyHostToDevice, e->stream));
+  if (p) PSX_HIP(hipMemcpyAsync(e->praw, p, fb, hipMemcpyHostToDevice, e->stream));
+  else PSX_HIP(hipMemsetAsync(e->praw, 0, fb, e->stream));
+  PSX_TRY(set_pbar_zero(e));
+  fields_changed(e);
+  return PSX_OK;
+}
+
+static psx_status get_fields_impl(psx_scft* e, double* wA, double* wB, double* p, hipMemcpyKind k) {
+  const size_t fb = e->N * sizeof(double);
+  if (wA) PSX_HIP(hipMemcpyAsync(wA, e->wA, fb, k, e->stream));
+  if (wB) PSX_HIP(hipMemcpyAsync(wB, e->wB, fb, k, e->stream));
+  if (p) {
+    if (!e->hstage) PSX_TRY(dalloc(&e->hstage, fb));
+    hipLaunchKernelGGL(k_pressure, dim3(grid_1d(e->N)), dim3(kThreads), 0, e->stream,
+                       (const double*)e->praw, (const DevState*)e->st, e->N, (double*)e->hstage);
+    PSX_HIP(hipGetLastError());
+    PSX_HIP(hipMemcpyAsync(p, e->hstage, fb, k, e->stream));
+  }
+  PSX_HIP(hipStreamSynchronize(e->stream));
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_scft_get_fields(psx_scft* e, double* wA, double* wB, double* p) {
+  if (!e) return fail(PSX_ERR_INVALID, "null engine");
+  return get_fields_impl(e, wA, wB, p, hipMemcpyDeviceToDevice);
+}
+
+extern "C" psx_status psx_scft_get_fields_host(psx_scft* e, double* wA, double* wB, double* p) {
+  if (!e) return fail(PSX_ERR_INVALID, "null engine");
+  return get_fields_impl(e, wA, wB, p, hipMemcpyDeviceToHost);
+}
+
+extern "C" psx_status psx_scft_propagate(psx_scft* e) {
+  if (!e) return fail(PSX_ERR_INVALID, "null engine");
+  PSX_TRY(compute_wfac(e));
+  PSX_TRY(sweep(e));
+  e->swept = true;
+  e->dens_valid = false;
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_scft_Q(psx_scft* e, double* Q) {
+  if (!e || !Q) return fail(PSX_ERR_INVALID, "null argument");
+  if (!e->swept) return fail(PSX_ERR_STATE, "propagate before Q");
+  PSX_TRY(launch_Q(e));
+  DevState h;
+  PSX_TRY(read_state(e, &h));
+  *Q = h.Q;
+  if (h.nanflag || !(h.Q == h.Q) || std::isinf(h.Q)) {
+    int z = 0;
+    (void)hipMemcpy(&e->st->nanflag, &z, sizeof(int), hipMemcpyHostToDevice);
+    return fail(PSX_ERR_NAN, "Q is NaN/Inf (PsBlock::calcBigQ check)");
+  }
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_scft_densities(psx_scft* e, double* phiA, double* phiB) {
+  if (!e) return fail(PSX_ERR_INVALID, "null engine");
+  if (!e->swept) return fail(PSX_ERR_STATE, "propagate before densities");
+  double Q;
+  PSX_TRY(psx_scft_Q(e, &Q));
+  PSX_TRY(launch_quad(e, QM_DENS, nullptr));
+  const size_t fb = e->N * sizeof(double);
+  if (phiA) PSX_HIP(hipMemcpyAsync(phiA, e->phiA, fb, hipMemcpyDeviceToDevice, e->stream));
+  if (phiB) PSX_HIP(hipMemcpyAsync(phiB, e->phiB, fb, hipMemcpyDeviceToDevice, e->stream));
+  PSX_HIP(hipStreamSynchronize(e->stream));
+  e->dens_valid = true;
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_scft_get_densities_host(psx_scft* e, double* phiA, double* phiB) {
+  if (!e) return fail(PSX_ERR_INVALID, "null engine");
+  if (!e->dens_valid) return fail(PSX_ERR_STATE, "no densities computed yet");
+  const size_t fb = e->N * sizeof(double);
+  if (phiA) PSX_HIP(hipMemcpyAsync(phiA, e->phiA, fb, hipMemcpyDeviceToHost, e->stream));
+  if (phiB) PSX_HIP(hipMemcpyAsync(phiB, e->phiB, fb, hipMemcpyDeviceToHost, e->stream));
+  PSX_HIP(hipStreamSynchronize(e->stream));
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_scft_get_slice_host(psx_scft* e, int backward, int t, double* out) {
+  if (!e || !out) return fail(PSX_ERR_INVALID, "null argument");
+  if (t < 0 || t > e->Ns) return fail(PSX_ERR_INVALID, "slice index out of range");
+  SweepBufs& s = e->sw[e->prec];
+  const size_t fb = e->N * sizeof(double);
+  if (!e->hstage) PSX_TRY(dalloc(&e->hstage, fb));
+  const char* base = (const char*)(backward ? s.G : s.F) + (size_t)t * e->N * prec_bytes(e->prec);
+  if (e->prec == PSX_FP64)
+    hipLaunchKernelGGL(k_to_f64<double>, dim3(grid_1d(e->N)), dim3(kThreads), 0, e->stream,
+                       (const double*)base, e->N, (double*)e->hstage);
+  else
+    hipLaunchKernelGGL(k_to_f64<float>, dim3(grid_1d(e->N)), dim3(kThreads), 0, e->stream,
+                       (const float*)base, e->N, (double*)e->hstage);
+  PSX_HIP(hipGetLastError());
+  PSX_HIP(hipMemcpyAsync(out, e->hstage, fb, hipMemcpyDeviceToHost, e->stream));
+  PSX_HIP(hipStreamSynchronize(e->stream));
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_scft_update_steepest(psx_scft* e, double lambda0, double lambda1,
+                                               double* F) {
+  if (!e) return fail(PSX_ERR_INVALID, "null engine");
+  if (!e->dens_valid) return fail(PSX_ERR_STATE, "densities before the update");
+  psx_mixing mix{};
+  mix.kind = 0; mix.lambda0 = lambda0; mix.lambda1 = lambda1; mix.f_order = 0;
+  hipLaunchKernelGGL(k_state_set_call, dim3(1), dim3(1), 0, e->stream, e->st);
+  PSX_TRY(launch_quad(e, QM_SD_PHI, &mix));
+  PSX_TRY(launch_finish(e, FIN_ITER_SD, nullptr));
+  DevState h;
+  PSX_TRY(read_state(e, &h));
+  if (F) *F = h.F;
+  e->sw[e->prec].wfac_valid = true;
+  e->sw[e->prec ^ 1].wfac_valid = false;
+  e->swept = false;
+  e->dens_valid = false;
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_scft_free_energy(psx_scft* e, double* F) {
+  if (!e || !F) return fail(PSX_ERR_INVALID, "null argument");
+  if (!e->dens_valid) return fail(PSX_ERR_STATE, "densities before the free energy");
+  PSX_TRY(launch_quad(e, QM_FE, nullptr));
+  PSX_TRY(launch_finish(e, FIN_FE, nullptr));
+  DevState h;
+  PSX_TRY(read_state(e, &h));
+  *F = h.F;
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_scft_reset_history(psx_scft* e) {
+  if (!e) return fail(PSX_ERR_INVALID, "null engine");
+  hipLaunchKernelGGL(k_state_reset_hist, dim3(1), dim3(1), 0, e->stream, e->st);
+  PSX_HIP(hipGetLastError());
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_scft_iterate(psx_scft* e, const psx_mixing* mix, int n_iters, double tol,
+                                       int use_graph, double* hF, double* hQ, double* herr,
+                                       int* done) {
+  if (!e || !mix) return fail(PSX_ERR_INVALID, "null argument");
+  if (done) *done = 0;
+  if (n_iters <= 0) return PSX_OK;
+  if (mix->kind != 0 && mix->kind != 1) return fail(PSX_ERR_INVALID, "mixing kind must be 0 or 1");
+  if (mix->kind == 1 && (mix->depth < 0 || mix->depth > kMaxDepth))
+    return fail(PSX_ERR_INVALID, "Anderson depth must be in [0, 32]");
+  if (mix->kind == 1) PSX_TRY(ensure_ring(e, mix->depth));
+  if (n_iters > e->hist_cap) {
+    drop_graphs(e);
+    dfree((void*&)e->hist);
+    PSX_TRY(dalloc((void**)&e->hist, (size_t)n_iters * 3 * sizeof(double)));
+    e->hist_cap = n_iters;
+  }
+  if (!e->sw[e->prec].wfac_valid) PSX_TRY(compute_wfac(e));
+  hipLaunchKernelGGL(k_state_set_call, dim3(1), dim3(1), 0, e->stream, e->st);
+  PSX_HIP(hipGetLastError());
+
+  hipGraphExec_t exec = nullptr;
+  if (use_graph) {
+    GraphKey key;
+    std::memset(&key, 0, sizeof(key));
+    key.prec = e->prec; key.kind = mix->kind; key.depth = mix->kind ? mix->depth : 0;
+    key.f_order = mix->f_order; key.lam0 = mix->lambda0; key.lam1 = mix->lambda1;
+    key.beta = mix->kind ? mix->beta : 0.0; key.start = mix->kind ? mix->anderson_start : 0.0;
+    auto it = e->graphs.find(key);
+    if (it != e->graphs.end()) exec = it->second;
+    else {
+      hipGraph_t graph;
+      PSX_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+      psx_status cs = enqueue_iteration(e, mix);
+      hipError_t ce = hipStreamEndCapture(e->stream, &graph);
+      if (cs != PSX_OK) { if (ce == hipSuccess) (void)hipGraphDestroy(graph); return cs; }
+      if (ce != hipSuccess) return fail(PSX_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+      hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ie != hipSuccess) return fail(PSX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+      e->graphs[key] = exec;
+    }
+  }
+  int it = 0;
+  DevState h;
+  psx_status result = PSX_OK;
+  for (; it < n_iters; ++it) {
+    if (exec) PSX_HIP(hipGraphLaunch(exec, e->stream));
+    else PSX_TRY(enqueue_iteration(e, mix));
+    if (tol > 0.0) {
+      PSX_TRY(read_state(e, &h));
+      if (h.nanflag) { result = fail(PSX_ERR_NAN, "Q is NaN/Inf (PsBlock::calcBigQ check)"); ++it; break; }
+      if (h.err < tol) { ++it; break; }
+    }
+  }
+  PSX_TRY(read_state(e, &h));
+  if (result == PSX_OK && h.nanflag) result = fail(PSX_ERR_NAN, "Q is NaN/Inf (PsBlock::calcBigQ check)");
+  if (h.nanflag) {
+    int z = 0;
+    (void)hipMemcpy(&e->st->nanflag, &z, sizeof(int), hipMemcpyHostToDevice);
+  }
+  if (done) *done = it;
+  if (hF || hQ || herr) {
+    std::vector<double> hh((size_t)it * 3);
+    PSX_HIP(hipMemcpy(hh.data(), e->hist, hh.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < it; ++i) {
+      if (hF) hF[i] = hh[3 * i];
+      if (hQ) hQ[i] = hh[3 * i + 1];
+      if (herr) herr[i] = hh[3 * i + 2];
+    }
+  }
+  e->sw[e->prec].wfac_valid = true;
+  e->sw[e->prec ^ 1].wfac_valid = false;
+  e->swept = false;
+  e->dens_valid = true;   // phi of the last iteration's sweep
+  return result;
+}
+
+extern "C" psx_status psx_scft_box_stress(psx_scft* e, double* dFdL) {
+  if (!e || !dFdL) return fail(PSX_ERR_INVALID, "null argument");
+  return fail(PSX_ERR_STATE, "box stress not built in this version");
+}
+
+extern "C" psx_status psx_scft_time_step(psx_scft* e, int reps, double* ms_step, double* ms_pass) {
+  if (!e || reps < 1) return fail(PSX_ERR_INVALID, "bad argument");
+  if (!e->sw[e->prec].wfac_valid) PSX_TRY(compute_wfac(e));
+  hipEvent_t a, b;
+  PSX_HIP(hipEventCreate(&a));
+  PSX_HIP(hipEventCreate(&b));
+  const int na = (int)e->axes.size();
+  auto run = [&](int pass) -> psx_status {
+    for (int r = 0; r < reps; ++r) {
+      if (e->prec == PSX_FP64) PSX_TRY(dual_step<double>(e, 0, pass));
+      else PSX_TRY(dual_step<float>(e, 0, pass));
+    }
+    return PSX_OK;
+  };
+  PSX_TRY(run(-1));  // warm
+  PSX_HIP(hipEventRecord(a, e->stream));
+  PSX_TRY(run(-1));
+  PSX_HIP(hipEventRecord(b, e->stream));
+  PSX_HIP(hipEventSynchronize(b));
+  float ms = 0.f;
+  PSX_HIP(hipEventElapsedTime(&ms, a, b));
+  if (ms_step) *ms_step = ms / reps;
+  for (int i = 0; i < 3; ++i) {
+    double v = 0.0;
+    if (i < na) {
+      PSX_HIP(hipEventRecord(a, e->stream));
+      PSX_TRY(run(i));
+      PSX_HIP(hipEventRecord(b, e->stream));
+      PSX_HIP(hipEventSynchronize(b));
+      PSX_HIP(hipEventElapsedTime(&ms, a, b));
+      v = ms / reps;
+    }
+    if (ms_pass) ms_pass[i] = v;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  e->swept = false;
+  return PSX_OK;
+}

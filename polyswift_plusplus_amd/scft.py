@@ -1,0 +1,217 @@
+"""Host-side mirror of the reference operator interface for the SCFT hot path.
+
+Names and argument meanings follow PolySwift++:
+
+* ``PsFFT.scaledFFTPair(data, kdata)`` -- ``PsFFTBase::scaledFFTPair``
+  (psbase/PsFFTBase.h:105-106, psfft/PsFFTW.cpp:196-224): unnormalised
+  forward/backward pair, real part kept.
+* ``DiblockSCFT`` -- the AB diblock of ``PsBlockCopolymer`` with two
+  ``PsFlexPseudoSpec`` blocks (pspolymer/PsBlockCopolymer.cpp:189-239),
+  the ``flory`` interaction and the ``steepestDescent`` updater
+  (pseffhamil/PsSteepDUpdater.cpp:159-333), plus Anderson mixing.
+
+Every call goes through the C ABI of libpsx_scft.so on the current HIP
+device; errors raise ``PsxError`` the way the reference throws
+``TxDebugExcept``.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi
+from ._abi import PSX_FP32, PSX_FP64, PsxError, check
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _prec(p) -> int:
+    if p in (PSX_FP64, "fp64", "f64", "double", np.float64):
+        return PSX_FP64
+    if p in (PSX_FP32, "fp32", "f32", "float", np.float32):
+        return PSX_FP32
+    raise ValueError(f"unknown precision {p!r}")
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+class PsFFT:
+    """FFT pair plugin over the psx HIP line-FFT kernels (serial plan)."""
+
+    def __init__(self, shape, prec="fp64", stream=None):
+        self._L = _abi.load()
+        self.shape = tuple(int(s) for s in shape)
+        self.prec = _prec(prec)
+        self.dtype = np.float64 if self.prec == PSX_FP64 else np.float32
+        h = ctypes.c_void_p()
+        check(self._L.psx_fft_plan_create(*self.shape, self.prec, stream, ctypes.byref(h)))
+        self._h = h
+
+    def getFFTSize(self) -> int:
+        return int(self._L.psx_fft_size(self._h))
+
+    def scaledFFTPair(self, data, kdata) -> np.ndarray:
+        """Host arrays in, host array out (the reference's own calling form)."""
+        d = np.ascontiguousarray(data, dtype=self.dtype)
+        k = np.ascontiguousarray(kdata, dtype=self.dtype)
+        if d.size != self.getFFTSize() or k.size != d.size:
+            raise PsxError(2, "data/kdata size differs from the FFT size")
+        out = np.empty_like(d)
+        check(self._L.psx_fft_scaled_pair_host(self._h, d.ctypes.data, k.ctypes.data,
+                                                out.ctypes.data))
+        return out.reshape(self.shape)
+
+    def scaledFFTPair_device(self, d_data: int, d_kdata: int, d_res: int) -> None:
+        """Device pointers (e.g. torch ``tensor.data_ptr()``); asynchronous."""
+        check(self._L.psx_fft_scaled_pair(self._h, d_data, d_kdata, d_res))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.psx_fft_plan_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+
+@dataclass
+class Mixing:
+    """Field-update method (``steepestDescent`` [lambda0, lambda1] or Anderson)."""
+    kind: str = "steepest"       # "steepest" | "anderson"
+    lambda0: float = 0.1
+    lambda1: float = 0.0
+    depth: int = 10
+    beta: float = 0.1
+    anderson_start: float = 1e300
+    f_order: int = 0             # 0 reference (F after update), 1 consistent
+
+    def c(self) -> _abi.Mixing:
+        return _abi.Mixing(0 if self.kind == "steepest" else 1, self.lambda0, self.lambda1,
+                           int(self.depth), self.beta, self.anderson_start, int(self.f_order))
+
+
+class DiblockSCFT:
+    """Bulk AB diblock melt on an nx*ny*nz periodic grid (one HIP device)."""
+
+    def __init__(self, shape, cell, fA, ds, chiN, bA=1.0, bB=1.0, prec="fp64", stream=None):
+        self._L = _abi.load()
+        self.shape = tuple(int(s) for s in shape)
+        self.cell = tuple(float(c) for c in cell)
+        self.N = int(np.prod(self.shape))
+        self.fA, self.ds, self.chiN = float(fA), float(ds), float(chiN)
+        g = _abi.Grid(*self.shape, *self.cell)
+        c = _abi.Diblock(self.fA, self.ds, float(bA), float(bB), self.chiN)
+        h = ctypes.c_void_p()
+        check(self._L.psx_scft_create(ctypes.byref(g), ctypes.byref(c), _prec(prec), stream,
+                                      ctypes.byref(h)))
+        self._h = h
+        sA, sB = ctypes.c_int(), ctypes.c_int()
+        check(self._L.psx_scft_block_steps(h, ctypes.byref(sA), ctypes.byref(sB)))
+        self.sA, self.sB = sA.value, sB.value
+        self.Ns = self.sA + self.sB
+
+    # -- fields -------------------------------------------------------------
+    def setFields(self, wA, wB, p=None) -> None:
+        a, b = _f64(wA).reshape(-1), _f64(wB).reshape(-1)
+        if a.size != self.N or b.size != self.N:
+            raise PsxError(2, "field size differs from the grid")
+        pp = None if p is None else _f64(p).reshape(-1)
+        check(self._L.psx_scft_set_fields_host(self._h, _ptr(a), _ptr(b),
+                                               None if pp is None else _ptr(pp)))
+
+    def getFields(self):
+        a, b, p = (np.empty(self.N) for _ in range(3))
+        check(self._L.psx_scft_get_fields_host(self._h, _ptr(a), _ptr(b), _ptr(p)))
+        return a.reshape(self.shape), b.reshape(self.shape), p.reshape(self.shape)
+
+    def setFieldsDevice(self, d_wA: int, d_wB: int, d_p: int | None = None) -> None:
+        check(self._L.psx_scft_set_fields(self._h, d_wA, d_wB, d_p))
+
+    def setPrecision(self, prec) -> None:
+        check(self._L.psx_scft_set_precision(self._h, _prec(prec)))
+
+    # -- polymer: PsBlockCopolymer::update split into its reference steps ----
+    def solveQ(self) -> None:
+        """w-factors and all block propagators (reset + solveQ per block)."""
+        check(self._L.psx_scft_propagate(self._h))
+
+    def calcBigQ(self) -> float:
+        q = ctypes.c_double()
+        check(self._L.psx_scft_Q(self._h, ctypes.byref(q)))
+        return q.value
+
+    def densities(self):
+        """setCalcQQTIntegral + setPhysFields -> (phiA, phiB) on the host."""
+        check(self._L.psx_scft_densities(self._h, None, None))
+        return self.getDensities()
+
+    def getDensities(self):
+        """Densities of the last densities()/iterate() call."""
+        a, b = np.empty(self.N), np.empty(self.N)
+        check(self._L.psx_scft_get_densities_host(self._h, _ptr(a), _ptr(b)))
+        return a.reshape(self.shape), b.reshape(self.shape)
+
+    def update(self):
+        """PsBlockCopolymer::update: propagate, Q, densities."""
+        self.solveQ()
+        return self.densities()
+
+    def slice(self, backward: bool, t: int) -> np.ndarray:
+        out = np.empty(self.N)
+        check(self._L.psx_scft_get_slice_host(self._h, int(bool(backward)), int(t), _ptr(out)))
+        return out.reshape(self.shape)
+
+    # -- effective Hamiltonian ----------------------------------------------
+    def steepestDescent(self, lambda0: float, lambda1: float = 0.0) -> float:
+        """PsSteepDUpdater::update from the current densities; returns F."""
+        F = ctypes.c_double()
+        check(self._L.psx_scft_update_steepest(self._h, lambda0, lambda1, ctypes.byref(F)))
+        return F.value
+
+    def getFreeE(self) -> float:
+        F = ctypes.c_double()
+        check(self._L.psx_scft_free_energy(self._h, ctypes.byref(F)))
+        return F.value
+
+    def iterate(self, n_iters: int, mixing: Mixing | None = None, tol: float = 0.0,
+                use_graph: bool = True) -> dict:
+        """Full SCFT iterations (PsDomain::update); returns histories."""
+        mixing = mixing or Mixing()
+        F, Q, err = np.zeros(n_iters), np.zeros(n_iters), np.zeros(n_iters)
+        done = ctypes.c_int()
+        m = mixing.c()
+        check(self._L.psx_scft_iterate(self._h, ctypes.byref(m), int(n_iters), float(tol),
+                                       int(bool(use_graph)), _ptr(F), _ptr(Q), _ptr(err),
+                                       ctypes.byref(done)))
+        k = done.value
+        return dict(F=F[:k], Q=Q[:k], err=err[:k], iters=k)
+
+    def resetHistory(self) -> None:
+        check(self._L.psx_scft_reset_history(self._h))
+
+    def boxStress(self) -> np.ndarray:
+        out = np.zeros(3)
+        check(self._L.psx_scft_box_stress(self._h, _ptr(out)))
+        return out
+
+    def timeStep(self, reps: int = 20):
+        """(ms per dual contour step, [ms per z, y, x pass])."""
+        step = ctypes.c_double()
+        passes = (ctypes.c_double * 3)()
+        check(self._L.psx_scft_time_step(self._h, int(reps), ctypes.byref(step),
+                                         ctypes.cast(passes, _dp)))
+        return step.value, list(passes)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.psx_scft_destroy(self._h)
+            self._h = None
+
+    __del__ = close

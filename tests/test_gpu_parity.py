@@ -1,0 +1,146 @@
+"""GPU parity of the HIP engine against the CPU oracle (FP64 to 1e-10 rel).
+
+Every call goes through the C ABI of libpsx_scft.so.  Sizes are ones the
+oracle finishes in seconds; full-size behaviour is covered by invariants in
+test_gpu_invariants.py.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from polyswift_plusplus_amd import DiblockSCFT, Mixing, PsFFT, PsxError, fields
+
+pytestmark = pytest.mark.gpu
+
+RTOL64 = 1e-10
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("shape", [(8, 12, 10), (16, 16, 16), (4, 6, 8), (3, 5, 1),
+                                   (64, 64, 1), (7, 1, 1), (1, 1, 1), (2, 9, 10)])
+def test_fft_pair_fp64(shape):
+    print(f"fft pair fp64 {shape}", flush=True)
+    rng = np.random.default_rng(11)
+    d = rng.standard_normal(shape)
+    k = rng.random(shape)
+    ref = O.scaled_fft_pair(d, k)
+    out = PsFFT(shape, "fp64").scaledFFTPair(d, k)
+    assert rel(out, ref) < 1e-12
+
+
+def test_fft_pair_fp32():
+    shape = (32, 24, 16)
+    rng = np.random.default_rng(12)
+    d = rng.standard_normal(shape)
+    k = rng.random(shape)
+    ref = O.scaled_fft_pair(d, k)
+    out = PsFFT(shape, "fp32").scaledFFTPair(d, k)
+    assert rel(out, ref) < 2e-5
+
+
+def test_fft_pair_size_mismatch():
+    f = PsFFT((8, 8, 8))
+    with pytest.raises(PsxError):
+        f.scaledFFTPair(np.zeros(10), np.zeros(10))
+
+
+def _case(shape=(16, 12, 10), cell=(0.3, 0.25, 0.35), fA=0.3, ds=0.05, chiN=15.0, seed=3):
+    P = O.Params(*shape, *cell, ds=ds, fA=fA, chiN=chiN)
+    wA, wB = fields.seeded_fields(shape, cell, sigma=0.5, seed=seed, seed_kind="lamellar")
+    return P, wA, wB
+
+
+@pytest.mark.parametrize("shape,fA,ds", [((16, 12, 10), 0.3, 0.05), ((8, 8, 8), 0.35, 0.05),
+                                         ((12, 10, 1), 0.5, 0.1), ((20, 1, 1), 0.25, 0.05)])
+def test_propagator_slices(shape, fA, ds):
+    print(f"slices {shape} fA={fA} ds={ds}", flush=True)
+    cell = (0.3, 0.25, 0.35)
+    P, wA, wB = _case(shape, cell, fA, ds)
+    phiA, phiB, Q, (qA, qB, qtB, qtA) = O.diblock_densities(P, wA, wB, want_slices=True)
+    e = DiblockSCFT(shape, cell, fA, ds, P.chiN)
+    assert (e.sA, e.sB) == (qA.shape[0] - 1, qB.shape[0] - 1)
+    e.setFields(wA, wB)
+    e.solveQ()
+    for t in range(e.Ns + 1):
+        ref = qA[t] if t <= e.sA else qB[t - e.sA]
+        assert rel(e.slice(False, t), ref) < 1e-12, t
+        ref = qtB[t] if t <= e.sB else qtA[t - e.sB]
+        assert rel(e.slice(True, t), ref) < 1e-12, t
+    assert abs(e.calcBigQ() / Q - 1) < 1e-12
+    gA, gB = e.densities()
+    assert rel(gA, phiA) < RTOL64 and rel(gB, phiB) < RTOL64
+
+
+def test_steepest_descent_trajectory():
+    P, wA, wB = _case()
+    P.lambda0, P.lambda1 = 0.1, 0.05
+    ref = O.scft_run(P, wA, wB, n_iters=12)
+    e = DiblockSCFT(P.shape, (P.dx, P.dy, P.dz), P.fA, P.ds, P.chiN)
+    e.setFields(wA, wB)
+    h = e.iterate(12, Mixing("steepest", 0.1, 0.05), use_graph=False)
+    assert rel(h["F"], ref["F"]) < RTOL64
+    assert rel(h["Q"], ref["Q"]) < RTOL64
+    assert rel(h["err"], ref["err"]) < 1e-9
+    gA, gB, gp = e.getFields()
+    assert rel(gA, ref["wA"]) < RTOL64 and rel(gB, ref["wB"]) < RTOL64
+    assert np.max(np.abs(gp - ref["p"])) < 1e-10 * np.max(np.abs(ref["wA"]))
+
+
+def test_graph_replay_matches_eager():
+    P, wA, wB = _case()
+    mix = Mixing("steepest", 0.1, 0.0)
+    e1 = DiblockSCFT(P.shape, (P.dx, P.dy, P.dz), P.fA, P.ds, P.chiN)
+    e1.setFields(wA, wB)
+    h1 = e1.iterate(6, mix, use_graph=False)
+    e2 = DiblockSCFT(P.shape, (P.dx, P.dy, P.dz), P.fA, P.ds, P.chiN)
+    e2.setFields(wA, wB)
+    ha = e2.iterate(3, mix, use_graph=True)
+    hb = e2.iterate(3, mix, use_graph=True)   # replays the cached graph
+    F = np.concatenate([ha["F"], hb["F"]])
+    assert np.array_equal(F, h1["F"])
+
+
+def test_granular_api_matches_iterate():
+    P, wA, wB = _case()
+    e = DiblockSCFT(P.shape, (P.dx, P.dy, P.dz), P.fA, P.ds, P.chiN)
+    e.setFields(wA, wB)
+    Fs = []
+    for _ in range(3):
+        e.update()
+        Fs.append(e.steepestDescent(0.1, 0.0))
+    ref = O.scft_run(P, wA, wB, n_iters=3)
+    assert rel(Fs, ref["F"]) < RTOL64
+
+
+def test_anderson_trajectory():
+    P, wA, wB = _case(chiN=12.0)
+    P.mix, P.depth, P.beta, P.lambda0 = 1, 5, 0.2, 0.1
+    P.anderson_start = 1e300
+    ref = O.scft_run(P, wA, wB, n_iters=15)
+    e = DiblockSCFT(P.shape, (P.dx, P.dy, P.dz), P.fA, P.ds, P.chiN)
+    e.setFields(wA, wB)
+    h = e.iterate(15, Mixing("anderson", lambda0=0.1, depth=5, beta=0.2), use_graph=True)
+    assert rel(h["F"], ref["F"]) < 1e-9
+    assert rel(h["err"], ref["err"]) < 1e-8
+
+
+def test_nan_field_raises():
+    P, wA, wB = _case()
+    wA = wA.copy()
+    wA[0, 0, 0] = np.nan
+    e = DiblockSCFT(P.shape, (P.dx, P.dy, P.dz), P.fA, P.ds, P.chiN)
+    e.setFields(wA, wB)
+    e.solveQ()
+    with pytest.raises(PsxError) as ex:
+        e.calcBigQ()
+    assert ex.value.status == 4
+
+
+def test_bad_block_steps_raises():
+    with pytest.raises(PsxError) as ex:
+        DiblockSCFT((8, 8, 8), (0.1, 0.1, 0.1), 0.333, 0.01, 10.0)
+    assert ex.value.status == 3
