@@ -1,0 +1,233 @@
+// psx_fft_reg.hpp -- register-resident round-trip line FFT (one wave per tile).
+//
+// A line of n = N1 * N2 complex points is held by N2 lanes, N1 points each:
+// lane t owns x[t + N2 r], r < N1.  The forward transform is the two-level
+// split  X[k1 + N1 k2] = sum_t W_N2^{t k2} W_n^{t k1} sum_r x[t + N2 r] W_N1^{r k1}:
+// an in-register DFT_N1, a twiddle, one LDS transpose inside the wave, then
+// N1/N2 in-register DFT_N2 per lane.  The spectral multiply happens in
+// registers and the inverse retraces the same steps, so the result lands in
+// the lane layout of the input: one round trip costs two wave-local LDS
+// transposes.  A wave holds 64/N2 complex lines = 2*64/N2 real lines (two
+// real lines share one complex line; the operator is real and even, so
+// their results come back in the real and imaginary parts).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "psx_fft_lds.hpp"
+
+namespace psx {
+
+template <int N>
+constexpr int first_radix() {
+  return (N % 8 == 0 && N > 8) ? 8 : (N % 4 == 0 ? 4 : (N % 2 == 0 ? 2 : (N % 3 == 0 ? 3 : (N % 5 == 0 ? 5 : N))));
+}
+
+template <int SIGN, typename C>
+__device__ __forceinline__ C twiddle(const C* __restrict__ tw, int idx) {
+  C w = tw[idx];
+  if (SIGN > 0) w.y = -w.y;
+  return w;
+}
+
+// In-register natural-order DFT of size N (all indices compile-time).
+// tw is the size-n table exp(-2 pi i j / n); twst = n / N.
+template <int N, int SIGN, typename T, typename C>
+__device__ __forceinline__ void dft_reg(C* v, const C* __restrict__ tw, int twst) {
+  if constexpr (N == 1) {
+    return;
+  } else if constexpr (N == 2 || N == 3 || N == 4 || N == 5 || N == 8) {
+    bfly<N, SIGN, T>(v);
+  } else {
+    constexpr int R = first_radix<N>();
+    constexpr int M = N / R;
+    static_assert(R * M == N && R > 1 && M > 1, "unsupported in-register DFT size");
+    C U[N];
+#pragma unroll
+    for (int b = 0; b < M; ++b) {
+      C u[R];
+#pragma unroll
+      for (int a = 0; a < R; ++a) u[a] = v[b + M * a];
+      bfly<R, SIGN, T>(u);
+#pragma unroll
+      for (int c = 0; c < R; ++c) {
+        if (b * c == 0) U[c * M + b] = u[c];
+        else U[c * M + b] = cmul(u[c], twiddle<SIGN>(tw, b * c * twst));
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < R; ++c) {
+      C m[M];
+#pragma unroll
+      for (int b = 0; b < M; ++b) m[b] = U[c * M + b];
+      dft_reg<M, SIGN, T>(m, tw, twst * R);
+#pragma unroll
+      for (int k2 = 0; k2 < M; ++k2) v[c + R * k2] = m[k2];
+    }
+  }
+}
+
+// Round trip out = post * IFFT(emul * FFT(pre * in)) along one axis, one
+// wave per 2*64/N2 real lines.  VEC: lines of a wave are consecutive in
+// memory (inner % lines-per-wave == 0), so a real pair is one aligned
+// complex load.
+template <typename T, int N1, int N2, bool CONTIG, bool VEC>
+__global__ void __launch_bounds__(64)
+k_line_reg(LineGeom g, PairIO f0, PairIO f1) {
+  typedef typename Cx<T>::type C;
+  constexpr int n = N1 * N2;
+  constexpr int CL = 64 / N2;     // complex lines per wave
+  constexpr int RL = 2 * CL;      // real lines per wave
+  constexpr int U = N1 / N2;      // DFT_N2 per lane after the transpose
+  static_assert(N1 % N2 == 0 && 64 % N2 == 0, "bad split");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* lds = (C*)smem_raw;
+
+  const bool second = blockIdx.y != 0;
+  const T* __restrict__ in = (const T*)(second ? f1.in : f0.in);
+  T* __restrict__ out = (T*)(second ? f1.out : f0.out);
+  const T* __restrict__ pre = (const T*)(second ? f1.pre : f0.pre);
+  const T* __restrict__ post = (const T*)(second ? f1.post : f0.post);
+  const T* __restrict__ emul = (const T*)(second ? f1.emul : f0.emul);
+  const C* __restrict__ tw = (const C*)g.tw;
+
+  const int lane = threadIdx.x;
+  const int p = lane / N2, t = lane % N2;
+  const int la = blockIdx.x * RL + 2 * p;   // real lines la, la + 1
+  const bool va = la < g.nlines, vb = la + 1 < g.nlines;
+  const int inner = g.inner;
+  // element k of line l: base(l) + k * stride
+  int ba, bb, stride;
+  if (CONTIG) { ba = la * n; bb = ba + n; stride = 1; }
+  else {
+    const int oa = la / inner, ob = (la + 1) / inner;
+    ba = oa * n * inner + (la - oa * inner);
+    bb = ob * n * inner + (la + 1 - ob * inner);
+    stride = inner;
+  }
+
+  C v[N1];
+#pragma unroll
+  for (int r = 0; r < N1; ++r) {
+    const int k = t + N2 * r;
+    C x;
+    if (VEC) {
+      if (va) {
+        x = *(const C*)(in + ba + k * stride);
+        if (pre) { const C w = *(const C*)(pre + ba + k * stride); x.x *= w.x; x.y *= w.y; }
+      } else { x.x = (T)0; x.y = (T)0; }
+    } else {
+      x.x = va ? in[ba + k * stride] : (T)0;
+      x.y = vb ? in[bb + k * stride] : (T)0;
+      if (pre) {
+        if (va) x.x *= pre[ba + k * stride];
+        if (vb) x.y *= pre[bb + k * stride];
+      }
+    }
+    v[r] = x;
+  }
+
+  // forward: DFT_N1 in registers, twiddle W_n^{t k1}
+  dft_reg<N1, -1, T>(v, tw, N2);
+#pragma unroll
+  for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], twiddle<-1>(tw, t * k1));
+
+  // transpose: element (k1, t) of line p at p*n + k1*N2 + (t ^ (k1 % N2))
+  C* L = lds + p * n;
+#pragma unroll
+  for (int k1 = 0; k1 < N1; ++k1) L[k1 * N2 + (t ^ (k1 % N2))] = v[k1];
+  __syncthreads();
+  // lane t now takes rows k1 = t + N2 u, u < U; y[u][s] = element (k1, s)
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+#pragma unroll
+    for (int s = 0; s < N2; ++s) v[u * N2 + s] = L[(t + N2 * u) * N2 + (s ^ t)];
+
+  // DFT_N2, spectral multiply at k = k1 + N1 k2, inverse DFT_N2
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    C* y = v + u * N2;
+    dft_reg<N2, -1, T>(y, tw, N1);
+    const int k1 = t + N2 * u;
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) {
+      const T m = emul[k1 + N1 * k2];
+      y[k2].x *= m; y[k2].y *= m;
+    }
+    dft_reg<N2, +1, T>(y, tw, N1);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+#pragma unroll
+    for (int s = 0; s < N2; ++s) L[(t + N2 * u) * N2 + (s ^ t)] = v[u * N2 + s];
+  __syncthreads();
+#pragma unroll
+  for (int k1 = 0; k1 < N1; ++k1) v[k1] = L[k1 * N2 + (t ^ (k1 % N2))];
+
+  // inverse: conjugate twiddle, inverse DFT_N1; lanes hold x[t + N2 r]
+#pragma unroll
+  for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], twiddle<+1>(tw, t * k1));
+  dft_reg<N1, +1, T>(v, tw, N2);
+
+#pragma unroll
+  for (int r = 0; r < N1; ++r) {
+    const int k = t + N2 * r;
+    C x = v[r];
+    if (VEC) {
+      if (va) {
+        if (post) { const C w = *(const C*)(post + ba + k * stride); x.x *= w.x; x.y *= w.y; }
+        *(C*)(out + ba + k * stride) = x;
+      }
+    } else {
+      if (va) out[ba + k * stride] = post ? x.x * post[ba + k * stride] : x.x;
+      if (vb) out[bb + k * stride] = post ? x.y * post[bb + k * stride] : x.y;
+    }
+  }
+}
+
+// Whole line in one lane (n <= 8): no transposes.
+template <typename T, int NN, bool CONTIG>
+__global__ void __launch_bounds__(64)
+k_line_lane(LineGeom g, PairIO f0, PairIO f1) {
+  typedef typename Cx<T>::type C;
+  const bool second = blockIdx.y != 0;
+  const T* __restrict__ in = (const T*)(second ? f1.in : f0.in);
+  T* __restrict__ out = (T*)(second ? f1.out : f0.out);
+  const T* __restrict__ pre = (const T*)(second ? f1.pre : f0.pre);
+  const T* __restrict__ post = (const T*)(second ? f1.post : f0.post);
+  const T* __restrict__ emul = (const T*)(second ? f1.emul : f0.emul);
+  const C* __restrict__ tw = (const C*)g.tw;
+  const int la = (blockIdx.x * 64 + threadIdx.x) * 2;
+  const bool va = la < g.nlines, vb = la + 1 < g.nlines;
+  const int inner = g.inner;
+  int ba, bb, stride;
+  if (CONTIG) { ba = la * NN; bb = ba + NN; stride = 1; }
+  else {
+    const int oa = la / inner, ob = (la + 1) / inner;
+    ba = oa * NN * inner + (la - oa * inner);
+    bb = ob * NN * inner + (la + 1 - ob * inner);
+    stride = inner;
+  }
+  C v[NN];
+#pragma unroll
+  for (int k = 0; k < NN; ++k) {
+    C x;
+    x.x = va ? in[ba + k * stride] : (T)0;
+    x.y = vb ? in[bb + k * stride] : (T)0;
+    if (pre) {
+      if (va) x.x *= pre[ba + k * stride];
+      if (vb) x.y *= pre[bb + k * stride];
+    }
+    v[k] = x;
+  }
+  dft_reg<NN, -1, T>(v, tw, 1);
+#pragma unroll
+  for (int k = 0; k < NN; ++k) { const T m = emul[k]; v[k].x *= m; v[k].y *= m; }
+  dft_reg<NN, +1, T>(v, tw, 1);
+#pragma unroll
+  for (int k = 0; k < NN; ++k) {
+    if (va) out[ba + k * stride] = post ? v[k].x * post[ba + k * stride] : v[k].x;
+    if (vb) out[bb + k * stride] = post ? v[k].y * post[bb + k * stride] : v[k].y;
+  }
+}
+
+}  // namespace psx

@@ -404,51 +404,62 @@ __global__ void k_finish(const double* __restrict__ part, int nb, double invN,
   st->call_iter += 1;
 }
 
-// Anderson: per-block partial dots <f_cur, f_slot> for all ring slots.
+// Anderson: per-block partial dots <f_cur, f_slot> for all ring slots
+// (MAXM1 >= m1 is a compile-time bound so the accumulators stay in VGPRs).
+template <int MAXM1>
 __global__ void __launch_bounds__(kThreads)
 k_gram(const double* __restrict__ ringf, long long N, int m1, const DevState* st,
        double* __restrict__ part) {
   const long long cur = st->iter % m1;
   const double* fc = ringf + cur * 2 * N;
-  double acc[kMaxDepth + 1];
-  for (int j = 0; j < m1; ++j) acc[j] = 0.0;
+  double acc[MAXM1];
+#pragma unroll
+  for (int j = 0; j < MAXM1; ++j) acc[j] = 0.0;
   for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < 2 * N;
        r += (long long)gridDim.x * blockDim.x) {
     const double x = fc[r];
-    for (int j = 0; j < m1; ++j) acc[j] += x * ringf[(long long)j * 2 * N + r];
+#pragma unroll
+    for (int j = 0; j < MAXM1; ++j)
+      if (j < m1) acc[j] += x * ringf[(long long)j * 2 * N + r];
   }
-  // reduce each slot's partial over the block
-  __shared__ double sh[kThreads / 64];
+  __shared__ double sh[MAXM1][kThreads / 64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (int j = 0; j < m1; ++j) {
+#pragma unroll
+  for (int j = 0; j < MAXM1; ++j) {
     double x = acc[j];
+#pragma unroll
     for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
-    if (lane == 0) sh[wid] = x;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double s = 0.0;
-      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sh[w];
-      part[(long long)blockIdx.x * m1 + j] = s;
-    }
-    __syncthreads();
+    if (lane == 0) sh[j][wid] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < m1) {
+    double s = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sh[threadIdx.x][w];
+    part[(long long)blockIdx.x * m1 + threadIdx.x] = s;
+  }
+}
+
+// Anderson: column sums of the Gram partials -> row `cur` of the Gram matrix
+// (one block per ring slot).
+__global__ void __launch_bounds__(kThreads)
+k_gram_finish(const double* __restrict__ part, int nb, int m1, DevState* st) {
+  const int j = blockIdx.x;
+  double v[1] = {0.0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) v[0] += part[(long long)b * m1 + j];
+  block_sum<1>(v);
+  if (threadIdx.x == 0) {
+    const long long cur = st->iter % m1;
+    st->gram[cur * m1 + j] = v[0];
+    st->gram[j * m1 + cur] = v[0];
   }
 }
 
 // Anderson: finish the Gram row, then solve U c = V for the mixing
 // coefficients (one block; the solve itself is one thread, m <= 32).
-__global__ void k_anderson_solve(const double* __restrict__ part, int nb, int m,
-                                 double beta, double lam0, double start,
+__global__ void k_anderson_solve(int m, double beta, double lam0, double start,
                                  DevState* st) {
   const int m1 = m + 1;
   const long long cur = st->iter % m1;
-  for (int j = threadIdx.x; j < m1; j += blockDim.x) {
-    double s = 0.0;
-    for (int b = 0; b < nb; ++b) s += part[(long long)b * m1 + j];
-    st->gram[cur * m1 + j] = s;
-    st->gram[j * m1 + cur] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
   long long avail = st->since_reset < m ? st->since_reset : m;
   int mk = (int)avail;
   if (st->err > start) mk = 0;
@@ -508,9 +519,6 @@ k_mix(QuadArgs a) {
   const long long cur = st->iter % m1;
   const int mk = st->mk;
   const double beta = st->beta_used;
-  double c[kMaxDepth];
-  long long sl[kMaxDepth];
-  for (int i = 0; i < mk; ++i) { c[i] = st->coef[i]; sl[i] = (cur - 1 - i + m1) % m1; }
   const double* fc = a.ringf + cur * 2 * N;
   double acc[kNPart] = {0.0, 0.0, 0.0, 0.0};
   for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < N;
@@ -522,8 +530,10 @@ k_mix(QuadArgs a) {
       const double f = fc[o];
       double xb = x, fb = f;
       for (int i = 0; i < mk; ++i) {
-        xb += c[i] * (a.ringx[sl[i] * 2 * N + o] - x);
-        fb += c[i] * (a.ringf[sl[i] * 2 * N + o] - f);
+        const double ci = st->coef[i];
+        const long long sl = (cur - 1 - i + m1) % m1;
+        xb += ci * (a.ringx[sl * 2 * N + o] - x);
+        fb += ci * (a.ringf[sl * 2 * N + o] - f);
       }
       old[h] = x;
       nw[h] = xb + beta * fb;

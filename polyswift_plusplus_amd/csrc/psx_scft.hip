@@ -9,6 +9,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -16,6 +17,7 @@
 
 #include "../../include/psx_scft.h"
 #include "psx_kernels.hpp"
+#include "psx_fft_reg.hpp"
 
 using namespace psx;
 
@@ -177,9 +179,36 @@ LineGeom geom(const Axis& a, int prec, int L) {
   return g;
 }
 
+template <typename T, int N1, int N2>
+void launch_reg(const Axis& a, int prec, const PairIO& f0, const PairIO& f1, int nfields,
+                hipStream_t s) {
+  constexpr int RL = 2 * 64 / N2;
+  LineGeom g = geom(a, prec, RL);
+  const size_t smem = (size_t)64 * N1 * 2 * sizeof(T);
+  dim3 grid((a.nlines + RL - 1) / RL, nfields);
+  if (a.contig)
+    hipLaunchKernelGGL((k_line_reg<T, N1, N2, true, false>), grid, dim3(64), smem, s, g, f0, f1);
+  else if (a.inner % RL == 0)
+    hipLaunchKernelGGL((k_line_reg<T, N1, N2, false, true>), grid, dim3(64), smem, s, g, f0, f1);
+  else
+    hipLaunchKernelGGL((k_line_reg<T, N1, N2, false, false>), grid, dim3(64), smem, s, g, f0, f1);
+}
+
+template <typename T, int NN>
+void launch_lane(const Axis& a, int prec, const PairIO& f0, const PairIO& f1, int nfields,
+                 hipStream_t s) {
+  LineGeom g = geom(a, prec, 2);
+  dim3 grid((a.nlines + 127) / 128, nfields);
+  if (a.contig)
+    hipLaunchKernelGGL((k_line_lane<T, NN, true>), grid, dim3(64), 0, s, g, f0, f1);
+  else
+    hipLaunchKernelGGL((k_line_lane<T, NN, false>), grid, dim3(64), 0, s, g, f0, f1);
+}
+
+// Generic Stockham LDS kernel: any n with prime factors <= 13.
 template <typename T>
-psx_status launch_pair(const Axis& a, int prec, const PairIO& f0, const PairIO& f1,
-                       int nfields, hipStream_t s) {
+void launch_stockham(const Axis& a, int prec, const PairIO& f0, const PairIO& f1, int nfields,
+                     hipStream_t s) {
   const int L = pair_tile(a, prec);
   LineGeom g = geom(a, prec, L);
   const int P = L / 2;
@@ -189,6 +218,41 @@ psx_status launch_pair(const Axis& a, int prec, const PairIO& f0, const PairIO& 
     hipLaunchKernelGGL((k_pair_roundtrip<T, true>), grid, dim3(kThreads), smem, s, g, f0, f1);
   else
     hipLaunchKernelGGL((k_pair_roundtrip<T, false>), grid, dim3(kThreads), smem, s, g, f0, f1);
+}
+
+// Round-trip pass along one axis: register-resident kernels for the line
+// lengths of the workloads, the LDS Stockham kernel for everything else.
+// PSX_FORCE_STOCKHAM=1 in the environment selects the Stockham kernel for
+// every length (used to cross-check the two kernel families).
+int g_force_stockham = -1;
+
+template <typename T>
+psx_status launch_pair(const Axis& a, int prec, const PairIO& f0, const PairIO& f1,
+                       int nfields, hipStream_t s) {
+  if (g_force_stockham < 0) {
+    const char* v = getenv("PSX_FORCE_STOCKHAM");
+    g_force_stockham = (v && v[0] == '1') ? 1 : 0;
+  }
+  if (g_force_stockham == 1) {
+    launch_stockham<T>(a, prec, f0, f1, nfields, s);
+  } else {
+    switch (a.n) {
+      case 1: launch_lane<T, 1>(a, prec, f0, f1, nfields, s); break;
+      case 2: launch_lane<T, 2>(a, prec, f0, f1, nfields, s); break;
+      case 3: launch_lane<T, 3>(a, prec, f0, f1, nfields, s); break;
+      case 4: launch_lane<T, 4>(a, prec, f0, f1, nfields, s); break;
+      case 5: launch_lane<T, 5>(a, prec, f0, f1, nfields, s); break;
+      case 8: launch_lane<T, 8>(a, prec, f0, f1, nfields, s); break;
+      case 16: launch_reg<T, 4, 4>(a, prec, f0, f1, nfields, s); break;
+      case 32: launch_reg<T, 8, 4>(a, prec, f0, f1, nfields, s); break;
+      case 48: launch_reg<T, 12, 4>(a, prec, f0, f1, nfields, s); break;
+      case 64: launch_reg<T, 8, 8>(a, prec, f0, f1, nfields, s); break;
+      case 96: launch_reg<T, 24, 4>(a, prec, f0, f1, nfields, s); break;
+      case 128: launch_reg<T, 16, 8>(a, prec, f0, f1, nfields, s); break;
+      case 256: launch_reg<T, 16, 16>(a, prec, f0, f1, nfields, s); break;
+      default: launch_stockham<T>(a, prec, f0, f1, nfields, s); break;
+    }
+  }
   PSX_HIP(hipGetLastError());
   return PSX_OK;
 }
@@ -602,10 +666,22 @@ psx_status enqueue_iteration(psx_scft* e, const psx_mixing* mix) {
   const int m1 = m + 1;
   PSX_TRY(launch_quad(e, QM_RESID, mix));
   PSX_TRY(launch_finish(e, FIN_ERR, nullptr));
-  hipLaunchKernelGGL(k_gram, dim3(e->nb), dim3(kThreads), 0, e->stream, (const double*)e->ringf, e->N, m1,
-                     (const DevState*)e->st, e->part);
-  hipLaunchKernelGGL(k_anderson_solve, dim3(1), dim3(64), 0, e->stream, (const double*)e->part, e->nb, m,
-                     mix->beta, mix->lambda0, mix->anderson_start, e->st);
+  if (m1 <= 4)
+    hipLaunchKernelGGL(k_gram<4>, dim3(e->nb), dim3(kThreads), 0, e->stream, (const double*)e->ringf, e->N, m1,
+                       (const DevState*)e->st, e->part);
+  else if (m1 <= 11)
+    hipLaunchKernelGGL(k_gram<11>, dim3(e->nb), dim3(kThreads), 0, e->stream, (const double*)e->ringf, e->N, m1,
+                       (const DevState*)e->st, e->part);
+  else if (m1 <= 21)
+    hipLaunchKernelGGL(k_gram<21>, dim3(e->nb), dim3(kThreads), 0, e->stream, (const double*)e->ringf, e->N, m1,
+                       (const DevState*)e->st, e->part);
+  else
+    hipLaunchKernelGGL(k_gram<kMaxDepth + 1>, dim3(e->nb), dim3(kThreads), 0, e->stream, (const double*)e->ringf,
+                       e->N, m1, (const DevState*)e->st, e->part);
+  hipLaunchKernelGGL(k_gram_finish, dim3(m1), dim3(kThreads), 0, e->stream, (const double*)e->part, e->nb, m1,
+                     e->st);
+  hipLaunchKernelGGL(k_anderson_solve, dim3(1), dim3(1), 0, e->stream, m, mix->beta, mix->lambda0,
+                     mix->anderson_start, e->st);
   QuadArgs a = quad_args(e, QM_SD, mix);
   if (e->prec == PSX_FP64)
     hipLaunchKernelGGL(k_mix<double>, dim3(e->nb), dim3(kThreads), 0, e->stream, a);

@@ -54,8 +54,18 @@ def _case(shape=(16, 12, 10), cell=(0.3, 0.25, 0.35), fA=0.3, ds=0.05, chiN=15.0
     return P, wA, wB
 
 
-@pytest.mark.parametrize("shape,fA,ds", [((16, 12, 10), 0.3, 0.05), ((8, 8, 8), 0.35, 0.05),
-                                         ((12, 10, 1), 0.5, 0.1), ((20, 1, 1), 0.25, 0.05)])
+# Shapes chosen to run every line-kernel variant: register kernels for
+# n = 16, 32, 48, 64, 96, 128 (contiguous z lines, strided lines with and
+# without paired vector loads), one-lane kernels for n <= 8, and the
+# Stockham kernel for the remaining lengths (10, 12, 20, 6, ...).
+SLICE_CASES = [((16, 12, 10), 0.3, 0.05), ((8, 8, 8), 0.35, 0.05), ((12, 10, 1), 0.5, 0.1),
+               ((20, 1, 1), 0.25, 0.05), ((4, 8, 128), 0.3, 0.05), ((4, 128, 8), 0.3, 0.05),
+               ((128, 4, 8), 0.3, 0.05), ((96, 2, 16), 0.4, 0.05), ((8, 48, 6), 0.3, 0.05),
+               ((32, 32, 32), 0.35, 0.05), ((64, 16, 2), 0.3, 0.05), ((2, 3, 96), 0.3, 0.1),
+               ((16, 16, 16), 0.3, 0.05)]
+
+
+@pytest.mark.parametrize("shape,fA,ds", SLICE_CASES)
 def test_propagator_slices(shape, fA, ds):
     print(f"slices {shape} fA={fA} ds={ds}", flush=True)
     cell = (0.3, 0.25, 0.35)
@@ -144,3 +154,19 @@ def test_bad_block_steps_raises():
     with pytest.raises(PsxError) as ex:
         DiblockSCFT((8, 8, 8), (0.1, 0.1, 0.1), 0.333, 0.01, 10.0)
     assert ex.value.status == 3
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32), (4, 128, 8), (96, 2, 16), (12, 10, 6)])
+def test_propagator_fp32_error(shape):
+    """FP32 sweep: slices and Q within FP32 rounding of the FP64 oracle."""
+    cell = (0.3, 0.25, 0.35)
+    P, wA, wB = _case(shape, cell, 0.3, 0.05)
+    phiA, phiB, Q, (qA, qB, qtB, qtA) = O.diblock_densities(P, wA, wB, want_slices=True)
+    e = DiblockSCFT(shape, cell, 0.3, 0.05, P.chiN, prec="fp32")
+    e.setFields(wA, wB)
+    e.solveQ()
+    assert rel(e.slice(False, e.Ns), qB[-1]) < 2e-5
+    assert rel(e.slice(True, e.Ns), qtA[-1]) < 2e-5
+    assert abs(e.calcBigQ() / Q - 1) < 2e-5
+    gA, gB = e.densities()
+    assert rel(gA, phiA) < 2e-5 and rel(gB, phiB) < 2e-5
