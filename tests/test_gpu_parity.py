@@ -61,7 +61,7 @@ def _case(shape=(16, 12, 10), cell=(0.3, 0.25, 0.35), fA=0.3, ds=0.05, chiN=15.0
 SLICE_CASES = [((16, 12, 10), 0.3, 0.05), ((8, 8, 8), 0.35, 0.05), ((12, 10, 1), 0.5, 0.1),
                ((20, 1, 1), 0.25, 0.05), ((4, 8, 128), 0.3, 0.05), ((4, 128, 8), 0.3, 0.05),
                ((128, 4, 8), 0.3, 0.05), ((96, 2, 16), 0.4, 0.05), ((8, 48, 6), 0.3, 0.05),
-               ((32, 32, 32), 0.35, 0.05), ((64, 16, 2), 0.3, 0.05), ((2, 3, 96), 0.3, 0.1),
+               ((32, 32, 32), 0.35, 0.05), ((64, 16, 2), 0.3, 0.05), ((2, 3, 96), 0.4, 0.1),
                ((16, 16, 16), 0.3, 0.05)]
 
 
