@@ -16,6 +16,27 @@
 
 namespace psx {
 
+// 16-byte vectors for staged contiguous rows.
+template <typename T> struct Vec16;
+template <> struct Vec16<double> { typedef double2 type; };
+template <> struct Vec16<float> { typedef float4 type; };
+__device__ __forceinline__ void vmul(double2& a, const double2& b) { a.x *= b.x; a.y *= b.y; }
+__device__ __forceinline__ void vmul(float4& a, const float4& b) {
+  a.x *= b.x; a.y *= b.y; a.z *= b.z; a.w *= b.w;
+}
+// LDS rows are padded, so staged 16-byte chunks are only element-aligned.
+__device__ __forceinline__ void vstore_lds(double* p, const double2& v) { p[0] = v.x; p[1] = v.y; }
+__device__ __forceinline__ void vstore_lds(float* p, const float4& v) {
+  p[0] = v.x; p[1] = v.y; p[2] = v.z; p[3] = v.w;
+}
+template <typename T> __device__ __forceinline__ typename Vec16<T>::type vload_lds(const T* p);
+template <> __device__ __forceinline__ double2 vload_lds<double>(const double* p) {
+  double2 v; v.x = p[0]; v.y = p[1]; return v;
+}
+template <> __device__ __forceinline__ float4 vload_lds<float>(const float* p) {
+  float4 v; v.x = p[0]; v.y = p[1]; v.z = p[2]; v.w = p[3]; return v;
+}
+
 template <int N>
 constexpr int first_radix() {
   return (N % 8 == 0 && N > 8) ? 8 : (N % 4 == 0 ? 4 : (N % 2 == 0 ? 2 : (N % 3 == 0 ? 3 : (N % 5 == 0 ? 5 : N))));
@@ -105,8 +126,40 @@ k_line_reg(LineGeom g, PairIO f0, PairIO f1) {
   }
 
   C v[N1];
+  // Contiguous lines: the wave's RL rows are one contiguous block of RL*n
+  // elements.  Stage it through LDS with 16-byte loads, rows padded by
+  // N2/2 elements so that the FFT-layout reads hit distinct banks.
+  constexpr int PAD = N2 / 2;
+  constexpr int RS = n + PAD;                    // staged row stride
+  constexpr int VW = 16 / sizeof(T);             // elements per 16-byte chunk
+  if (CONTIG) {
+    T* S = (T*)smem_raw;
+    const int row0 = blockIdx.x * RL;
+    const int nrows = min(RL, g.nlines - row0);
+    const T* src = in + (size_t)row0 * n;
+    const T* psrc = pre ? pre + (size_t)row0 * n : nullptr;
+    const int nchunk = nrows * n / VW;
+#pragma unroll 4
+    for (int c = lane; c < nchunk; c += 64) {
+      const int e = c * VW, row = e / n, col = e - row * n;
+      typedef typename Vec16<T>::type V;
+      V x = *(const V*)(src + e);
+      if (psrc) { const V w = *(const V*)(psrc + e); vmul(x, w); }
+      vstore_lds(S + row * RS + col, x);
+    }
+    __syncthreads();
 #pragma unroll
-  for (int r = 0; r < N1; ++r) {
+    for (int r = 0; r < N1; ++r) {
+      const int k = t + N2 * r;
+      C x;
+      x.x = (2 * p < nrows) ? S[(2 * p) * RS + k] : (T)0;
+      x.y = (2 * p + 1 < nrows) ? S[(2 * p + 1) * RS + k] : (T)0;
+      v[r] = x;
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < N1 && !CONTIG; ++r) {
     const int k = t + N2 * r;
     C x;
     if (VEC) {
@@ -168,6 +221,31 @@ k_line_reg(LineGeom g, PairIO f0, PairIO f1) {
   for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], twiddle<+1>(tw, t * k1));
   dft_reg<N1, +1, T>(v, tw, N2);
 
+  if (CONTIG) {
+    T* S = (T*)smem_raw;
+    const int row0 = blockIdx.x * RL;
+    const int nrows = min(RL, g.nlines - row0);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < N1; ++r) {
+      const int k = t + N2 * r;
+      if (2 * p < nrows) S[(2 * p) * RS + k] = v[r].x;
+      if (2 * p + 1 < nrows) S[(2 * p + 1) * RS + k] = v[r].y;
+    }
+    __syncthreads();
+    T* dst = out + (size_t)row0 * n;
+    const T* pp = post ? post + (size_t)row0 * n : nullptr;
+    const int nchunk = nrows * n / VW;
+#pragma unroll 4
+    for (int c = lane; c < nchunk; c += 64) {
+      const int e = c * VW, row = e / n, col = e - row * n;
+      typedef typename Vec16<T>::type V;
+      V x = vload_lds<T>(S + row * RS + col);
+      if (pp) { const V w = *(const V*)(pp + e); vmul(x, w); }
+      *(V*)(dst + e) = x;
+    }
+    return;
+  }
 #pragma unroll
   for (int r = 0; r < N1; ++r) {
     const int k = t + N2 * r;

@@ -184,7 +184,10 @@ void launch_reg(const Axis& a, int prec, const PairIO& f0, const PairIO& f1, int
                 hipStream_t s) {
   constexpr int RL = 2 * 64 / N2;
   LineGeom g = geom(a, prec, RL);
-  const size_t smem = (size_t)64 * N1 * 2 * sizeof(T);
+  // transpose buffer (64 N1 complex) or the staged rows (RL (n + N2/2) reals)
+  const size_t tbuf = (size_t)64 * N1 * 2 * sizeof(T);
+  const size_t sbuf = (size_t)RL * (N1 * N2 + N2 / 2) * sizeof(T);
+  const size_t smem = tbuf > sbuf ? tbuf : sbuf;
   dim3 grid((a.nlines + RL - 1) / RL, nfields);
   if (a.contig)
     hipLaunchKernelGGL((k_line_reg<T, N1, N2, true, false>), grid, dim3(64), smem, s, g, f0, f1);
@@ -243,6 +246,9 @@ psx_status launch_pair(const Axis& a, int prec, const PairIO& f0, const PairIO& 
       case 4: launch_lane<T, 4>(a, prec, f0, f1, nfields, s); break;
       case 5: launch_lane<T, 5>(a, prec, f0, f1, nfields, s); break;
       case 8: launch_lane<T, 8>(a, prec, f0, f1, nfields, s); break;
+      // 8 lanes per line where the length allows (8 complex lines per wave:
+      // 128-byte rows in FP64); fewer points per lane keep more waves in
+      // flight, which matters more here than FP32 row width.
       case 16: launch_reg<T, 4, 4>(a, prec, f0, f1, nfields, s); break;
       case 32: launch_reg<T, 8, 4>(a, prec, f0, f1, nfields, s); break;
       case 48: launch_reg<T, 12, 4>(a, prec, f0, f1, nfields, s); break;
