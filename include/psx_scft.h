@@ -165,6 +165,29 @@ psx_status psx_scft_free_energy(psx_scft* e, double* h_F);
  * analytic stress).  Requires a preceding propagate. */
 psx_status psx_scft_box_stress(psx_scft* e, double* h_dFdL);
 
+/* Contour quadrature of the densities: 0 = the reference's Simpson rule
+ * with the literal 0.33333 and a trapezoid tail (pspolymer/PsBlock.cpp:
+ * 256-380; the default), 1 = trapezoid weights ds [1/2, 1, ..., 1, 1/2],
+ * the exact derivative of the split-step Q with respect to w, under which
+ * the discrete free energy is stationary at the fixed point and the box
+ * stress is its exact total derivative.  Mode 1 has no counterpart in the
+ * reference. */
+psx_status psx_scft_set_quadrature(psx_scft* e, int mode);
+
+/* Change the cell sizes (box lengths / grid counts) in place: rewrites the
+ * exp(-ds b^2 k^2) tables (PsFlexPseudoSpec::build_k2 for the new
+ * cellSizes); fields are kept.  Used by the box relaxation. */
+psx_status psx_scft_set_cell(psx_scft* e, double dx, double dy, double dz);
+
+/* Derivatives of the free energy along the fixed points of the field
+ * update (evaluate at a converged state): dF/dchiN = mean(phiA phiB) - fA fB
+ * and dF/dfA = -(1/Q) mean(q(fA) q+(fA) (wB - wA)), the derivative with
+ * respect to the junction position (the discrete F moves in whole contour
+ * steps, so finite differences agree to O(ds^2)).  Requires a preceding
+ * propagate and densities.  No reference counterpart. */
+psx_status psx_scft_param_grad(psx_scft* e, double* h_dFdchiN,
+                               double* h_dFdfA);
+
 /* Average device time (ms) of one contour step of the dual sweep and of
  * each of its line passes (z, y, x), timed with hipEvents on the engine's
  * stream over `reps` repetitions. */

@@ -34,6 +34,7 @@ class OrcParams(ctypes.Structure):
         ("depth", ctypes.c_int), ("beta", ctypes.c_double),
         ("anderson_start", ctypes.c_double),
         ("f_order", ctypes.c_int),
+        ("quad", ctypes.c_int),
     ]
 
 
@@ -58,6 +59,7 @@ class Params:
     beta: float = 0.1
     anderson_start: float = 1e300
     f_order: int = 0        # 0 reference (F after update), 1 consistent
+    quad: int = 0           # 0 Simpson 0.33333 (reference), 1 trapezoid
 
     @property
     def N(self) -> int:
@@ -71,7 +73,7 @@ class Params:
         return OrcParams(self.nx, self.ny, self.nz, self.dx, self.dy, self.dz,
                          self.ds, self.fA, self.chiN, self.bA, self.bB, self.mix,
                          self.lambda0, self.lambda1, self.depth, self.beta,
-                         self.anderson_start, self.f_order)
+                         self.anderson_start, self.f_order, self.quad)
 
 
 def lib():

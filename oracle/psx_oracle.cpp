@@ -302,6 +302,21 @@ extern "C" void orc_quadrature(long N, int steps, double ds, const double* qs,
   for (long r = 0; r < N; ++r) out[r] *= iq;
 }
 
+extern "C" void orc_quadrature_trap(long N, int steps, double ds, const double* qs,
+                                    const double* qts, double Q, double* out) {
+  // d(-ln Q)/dw(r) of the split-step sweep: each step applies exp(-ds w/2)
+  // at both of its ends, so contour point j gets ds/2 from each adjacent step.
+  const long M = steps + 1;
+  for (long r = 0; r < N; ++r) out[r] = 0.0;
+  for (long j = 0; j < M; ++j) {
+    const double c = (j == 0 || j == M - 1) ? 0.5 * ds : ds;
+    const double* q = qs + j * N;
+    const double* t = qts + (M - 1 - j) * N;
+    for (long r = 0; r < N; ++r) out[r] += c * q[r] * t[r];
+  }
+  for (long r = 0; r < N; ++r) out[r] /= Q;
+}
+
 extern "C" int orc_diblock_densities(const orc_params* P, const double* wA,
                                      const double* wB, double* phiA,
                                      double* phiB, double* Qout,
@@ -339,8 +354,13 @@ extern "C" int orc_diblock_densities(const orc_params* P, const double* wA,
   if (Q != Q) return -2;
   *Qout = Q;
   // densities: vf = 1, lengthRatio = 1, densWt = 1 (pspolymer/PsBlock.cpp:154-196)
-  orc_quadrature(N, sA, P->ds, qA, qtA, Q, phiA);
-  orc_quadrature(N, sB, P->ds, qB, qtB, Q, phiB);
+  if (P->quad == 1) {
+    orc_quadrature_trap(N, sA, P->ds, qA, qtA, Q, phiA);
+    orc_quadrature_trap(N, sB, P->ds, qB, qtB, Q, phiB);
+  } else {
+    orc_quadrature(N, sA, P->ds, qA, qtA, Q, phiA);
+    orc_quadrature(N, sB, P->ds, qB, qtB, Q, phiB);
+  }
   return 0;
 }
 

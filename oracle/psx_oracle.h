@@ -49,6 +49,7 @@ typedef struct orc_params {
   double beta;           /* Anderson mixing parameter */
   double anderson_start; /* use Picard while residual RMS > this */
   int f_order;           /* 0: F after the w update (reference); 1: consistent */
+  int quad;              /* 0: Simpson 0.33333 (reference); 1: trapezoid */
 } orc_params;
 
 /* Thread count used by the FFT line loops and element-wise loops (OpenMP). */
@@ -81,6 +82,11 @@ int orc_solve_q(int nx, int ny, int nz, const double* kfac, const double* wfac,
  * (pspolymer/PsBlock.cpp:256-380).  qs, qts hold M = steps+1 slices. */
 void orc_quadrature(long n, int steps, double ds, const double* qs,
                     const double* qts, double Q, double* out);
+
+/* Trapezoid contour rule ds [1/2, 1, ..., 1, 1/2] (no reference counterpart:
+ * the exact derivative of the split-step Q with respect to w). */
+void orc_quadrature_trap(long n, int steps, double ds, const double* qs,
+                         const double* qts, double Q, double* out);
 
 /* Full propagation of a diblock: fills densities and Q.  If slices != NULL it
  * receives qA[0..sA], qB[0..sB], qtB[0..sB], qtA[0..sA] in that order. */

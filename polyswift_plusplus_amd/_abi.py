@@ -70,6 +70,9 @@ EXPORTS = {
     "psx_scft_reset_history": (ctypes.c_int, [_vp]),
     "psx_scft_free_energy": (ctypes.c_int, [_vp, _dp]),
     "psx_scft_box_stress": (ctypes.c_int, [_vp, _dp]),
+    "psx_scft_set_quadrature": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "psx_scft_set_cell": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+    "psx_scft_param_grad": (ctypes.c_int, [_vp, _dp, _dp]),
     "psx_scft_time_step": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
 }
 

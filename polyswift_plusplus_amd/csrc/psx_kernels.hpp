@@ -559,3 +559,104 @@ __global__ void k_state_set_call(DevState* st) { st->call_iter = 0; }
 __global__ void k_state_reset_hist(DevState* st) { st->since_reset = 0; }
 
 }  // namespace psx
+
+namespace psx {
+
+// ------------------------------------------------------------------------
+// Box stress (adjoint of the split-step sweep).  For chain step t with
+// w-factor W and kfac K = Kx Ky Kz, the partition function obeys
+//   N Q = <q+_{Ns-1-t}, W F^-1 K F W q_t>,
+// so dQ/dL_d = (1/N^2) sum_t sum_k conj(B) A dK/dL_d with A = F(W q_t),
+// B = F(W q+_{Ns-1-t}) and dK/dL_d = K ds b^2 2 k_d^2 / L_d.  Both spectra
+// come from one complex FFT of Z = W q_t + i W q+_{Ns-1-t}; for an even
+// weight M, sum_k M conj(B) A = (1/2) sum_k M Im(Z(k) Z(-k)).
+
+// Z = (W F[t], W G[Ns-1-t]) as FP64 complex.
+template <typename T>
+__global__ void k_stress_pack(const T* __restrict__ f, const T* __restrict__ g,
+                              const T* __restrict__ w, long long N, double2* __restrict__ z) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double ww = (double)w[i];
+    double2 v;
+    v.x = ww * (double)f[i];
+    v.y = ww * (double)g[i];
+    z[i] = v;
+  }
+}
+
+struct StressArgs {
+  const double2* z;
+  int nx, ny, nz;
+  const double* kf[3];    // exp(-ds b^2 k_d^2) per axis index (block of step t)
+  const double* kd2[3];   // k_d^2 per axis index
+  double* part;           // [gridDim.x][3]
+};
+
+__global__ void __launch_bounds__(kThreads)
+k_stress_reduce(StressArgs a) {
+  const long long N = (long long)a.nx * a.ny * a.nz;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < N;
+       r += (long long)gridDim.x * blockDim.x) {
+    const int k = (int)(r % a.nz);
+    const long long q = r / a.nz;
+    const int j = (int)(q % a.ny);
+    const int i = (int)(q / a.ny);
+    const int mi = (a.nx - i) % a.nx, mj = (a.ny - j) % a.ny, mk = (a.nz - k) % a.nz;
+    const double2 zr = a.z[r];
+    const double2 zm = a.z[((long long)mi * a.ny + mj) * a.nz + mk];
+    const double im = zr.x * zm.y + zr.y * zm.x;
+    const double K = a.kf[0][i] * a.kf[1][j] * a.kf[2][k];
+    const double s = K * im;
+    acc[0] += s * a.kd2[0][i];
+    acc[1] += s * a.kd2[1][j];
+    acc[2] += s * a.kd2[2][k];
+  }
+  block_sum<3>(acc);
+  if (threadIdx.x == 0) {
+    a.part[blockIdx.x * 3 + 0] = acc[0];
+    a.part[blockIdx.x * 3 + 1] = acc[1];
+    a.part[blockIdx.x * 3 + 2] = acc[2];
+  }
+}
+
+// acc[d] += scale * sum of partials (scale = ds b^2 of the step's block).
+__global__ void k_stress_accum(const double* __restrict__ part, int nb, double scale,
+                               double* acc) {
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    v[0] += part[i * 3 + 0];
+    v[1] += part[i * 3 + 1];
+    v[2] += part[i * 3 + 2];
+  }
+  block_sum<3>(v);
+  if (threadIdx.x == 0) {
+    acc[0] += scale * v[0];
+    acc[1] += scale * v[1];
+    acc[2] += scale * v[2];
+  }
+}
+
+// Parameter derivatives at fixed fields: partials of
+//   [0] sum phiA phiB                       (dF/dchiN)
+//   [1] sum F[sA] G[Ns-sA] (wB - wA)        (junction term of dF/dfA)
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_param_grad(const double* __restrict__ phiA, const double* __restrict__ phiB,
+             const T* __restrict__ fj, const T* __restrict__ gj, const double* __restrict__ wA,
+             const double* __restrict__ wB, long long N, double* __restrict__ part) {
+  double acc[2] = {0.0, 0.0};
+  for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < N;
+       r += (long long)gridDim.x * blockDim.x) {
+    acc[0] += phiA[r] * phiB[r];
+    acc[1] += (double)fj[r] * (double)gj[r] * (wB[r] - wA[r]);
+  }
+  block_sum<2>(acc);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x * 2 + 0] = acc[0];
+    part[blockIdx.x * 2 + 1] = acc[1];
+  }
+}
+
+}  // namespace psx

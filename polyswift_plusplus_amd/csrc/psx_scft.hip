@@ -456,6 +456,12 @@ struct psx_scft {
   bool swept = false;            // slices valid for the current fields
   bool dens_valid = false;       // phi valid for the current slices
   std::map<GraphKey, hipGraphExec_t> graphs;
+  // box stress: FP64 complex work grid, per-axis k^2 and kfac tables
+  void* zbuf = nullptr;
+  double* kd2[3] = {nullptr, nullptr, nullptr};
+  double* kf1[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  double* sacc = nullptr;
+  std::vector<Axis> caxes;     // axes of the complex stress transforms
   // cached host-visible scratch for the host API
   void* hstage = nullptr;
 };
@@ -716,6 +722,10 @@ extern "C" psx_status psx_scft_destroy(psx_scft* e) {
   dfree((void*&)e->phiA); dfree((void*&)e->phiB); dfree((void*&)e->cw);
   dfree((void*&)e->part); dfree((void*&)e->st); dfree((void*&)e->hist);
   dfree((void*&)e->ringx); dfree((void*&)e->ringf); dfree(e->hstage);
+  dfree(e->zbuf); dfree((void*&)e->sacc);
+  for (auto& p : e->kd2) dfree((void*&)p);
+  for (auto& b : e->kf1) for (auto& p : b) dfree((void*&)p);
+  for (auto& a : e->caxes) { dfree(a.tw[0]); dfree(a.tw[1]); }
   if (e->own_stream && e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
   return PSX_OK;
@@ -734,6 +744,15 @@ static int block_steps(double len, double ds) {
   else r = f;
   if (r < 4) return -2;
   return (int)r;
+}
+
+// Trapezoid weights ds [1/2, 1, ..., 1, 1/2]: the exact derivative of the
+// split-step Q with respect to w(r), since each step's two w-factor
+// half-multiplies touch the contour points at both of its ends.
+static void trap_weights(int steps, double ds, double* w) {
+  for (int j = 0; j <= steps; ++j) w[j] = ds;
+  w[0] = 0.5 * ds;
+  w[steps] = 0.5 * ds;
 }
 
 // Simpson (literal 0.33333) with a trapezoid tail for an even point count,
@@ -809,6 +828,23 @@ extern "C" psx_status psx_scft_create(const psx_grid* g, const psx_diblock* c,
 #undef E_TRY
   if (st != PSX_OK) { psx_scft_destroy(e); return st; }
   *out = e;
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_scft_set_quadrature(psx_scft* e, int mode) {
+  if (!e) return fail(PSX_ERR_INVALID, "null engine");
+  if (mode != 0 && mode != 1) return fail(PSX_ERR_INVALID, "quadrature mode must be 0 or 1");
+  std::vector<double> w(e->Ns + 2);
+  if (mode == 0) {
+    quad_weights(e->sA, e->ds, w.data());
+    quad_weights(e->sB, e->ds, w.data() + e->sA + 1);
+  } else {
+    trap_weights(e->sA, e->ds, w.data());
+    trap_weights(e->sB, e->ds, w.data() + e->sA + 1);
+  }
+  PSX_HIP(hipStreamSynchronize(e->stream));
+  PSX_HIP(hipMemcpy(e->cw, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
+  e->dens_valid = false;
   return PSX_OK;
 }
 
@@ -1078,9 +1114,166 @@ extern "C" psx_status psx_scft_iterate(psx_scft* e, const psx_mixing* mix, int n
   return result;
 }
 
+// (Re)write the spectral tables of precision `prec` for the current cell.
+static psx_status write_emul(psx_scft* e, int prec) {
+  SweepBufs& s = e->sw[prec];
+  for (size_t i = 0; i < e->axes.size(); ++i) {
+    const Axis& a = e->axes[i];
+    for (int blk = 0; blk < 2; ++blk) {
+      const double b = blk == 0 ? e->bA : e->bB;
+      const double h = a.n == 1 ? 1.0 : e->cell[a.axis];
+      std::vector<double> em = axis_emul(a, h, e->ds, b);
+      if (prec == PSX_FP64) {
+        PSX_HIP(hipMemcpy(s.emul[blk][i], em.data(), em.size() * 8, hipMemcpyHostToDevice));
+      } else {
+        std::vector<float> f = cast_vec<float>(em);
+        PSX_HIP(hipMemcpy(s.emul[blk][i], f.data(), f.size() * 4, hipMemcpyHostToDevice));
+      }
+    }
+  }
+  return PSX_OK;
+}
+
+static psx_status write_stress_tables(psx_scft* e, bool alloc) {
+  const double twopi = 2.0 * std::acos(-1.0);
+  for (int d = 0; d < 3; ++d) {
+    const int n = e->dims[d];
+    std::vector<double> k2(n), kfA(n), kfB(n);
+    const double n2 = (double)n / 2.0;
+    for (int i = 0; i < n; ++i) {
+      const double nk = n2 - std::fabs((double)i - n2);
+      const double kv = twopi * nk / (double)n / e->cell[d];
+      k2[i] = kv * kv;
+      kfA[i] = std::exp(-1.0 * e->ds * k2[i] * e->bA * e->bA);
+      kfB[i] = std::exp(-1.0 * e->ds * k2[i] * e->bB * e->bB);
+    }
+    if (alloc) {
+      PSX_TRY(upload((void**)&e->kd2[d], k2.data(), n * 8));
+      PSX_TRY(upload((void**)&e->kf1[0][d], kfA.data(), n * 8));
+      PSX_TRY(upload((void**)&e->kf1[1][d], kfB.data(), n * 8));
+    } else {
+      PSX_HIP(hipMemcpy(e->kd2[d], k2.data(), n * 8, hipMemcpyHostToDevice));
+      PSX_HIP(hipMemcpy(e->kf1[0][d], kfA.data(), n * 8, hipMemcpyHostToDevice));
+      PSX_HIP(hipMemcpy(e->kf1[1][d], kfB.data(), n * 8, hipMemcpyHostToDevice));
+    }
+  }
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_scft_set_cell(psx_scft* e, double dx, double dy, double dz) {
+  if (!e) return fail(PSX_ERR_INVALID, "null engine");
+  if (!(dx > 0 && dy > 0 && dz > 0)) return fail(PSX_ERR_INVALID, "cell sizes must be > 0");
+  PSX_HIP(hipStreamSynchronize(e->stream));
+  e->cell[0] = dx; e->cell[1] = dy; e->cell[2] = dz;
+  for (int p = 0; p < 2; ++p)
+    if (e->sw[p].ready) PSX_TRY(write_emul(e, p));
+  if (e->zbuf) PSX_TRY(write_stress_tables(e, false));
+  e->swept = false;
+  e->dens_valid = false;
+  return PSX_OK;
+}
+
+static psx_status ensure_stress(psx_scft* e) {
+  if (e->zbuf) return PSX_OK;
+  PSX_TRY(dalloc(&e->zbuf, (size_t)e->N * 16));
+  PSX_TRY(dalloc((void**)&e->sacc, 4 * sizeof(double)));
+  PSX_TRY(make_axes(e->dims, &e->caxes));
+  for (auto& a : e->caxes) PSX_TRY(axis_tw(a, PSX_FP64));
+  PSX_TRY(write_stress_tables(e, true));
+  const int need = e->nb * 3;
+  if (need > e->part_cap) {
+    dfree((void*&)e->part);
+    PSX_TRY(dalloc((void**)&e->part, need * sizeof(double)));
+    e->part_cap = need;
+  }
+  return PSX_OK;
+}
+
+template <typename T>
+static psx_status stress_sweep(psx_scft* e) {
+  SweepBufs& s = e->sw[e->prec];
+  const size_t tb = sizeof(T);
+  const int g = grid_1d(e->N);
+  PSX_HIP(hipMemsetAsync(e->sacc, 0, 4 * sizeof(double), e->stream));
+  for (int t = 0; t < e->Ns; ++t) {
+    const bool inA = t < e->sA;
+    const T* f = (const T*)((const char*)s.F + (size_t)t * e->N * tb);
+    const T* gg = (const T*)((const char*)s.G + (size_t)(e->Ns - 1 - t) * e->N * tb);
+    const T* w = (const T*)(inA ? s.wfA : s.wfB);
+    hipLaunchKernelGGL(k_stress_pack<T>, dim3(g), dim3(kThreads), 0, e->stream, f, gg, w, e->N,
+                       (double2*)e->zbuf);
+    for (auto& a : e->caxes) {
+      CplxIO io{};
+      io.in = e->zbuf; io.out = e->zbuf; io.kmul = nullptr;
+      io.in_real = 0; io.out_real = 0; io.do_fwd = 1; io.do_inv = 0;
+      PSX_TRY(launch_cplx<double>(a, PSX_FP64, io, e->stream));
+    }
+    StressArgs sa;
+    sa.z = (const double2*)e->zbuf;
+    sa.nx = e->dims[0]; sa.ny = e->dims[1]; sa.nz = e->dims[2];
+    for (int d = 0; d < 3; ++d) { sa.kf[d] = e->kf1[inA ? 0 : 1][d]; sa.kd2[d] = e->kd2[d]; }
+    sa.part = e->part;
+    hipLaunchKernelGGL(k_stress_reduce, dim3(e->nb), dim3(kThreads), 0, e->stream, sa);
+    const double b = inA ? e->bA : e->bB;
+    hipLaunchKernelGGL(k_stress_accum, dim3(1), dim3(kThreads), 0, e->stream, (const double*)e->part,
+                       e->nb, e->ds * b * b, e->sacc);
+    PSX_HIP(hipGetLastError());
+  }
+  return PSX_OK;
+}
+
 extern "C" psx_status psx_scft_box_stress(psx_scft* e, double* dFdL) {
   if (!e || !dFdL) return fail(PSX_ERR_INVALID, "null argument");
-  return fail(PSX_ERR_STATE, "box stress not built in this version");
+  if (!e->swept) return fail(PSX_ERR_STATE, "propagate before the box stress");
+  PSX_TRY(ensure_stress(e));
+  double Q;
+  PSX_TRY(psx_scft_Q(e, &Q));
+  if (e->prec == PSX_FP64) PSX_TRY(stress_sweep<double>(e));
+  else PSX_TRY(stress_sweep<float>(e));
+  double acc[3];
+  PSX_HIP(hipMemcpyAsync(acc, e->sacc, 3 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+  PSX_HIP(hipStreamSynchronize(e->stream));
+  const double N = (double)e->N;
+  for (int d = 0; d < 3; ++d) {
+    const double L = e->dims[d] * e->cell[d];
+    const double dQ = acc[d] / L / N / N;
+    dFdL[d] = e->dims[d] > 1 ? -dQ / Q : 0.0;
+  }
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_scft_param_grad(psx_scft* e, double* dFdchiN, double* dFdfA) {
+  if (!e) return fail(PSX_ERR_INVALID, "null argument");
+  if (!e->dens_valid) return fail(PSX_ERR_STATE, "densities before the parameter gradient");
+  if (!e->swept) return fail(PSX_ERR_STATE, "propagate before the parameter gradient");
+  double Q;
+  PSX_TRY(psx_scft_Q(e, &Q));
+  SweepBufs& s = e->sw[e->prec];
+  const size_t tb = prec_bytes(e->prec);
+  const void* fj = (const char*)s.F + (size_t)e->sA * e->N * tb;
+  const void* gj = (const char*)s.G + (size_t)(e->Ns - e->sA) * e->N * tb;
+  if (e->prec == PSX_FP64)
+    hipLaunchKernelGGL(k_param_grad<double>, dim3(e->nb), dim3(kThreads), 0, e->stream,
+                       (const double*)e->phiA, (const double*)e->phiB, (const double*)fj,
+                       (const double*)gj, (const double*)e->wA, (const double*)e->wB, e->N, e->part);
+  else
+    hipLaunchKernelGGL(k_param_grad<float>, dim3(e->nb), dim3(kThreads), 0, e->stream,
+                       (const double*)e->phiA, (const double*)e->phiB, (const float*)fj,
+                       (const float*)gj, (const double*)e->wA, (const double*)e->wB, e->N, e->part);
+  PSX_HIP(hipGetLastError());
+  std::vector<double> h((size_t)e->nb * 2);
+  PSX_HIP(hipMemcpyAsync(h.data(), e->part, h.size() * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+  PSX_HIP(hipStreamSynchronize(e->stream));
+  double pp = 0.0, jt = 0.0;
+  for (int b = 0; b < e->nb; ++b) { pp += h[2 * b]; jt += h[2 * b + 1]; }
+  const double N = (double)e->N;
+  if (dFdchiN) *dFdchiN = pp / N - e->fA * e->fB;
+  // Along the fixed points of the field update, the explicit
+  // -chiN (fB - fA) of the disorder term cancels against the density terms
+  // chiN fB d<phiA>/dfA + chiN fA d<phiB>/dfA (the update shifts densities
+  // by fA, fB), leaving the junction term of -ln Q.
+  if (dFdfA) *dFdfA = -(jt / N) / Q;
+  return PSX_OK;
 }
 
 extern "C" psx_status psx_scft_time_step(psx_scft* e, int reps, double* ms_step, double* ms_pass) {

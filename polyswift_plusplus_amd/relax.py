@@ -1,0 +1,72 @@
+"""Solvers built on the engine: converge an SCFT saddle point, and relax the
+box lengths to zero stress (cell optimisation).
+
+The reference has neither (it runs a fixed number of steepest-descent
+steps, polyswift/polyswift.cxx:544, and has no stress); both are new.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .scft import DiblockSCFT, Mixing
+
+# Anderson settings that converge the diblock workloads here: Picard with
+# lambda0 = 0.2 until the residual RMS is below 0.05, then depth-10 Anderson
+# with beta = 1.
+DEFAULT_MIXING = Mixing("anderson", lambda0=0.2, depth=10, beta=1.0, anderson_start=0.05)
+
+
+def converge(e: DiblockSCFT, tol: float = 1e-10, max_iters: int = 5000,
+             mixing: Mixing | None = None) -> dict:
+    """Iterate to residual RMS < tol, then sweep once more at the final
+    fields so that slices, Q and densities belong to them."""
+    h = e.iterate(max_iters, mixing or DEFAULT_MIXING, tol=tol)
+    e.update()
+    return h
+
+
+def mixed_precision_solve(e: DiblockSCFT, tol: float = 1e-8, switch_tol: float = 2e-5,
+                          max_iters: int = 5000, mixing: Mixing | None = None) -> dict:
+    """FP32 sweeps until the residual RMS falls below switch_tol (the FP32
+    noise floor is near 1e-6), then FP64 iterations to tol.  Q, the
+    quadrature, the update and F are FP64 throughout."""
+    mix = mixing or DEFAULT_MIXING
+    e.setPrecision("fp32")
+    h32 = e.iterate(max_iters, mix, tol=switch_tol)
+    e.setPrecision("fp64")
+    e.resetHistory()
+    h64 = e.iterate(max_iters, mix, tol=tol)
+    return dict(fp32_iters=h32["iters"], fp64_iters=h64["iters"], F=h64["F"][-1] if h64["iters"] else None,
+                err=h64["err"][-1] if h64["iters"] else None, h32=h32, h64=h64)
+
+
+def relax_box(e: DiblockSCFT, tol_stress: float = 1e-7, tol_scft: float = 1e-11,
+              max_outer: int = 40, first_step: float = 5.0, axes=None,
+              mixing: Mixing | None = None) -> dict:
+    """Relax box lengths L_d = n_d * cell_d to dF/dL_d = 0 by a secant step
+    per axis (the orthorhombic cell keeps its angles).  Each outer step
+    converges the fields from the previous ones, then takes the stress from
+    the adjoint of the converged sweep."""
+    n = np.array(e.shape, dtype=float)
+    cell = np.array(e.cell, dtype=float)
+    active = np.array([s > 1 for s in e.shape]) if axes is None else np.array(axes)
+    hist = []
+    L_prev = g_prev = None
+    for outer in range(max_outer):
+        converge(e, tol=tol_scft, mixing=mixing)
+        g = e.boxStress()
+        F = e.getFreeE()
+        L = n * cell
+        hist.append(dict(L=L.copy(), dFdL=g.copy(), F=F))
+        if np.max(np.abs(g[active])) < tol_stress:
+            break
+        step = np.zeros(3)
+        for d in np.where(active)[0]:
+            if L_prev is None or g[d] == g_prev[d]:
+                step[d] = -first_step * g[d]
+            else:
+                step[d] = -g[d] * (L[d] - L_prev[d]) / (g[d] - g_prev[d])
+        L_prev, g_prev = L, g
+        cell = (L + step) / n
+        e.setCell(cell)
+    return dict(history=hist, L=n * cell, converged=bool(np.max(np.abs(hist[-1]["dFdL"][active])) < tol_stress))

@@ -100,7 +100,8 @@ class Mixing:
 class DiblockSCFT:
     """Bulk AB diblock melt on an nx*ny*nz periodic grid (one HIP device)."""
 
-    def __init__(self, shape, cell, fA, ds, chiN, bA=1.0, bB=1.0, prec="fp64", stream=None):
+    def __init__(self, shape, cell, fA, ds, chiN, bA=1.0, bB=1.0, prec="fp64", stream=None,
+                 quadrature="reference"):
         self._L = _abi.load()
         self.shape = tuple(int(s) for s in shape)
         self.cell = tuple(float(c) for c in cell)
@@ -116,6 +117,8 @@ class DiblockSCFT:
         check(self._L.psx_scft_block_steps(h, ctypes.byref(sA), ctypes.byref(sB)))
         self.sA, self.sB = sA.value, sB.value
         self.Ns = self.sA + self.sB
+        if quadrature != "reference":
+            self.setQuadrature(quadrature)
 
     # -- fields -------------------------------------------------------------
     def setFields(self, wA, wB, p=None) -> None:
@@ -200,6 +203,23 @@ class DiblockSCFT:
         out = np.zeros(3)
         check(self._L.psx_scft_box_stress(self._h, _ptr(out)))
         return out
+
+    def setQuadrature(self, mode) -> None:
+        """'reference' (Simpson 0.33333, default) or 'trapezoid' (split-step
+        consistent: the discrete F is stationary at the fixed point)."""
+        m = {"reference": 0, "simpson": 0, "trapezoid": 1, "consistent": 1}.get(mode, mode)
+        check(self._L.psx_scft_set_quadrature(self._h, int(m)))
+
+    def setCell(self, cell) -> None:
+        """New cell sizes (box / grid); fields are kept."""
+        self.cell = tuple(float(c) for c in cell)
+        check(self._L.psx_scft_set_cell(self._h, *self.cell))
+
+    def paramGrad(self):
+        """(dF/dchiN, dF/dfA) at fixed fields (after update() or iterate())."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        check(self._L.psx_scft_param_grad(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def timeStep(self, reps: int = 20):
         """(ms per dual contour step, [ms per z, y, x pass])."""
