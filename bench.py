@@ -114,6 +114,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--grid", type=int, default=128)
+    ap.add_argument("--profile", action="store_true",
+                    help="short run for rocprofv3 counters: the dual contour step of "
+                         "both precisions (5 reps each) and one FP64 copy of known size")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -126,6 +129,20 @@ def main():
     w = dict(WORKLOAD)
     w["shape"] = (args.grid,) * 3
     N = int(np.prod(w["shape"]))
+    if args.profile:
+        # Counter calibration: one k_to_f64<double> over N cells moves exactly
+        # 8N bytes in and 8N bytes out (psx_scft_get_slice_host on FP64).
+        out = {}
+        for prec in ("fp64", "fp32"):
+            e = make_engine(prec, w)
+            step_ms, pass_ms = e.timeStep(5)
+            if prec == "fp64":
+                e.slice(False, 1)
+            out[prec] = dict(step_us=step_ms * 1e3, pass_us=[p * 1e3 for p in pass_ms])
+            e.close()
+        out["calibration_copy_bytes_each_way"] = 8 * N
+        print(json.dumps(out))
+        return
     mix = Mixing("anderson", lambda0=0.1, depth=args.depth, beta=0.1, anderson_start=1e300)
 
     res = {}

@@ -101,8 +101,10 @@ k_line_reg(LineGeom g, PairIO f0, PairIO f1) {
   static_assert(N1 % N2 == 0 && 64 % N2 == 0, "bad split");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C* lds = (C*)smem_raw;
+  int tile, field;
+  xcd_tile(g.ntiles, g.nfields, &tile, &field);
 
-  const bool second = blockIdx.y != 0;
+  const bool second = field != 0;
   const T* __restrict__ in = (const T*)(second ? f1.in : f0.in);
   T* __restrict__ out = (T*)(second ? f1.out : f0.out);
   const T* __restrict__ pre = (const T*)(second ? f1.pre : f0.pre);
@@ -112,7 +114,7 @@ k_line_reg(LineGeom g, PairIO f0, PairIO f1) {
 
   const int lane = threadIdx.x;
   const int p = lane / N2, t = lane % N2;
-  const int la = blockIdx.x * RL + 2 * p;   // real lines la, la + 1
+  const int la = tile * RL + 2 * p;   // real lines la, la + 1
   const bool va = la < g.nlines, vb = la + 1 < g.nlines;
   const int inner = g.inner;
   // element k of line l: base(l) + k * stride
@@ -134,7 +136,7 @@ k_line_reg(LineGeom g, PairIO f0, PairIO f1) {
   constexpr int VW = 16 / sizeof(T);             // elements per 16-byte chunk
   if (CONTIG) {
     T* S = (T*)smem_raw;
-    const int row0 = blockIdx.x * RL;
+    const int row0 = tile * RL;
     const int nrows = min(RL, g.nlines - row0);
     const T* src = in + (size_t)row0 * n;
     const T* psrc = pre ? pre + (size_t)row0 * n : nullptr;
@@ -223,7 +225,7 @@ k_line_reg(LineGeom g, PairIO f0, PairIO f1) {
 
   if (CONTIG) {
     T* S = (T*)smem_raw;
-    const int row0 = blockIdx.x * RL;
+    const int row0 = tile * RL;
     const int nrows = min(RL, g.nlines - row0);
     __syncthreads();
 #pragma unroll
@@ -267,14 +269,16 @@ template <typename T, int NN, bool CONTIG>
 __global__ void __launch_bounds__(64)
 k_line_lane(LineGeom g, PairIO f0, PairIO f1) {
   typedef typename Cx<T>::type C;
-  const bool second = blockIdx.y != 0;
+  int tile, field;
+  xcd_tile(g.ntiles, g.nfields, &tile, &field);
+  const bool second = field != 0;
   const T* __restrict__ in = (const T*)(second ? f1.in : f0.in);
   T* __restrict__ out = (T*)(second ? f1.out : f0.out);
   const T* __restrict__ pre = (const T*)(second ? f1.pre : f0.pre);
   const T* __restrict__ post = (const T*)(second ? f1.post : f0.post);
   const T* __restrict__ emul = (const T*)(second ? f1.emul : f0.emul);
   const C* __restrict__ tw = (const C*)g.tw;
-  const int la = (blockIdx.x * 64 + threadIdx.x) * 2;
+  const int la = (tile * 64 + threadIdx.x) * 2;
   const bool va = la < g.nlines, vb = la + 1 < g.nlines;
   const int inner = g.inner;
   int ba, bb, stride;

@@ -22,7 +22,22 @@ struct LineGeom {
   int L;              // real lines per tile (even) / complex lines per tile
   int ls;             // LDS line stride in complex words (n + 1)
   const void* tw;     // n twiddles exp(-2 pi i j/n), type Cx<T>
+  int ntiles;         // tiles per field; the 1D grid has ntiles * nfields blocks
+  int nfields;
 };
+
+// XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
+// XCDs (block b runs on XCD b % 8), each with its own L2.  Giving every XCD
+// one contiguous run of tiles keeps neighbouring tiles -- which share 128-byte
+// lines of strided rows and finish at about the same time -- in one L2.
+__device__ __forceinline__ void xcd_tile(int ntiles, int nfields, int* tile, int* field) {
+  const int b = blockIdx.x;
+  const int total = ntiles * nfields;
+  const int xcd = b & 7, loc = b >> 3, per = total >> 3, rem = total & 7;
+  const int logical = xcd * per + (xcd < rem ? xcd : rem) + loc;
+  *field = logical / ntiles;
+  *tile = logical - *field * ntiles;
+}
 
 // One field of a pair round-trip pass.
 struct PairIO {
@@ -41,14 +56,16 @@ __device__ __forceinline__ int line_addr(int l, int k, int n, int inner) {
 //   out = post * IFFT_n( emul * FFT_n( pre * in ) )
 // Two real lines share one complex line (line 2p -> real part, 2p+1 ->
 // imaginary part); the operator is real and even in k, so the two results
-// come back in the real and imaginary parts unchanged.  blockIdx.y selects
+// come back in the real and imaginary parts unchanged.  The field index selects
 // one of two independent fields (the two contour sweeps run in one launch).
 template <typename T, bool CONTIG>
 __global__ void __launch_bounds__(kThreads)
 k_pair_roundtrip(LineGeom g, PairIO f0, PairIO f1) {
   typedef typename Cx<T>::type C;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const bool second = blockIdx.y != 0;
+  int tile, field;
+  xcd_tile(g.ntiles, g.nfields, &tile, &field);
+  const bool second = field != 0;
   const T* __restrict__ in = (const T*)(second ? f1.in : f0.in);
   T* __restrict__ out = (T*)(second ? f1.out : f0.out);
   const T* __restrict__ pre = (const T*)(second ? f1.pre : f0.pre);
@@ -60,7 +77,7 @@ k_pair_roundtrip(LineGeom g, PairIO f0, PairIO f1) {
   C* A = (C*)smem_raw;
   C* B = A + P * ls;
   const int tid = threadIdx.x, nt = blockDim.x;
-  const int l0 = blockIdx.x * L;
+  const int l0 = tile * L;
   const int tot = L * n;
 
   T* Ar = (T*)A;
@@ -327,6 +344,7 @@ k_quad_update(QuadArgs a) {
       phB = a.phiB[r];
     } else {
       double qa = 0.0, qb = 0.0;
+#pragma unroll 8
       for (int t = 0; t < sA; ++t)
         qa += cw[t] * ((double)F[(long long)t * N + r] * (double)G[(long long)(Ns - t) * N + r]);
       {
@@ -334,6 +352,7 @@ k_quad_update(QuadArgs a) {
         qa += cw[sA] * j;
         qb += cw[sA + 1] * j;
       }
+#pragma unroll 8
       for (int t = sA + 1; t <= Ns; ++t)
         qb += cw[t + 1] * ((double)F[(long long)t * N + r] * (double)G[(long long)(Ns - t) * N + r]);
       phA = qa * invQ * a.dens;

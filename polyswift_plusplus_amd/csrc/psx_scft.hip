@@ -176,6 +176,8 @@ LineGeom geom(const Axis& a, int prec, int L) {
   g.L = L;
   g.ls = a.n + 1;
   g.tw = a.tw[prec];
+  g.ntiles = 1;
+  g.nfields = 1;
   return g;
 }
 
@@ -184,11 +186,13 @@ void launch_reg(const Axis& a, int prec, const PairIO& f0, const PairIO& f1, int
                 hipStream_t s) {
   constexpr int RL = 2 * 64 / N2;
   LineGeom g = geom(a, prec, RL);
+  g.ntiles = (a.nlines + RL - 1) / RL;
+  g.nfields = nfields;
   // transpose buffer (64 N1 complex) or the staged rows (RL (n + N2/2) reals)
   const size_t tbuf = (size_t)64 * N1 * 2 * sizeof(T);
   const size_t sbuf = (size_t)RL * (N1 * N2 + N2 / 2) * sizeof(T);
   const size_t smem = tbuf > sbuf ? tbuf : sbuf;
-  dim3 grid((a.nlines + RL - 1) / RL, nfields);
+  dim3 grid(g.ntiles * nfields);
   if (a.contig)
     hipLaunchKernelGGL((k_line_reg<T, N1, N2, true, false>), grid, dim3(64), smem, s, g, f0, f1);
   else if (a.inner % RL == 0)
@@ -201,7 +205,9 @@ template <typename T, int NN>
 void launch_lane(const Axis& a, int prec, const PairIO& f0, const PairIO& f1, int nfields,
                  hipStream_t s) {
   LineGeom g = geom(a, prec, 2);
-  dim3 grid((a.nlines + 127) / 128, nfields);
+  g.ntiles = (a.nlines + 127) / 128;
+  g.nfields = nfields;
+  dim3 grid(g.ntiles * nfields);
   if (a.contig)
     hipLaunchKernelGGL((k_line_lane<T, NN, true>), grid, dim3(64), 0, s, g, f0, f1);
   else
@@ -214,9 +220,11 @@ void launch_stockham(const Axis& a, int prec, const PairIO& f0, const PairIO& f1
                      hipStream_t s) {
   const int L = pair_tile(a, prec);
   LineGeom g = geom(a, prec, L);
+  g.ntiles = (a.nlines + L - 1) / L;
+  g.nfields = nfields;
   const int P = L / 2;
   const size_t smem = 2 * (size_t)P * g.ls * 2 * sizeof(T);
-  dim3 grid((a.nlines + L - 1) / L, nfields);
+  dim3 grid(g.ntiles * nfields);
   if (a.contig)
     hipLaunchKernelGGL((k_pair_roundtrip<T, true>), grid, dim3(kThreads), smem, s, g, f0, f1);
   else

@@ -170,3 +170,21 @@ def test_propagator_fp32_error(shape):
     assert abs(e.calcBigQ() / Q - 1) < 2e-5
     gA, gB = e.densities()
     assert rel(gA, phiA) < 2e-5 and rel(gB, phiB) < 2e-5
+
+
+def test_anderson_with_picard_start_trajectory():
+    # Picard until the residual RMS drops below 0.05, then depth-10 Anderson:
+    # the engine follows the oracle through the switch.
+    shape, cell = (32, 1, 1), (4.0 / 32, 1.0, 1.0)
+    wA, wB = fields.seeded_fields(shape, cell, sigma=0.05, seed=5, seed_kind="lamellar", amplitude=2.0)
+    P = O.Params(*shape, *cell, ds=0.01, fA=0.3, chiN=16.0, mix=1, lambda0=0.2, depth=10,
+                 beta=1.0, anderson_start=0.05)
+    ref = O.scft_run(P, wA, wB, n_iters=60)
+    e = DiblockSCFT(shape, cell, 0.3, 0.01, 16.0)
+    e.setFields(wA, wB)
+    h = e.iterate(60, Mixing("anderson", lambda0=0.2, depth=10, beta=1.0, anderson_start=0.05))
+    k = int(np.argmax(ref["err"] < 0.05))
+    print(f"switch at {k}; err ratio gpu/oracle at the end "
+          f"{h['err'][-1] / ref['err'][-1]:.6f}", flush=True)
+    assert rel(h["err"][:k + 10], ref["err"][:k + 10]) < 1e-8
+    assert rel(h["F"][:k + 10], ref["F"][:k + 10]) < 1e-8
