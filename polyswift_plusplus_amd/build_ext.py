@@ -48,7 +48,24 @@ def build(verbose: bool = False) -> str:
     if verbose and errs.strip():
         print(errs, file=sys.stderr)
     os.replace(tmp, LIB)
+    build_demo(verbose)
     return LIB
+
+
+DEMO = os.path.join(HERE, "psx_demo")
+
+
+def build_demo(verbose: bool = False) -> str:
+    """C++ host program over the C ABI (links libpsx_scft.so by rpath)."""
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+           os.path.join(CSRC, "psx_demo.cpp"), "-L", HERE, "-lpsx_scft",
+           "-Wl,-rpath,$ORIGIN", "-o", DEMO]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed ({r.returncode}):\n{r.stderr}")
+    return DEMO
 
 
 if __name__ == "__main__":

@@ -188,3 +188,23 @@ def test_anderson_with_picard_start_trajectory():
           f"{h['err'][-1] / ref['err'][-1]:.6f}", flush=True)
     assert rel(h["err"][:k + 10], ref["err"][:k + 10]) < 1e-8
     assert rel(h["F"][:k + 10], ref["F"][:k + 10]) < 1e-8
+
+
+def test_cpp_host_program():
+    """The C++ host wrapper (psx_host.hpp) drives the same engine: the demo's
+    F history equals the Python path on the same inputs."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "polyswift_plusplus_amd", "psx_demo")
+    r = subprocess.run([exe, "32", "32", "1", "0.3", "20", "0.01", "20"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    F_cpp = float(r.stdout.strip().splitlines()[-1].split("=")[1].split()[0])
+    shape = (32, 32, 1)
+    x = (np.arange(32) / 32.0)[:, None, None] * np.ones(shape)
+    wA = np.cos(2 * np.pi * x)
+    e = DiblockSCFT(shape, (4.0 / 32, 4.0 / 32, 4.0 / 32), 0.3, 0.01, 20.0)
+    e.setFields(wA, -wA)
+    h = e.iterate(20, Mixing("steepest", 0.1, 0.0))
+    assert abs(F_cpp - h["F"][-1]) < 1e-11
