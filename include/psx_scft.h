@@ -194,6 +194,41 @@ psx_status psx_scft_param_grad(psx_scft* e, double* h_dFdchiN,
 psx_status psx_scft_time_step(psx_scft* e, int reps, double* h_ms_step,
                               double* h_ms_pass3);
 
+/* Time the candidate line kernels of each pass (register splits, LDS
+ * Stockham) on this engine's data and keep the fastest.  No reference
+ * counterpart. */
+psx_status psx_scft_autotune(psx_scft* e, int reps);
+
+/* ===================================================================== */
+/* Batched parameter sweeps: many independent diblock instances (grid,     */
+/* box, fA, chiN may all differ; the contour step count must match)        */
+/* advanced together.  Instances with the same grid share each line-pass   */
+/* launch (both sweeps of every member), the per-instance reductions and   */
+/* updates follow, and one batch iteration is one hipGraph.  No reference  */
+/* counterpart (the reference runs one system per process).                */
+/* ===================================================================== */
+typedef struct psx_batch psx_batch;
+psx_status psx_batch_create(int n, const psx_grid* grids,
+                            const psx_diblock* chains, psx_prec prec,
+                            void* stream, psx_batch** out);
+psx_status psx_batch_destroy(psx_batch* b);
+/* Instance count and number of distinct grid sizes. */
+psx_status psx_batch_size(const psx_batch* b, int* n, int* nclasses);
+/* The engine of instance i (owned by the batch): fields, densities, stress
+ * and the other psx_scft_* calls work on it directly. */
+psx_status psx_batch_instance(psx_batch* b, int i, psx_scft** out);
+psx_status psx_batch_set_precision(psx_batch* b, psx_prec prec);
+/* Per grid size and pass, time the candidate line kernels on the batch's
+ * own fields and keep the fastest; h_choice (3 per size class, may be
+ * NULL) receives the chosen variants (-1 register default, 1 Stockham,
+ * 2 alternative register split). */
+psx_status psx_batch_autotune(psx_batch* b, int reps, int* h_choice);
+/* n_iters iterations of every instance; histories are n * n_iters, row-major
+ * by instance (may be NULL). */
+psx_status psx_batch_iterate(psx_batch* b, const psx_mixing* mix, int n_iters,
+                             int use_graph, double* h_F, double* h_err,
+                             int* h_done);
+
 #ifdef __cplusplus
 }
 #endif

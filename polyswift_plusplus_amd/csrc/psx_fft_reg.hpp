@@ -92,7 +92,7 @@ __device__ __forceinline__ void dft_reg(C* v, const C* __restrict__ tw, int twst
 // complex load.
 template <typename T, int N1, int N2, bool CONTIG, bool VEC>
 __global__ void __launch_bounds__(64)
-k_line_reg(LineGeom g, PairIO f0, PairIO f1) {
+k_line_reg(LineGeom g, const PairIO* __restrict__ fio) {
   typedef typename Cx<T>::type C;
   constexpr int n = N1 * N2;
   constexpr int CL = 64 / N2;     // complex lines per wave
@@ -104,12 +104,12 @@ k_line_reg(LineGeom g, PairIO f0, PairIO f1) {
   int tile, field;
   xcd_tile(g.ntiles, g.nfields, &tile, &field);
 
-  const bool second = field != 0;
-  const T* __restrict__ in = (const T*)(second ? f1.in : f0.in);
-  T* __restrict__ out = (T*)(second ? f1.out : f0.out);
-  const T* __restrict__ pre = (const T*)(second ? f1.pre : f0.pre);
-  const T* __restrict__ post = (const T*)(second ? f1.post : f0.post);
-  const T* __restrict__ emul = (const T*)(second ? f1.emul : f0.emul);
+  const PairIO fd = fio[field];
+  const T* __restrict__ in = (const T*)fd.in;
+  T* __restrict__ out = (T*)fd.out;
+  const T* __restrict__ pre = (const T*)fd.pre;
+  const T* __restrict__ post = (const T*)fd.post;
+  const T* __restrict__ emul = (const T*)fd.emul;
   const C* __restrict__ tw = (const C*)g.tw;
 
   const int lane = threadIdx.x;
@@ -267,16 +267,16 @@ k_line_reg(LineGeom g, PairIO f0, PairIO f1) {
 // Whole line in one lane (n <= 8): no transposes.
 template <typename T, int NN, bool CONTIG>
 __global__ void __launch_bounds__(64)
-k_line_lane(LineGeom g, PairIO f0, PairIO f1) {
+k_line_lane(LineGeom g, const PairIO* __restrict__ fio) {
   typedef typename Cx<T>::type C;
   int tile, field;
   xcd_tile(g.ntiles, g.nfields, &tile, &field);
-  const bool second = field != 0;
-  const T* __restrict__ in = (const T*)(second ? f1.in : f0.in);
-  T* __restrict__ out = (T*)(second ? f1.out : f0.out);
-  const T* __restrict__ pre = (const T*)(second ? f1.pre : f0.pre);
-  const T* __restrict__ post = (const T*)(second ? f1.post : f0.post);
-  const T* __restrict__ emul = (const T*)(second ? f1.emul : f0.emul);
+  const PairIO fd = fio[field];
+  const T* __restrict__ in = (const T*)fd.in;
+  T* __restrict__ out = (T*)fd.out;
+  const T* __restrict__ pre = (const T*)fd.pre;
+  const T* __restrict__ post = (const T*)fd.post;
+  const T* __restrict__ emul = (const T*)fd.emul;
   const C* __restrict__ tw = (const C*)g.tw;
   const int la = (tile * 64 + threadIdx.x) * 2;
   const bool va = la < g.nlines, vb = la + 1 < g.nlines;

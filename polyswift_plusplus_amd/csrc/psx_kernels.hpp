@@ -56,21 +56,21 @@ __device__ __forceinline__ int line_addr(int l, int k, int n, int inner) {
 //   out = post * IFFT_n( emul * FFT_n( pre * in ) )
 // Two real lines share one complex line (line 2p -> real part, 2p+1 ->
 // imaginary part); the operator is real and even in k, so the two results
-// come back in the real and imaginary parts unchanged.  The field index selects
-// one of two independent fields (the two contour sweeps run in one launch).
+// come back in the real and imaginary parts unchanged.  Fields (the two
+// sweeps of an engine, or of every instance of a batch) come from the table fio.
 template <typename T, bool CONTIG>
 __global__ void __launch_bounds__(kThreads)
-k_pair_roundtrip(LineGeom g, PairIO f0, PairIO f1) {
+k_pair_roundtrip(LineGeom g, const PairIO* __restrict__ fio) {
   typedef typename Cx<T>::type C;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   int tile, field;
   xcd_tile(g.ntiles, g.nfields, &tile, &field);
-  const bool second = field != 0;
-  const T* __restrict__ in = (const T*)(second ? f1.in : f0.in);
-  T* __restrict__ out = (T*)(second ? f1.out : f0.out);
-  const T* __restrict__ pre = (const T*)(second ? f1.pre : f0.pre);
-  const T* __restrict__ post = (const T*)(second ? f1.post : f0.post);
-  const T* __restrict__ emul = (const T*)(second ? f1.emul : f0.emul);
+  const PairIO fd = fio[field];
+  const T* __restrict__ in = (const T*)fd.in;
+  T* __restrict__ out = (T*)fd.out;
+  const T* __restrict__ pre = (const T*)fd.pre;
+  const T* __restrict__ post = (const T*)fd.post;
+  const T* __restrict__ emul = (const T*)fd.emul;
   const C* __restrict__ tw = (const C*)g.tw;
 
   const int n = g.pl.n, L = g.L, P = L >> 1, ls = g.ls;

@@ -182,7 +182,7 @@ LineGeom geom(const Axis& a, int prec, int L) {
 }
 
 template <typename T, int N1, int N2>
-void launch_reg(const Axis& a, int prec, const PairIO& f0, const PairIO& f1, int nfields,
+void launch_reg(const Axis& a, int prec, const PairIO* fio, int nfields,
                 hipStream_t s) {
   constexpr int RL = 2 * 64 / N2;
   LineGeom g = geom(a, prec, RL);
@@ -194,29 +194,29 @@ void launch_reg(const Axis& a, int prec, const PairIO& f0, const PairIO& f1, int
   const size_t smem = tbuf > sbuf ? tbuf : sbuf;
   dim3 grid(g.ntiles * nfields);
   if (a.contig)
-    hipLaunchKernelGGL((k_line_reg<T, N1, N2, true, false>), grid, dim3(64), smem, s, g, f0, f1);
+    hipLaunchKernelGGL((k_line_reg<T, N1, N2, true, false>), grid, dim3(64), smem, s, g, fio);
   else if (a.inner % RL == 0)
-    hipLaunchKernelGGL((k_line_reg<T, N1, N2, false, true>), grid, dim3(64), smem, s, g, f0, f1);
+    hipLaunchKernelGGL((k_line_reg<T, N1, N2, false, true>), grid, dim3(64), smem, s, g, fio);
   else
-    hipLaunchKernelGGL((k_line_reg<T, N1, N2, false, false>), grid, dim3(64), smem, s, g, f0, f1);
+    hipLaunchKernelGGL((k_line_reg<T, N1, N2, false, false>), grid, dim3(64), smem, s, g, fio);
 }
 
 template <typename T, int NN>
-void launch_lane(const Axis& a, int prec, const PairIO& f0, const PairIO& f1, int nfields,
+void launch_lane(const Axis& a, int prec, const PairIO* fio, int nfields,
                  hipStream_t s) {
   LineGeom g = geom(a, prec, 2);
   g.ntiles = (a.nlines + 127) / 128;
   g.nfields = nfields;
   dim3 grid(g.ntiles * nfields);
   if (a.contig)
-    hipLaunchKernelGGL((k_line_lane<T, NN, true>), grid, dim3(64), 0, s, g, f0, f1);
+    hipLaunchKernelGGL((k_line_lane<T, NN, true>), grid, dim3(64), 0, s, g, fio);
   else
-    hipLaunchKernelGGL((k_line_lane<T, NN, false>), grid, dim3(64), 0, s, g, f0, f1);
+    hipLaunchKernelGGL((k_line_lane<T, NN, false>), grid, dim3(64), 0, s, g, fio);
 }
 
 // Generic Stockham LDS kernel: any n with prime factors <= 13.
 template <typename T>
-void launch_stockham(const Axis& a, int prec, const PairIO& f0, const PairIO& f1, int nfields,
+void launch_stockham(const Axis& a, int prec, const PairIO* fio, int nfields,
                      hipStream_t s) {
   const int L = pair_tile(a, prec);
   LineGeom g = geom(a, prec, L);
@@ -226,9 +226,9 @@ void launch_stockham(const Axis& a, int prec, const PairIO& f0, const PairIO& f1
   const size_t smem = 2 * (size_t)P * g.ls * 2 * sizeof(T);
   dim3 grid(g.ntiles * nfields);
   if (a.contig)
-    hipLaunchKernelGGL((k_pair_roundtrip<T, true>), grid, dim3(kThreads), smem, s, g, f0, f1);
+    hipLaunchKernelGGL((k_pair_roundtrip<T, true>), grid, dim3(kThreads), smem, s, g, fio);
   else
-    hipLaunchKernelGGL((k_pair_roundtrip<T, false>), grid, dim3(kThreads), smem, s, g, f0, f1);
+    hipLaunchKernelGGL((k_pair_roundtrip<T, false>), grid, dim3(kThreads), smem, s, g, fio);
 }
 
 // Round-trip pass along one axis: register-resident kernels for the line
@@ -238,33 +238,37 @@ void launch_stockham(const Axis& a, int prec, const PairIO& f0, const PairIO& f1
 int g_force_stockham = -1;
 
 template <typename T>
-psx_status launch_pair(const Axis& a, int prec, const PairIO& f0, const PairIO& f1,
-                       int nfields, hipStream_t s) {
+psx_status launch_pair(const Axis& a, int prec, const PairIO* fio,
+                       int nfields, hipStream_t s, int variant = -1) {
   if (g_force_stockham < 0) {
     const char* v = getenv("PSX_FORCE_STOCKHAM");
     g_force_stockham = (v && v[0] == '1') ? 1 : 0;
   }
-  if (g_force_stockham == 1) {
-    launch_stockham<T>(a, prec, f0, f1, nfields, s);
+  if (g_force_stockham == 1 || variant == 1) {
+    launch_stockham<T>(a, prec, fio, nfields, s);
+  } else if (variant == 2 && a.n == 64) {
+    launch_reg<T, 16, 4>(a, prec, fio, nfields, s);
+  } else if (variant == 2 && a.n == 128) {
+    launch_reg<T, 32, 4>(a, prec, fio, nfields, s);
   } else {
     switch (a.n) {
-      case 1: launch_lane<T, 1>(a, prec, f0, f1, nfields, s); break;
-      case 2: launch_lane<T, 2>(a, prec, f0, f1, nfields, s); break;
-      case 3: launch_lane<T, 3>(a, prec, f0, f1, nfields, s); break;
-      case 4: launch_lane<T, 4>(a, prec, f0, f1, nfields, s); break;
-      case 5: launch_lane<T, 5>(a, prec, f0, f1, nfields, s); break;
-      case 8: launch_lane<T, 8>(a, prec, f0, f1, nfields, s); break;
+      case 1: launch_lane<T, 1>(a, prec, fio, nfields, s); break;
+      case 2: launch_lane<T, 2>(a, prec, fio, nfields, s); break;
+      case 3: launch_lane<T, 3>(a, prec, fio, nfields, s); break;
+      case 4: launch_lane<T, 4>(a, prec, fio, nfields, s); break;
+      case 5: launch_lane<T, 5>(a, prec, fio, nfields, s); break;
+      case 8: launch_lane<T, 8>(a, prec, fio, nfields, s); break;
       // 8 lanes per line where the length allows (8 complex lines per wave:
       // 128-byte rows in FP64); fewer points per lane keep more waves in
       // flight, which matters more here than FP32 row width.
-      case 16: launch_reg<T, 4, 4>(a, prec, f0, f1, nfields, s); break;
-      case 32: launch_reg<T, 8, 4>(a, prec, f0, f1, nfields, s); break;
-      case 48: launch_reg<T, 12, 4>(a, prec, f0, f1, nfields, s); break;
-      case 64: launch_reg<T, 8, 8>(a, prec, f0, f1, nfields, s); break;
-      case 96: launch_reg<T, 24, 4>(a, prec, f0, f1, nfields, s); break;
-      case 128: launch_reg<T, 16, 8>(a, prec, f0, f1, nfields, s); break;
-      case 256: launch_reg<T, 16, 16>(a, prec, f0, f1, nfields, s); break;
-      default: launch_stockham<T>(a, prec, f0, f1, nfields, s); break;
+      case 16: launch_reg<T, 4, 4>(a, prec, fio, nfields, s); break;
+      case 32: launch_reg<T, 8, 4>(a, prec, fio, nfields, s); break;
+      case 48: launch_reg<T, 12, 4>(a, prec, fio, nfields, s); break;
+      case 64: launch_reg<T, 8, 8>(a, prec, fio, nfields, s); break;
+      case 96: launch_reg<T, 24, 4>(a, prec, fio, nfields, s); break;
+      case 128: launch_reg<T, 16, 8>(a, prec, fio, nfields, s); break;
+      case 256: launch_reg<T, 16, 16>(a, prec, fio, nfields, s); break;
+      default: launch_stockham<T>(a, prec, fio, nfields, s); break;
     }
   }
   PSX_HIP(hipGetLastError());
@@ -425,6 +429,7 @@ struct SweepBufs {
   void* wfA = nullptr;
   void* wfB = nullptr;
   void* emul[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};  // [block][axis idx]
+  PairIO* fio = nullptr;   // device table [t][pass][2]: both sweeps of every contour step
   bool wfac_valid = false;
 };
 
@@ -449,6 +454,7 @@ struct psx_scft {
   int sA, sB, Ns;
   int prec;
   std::vector<Axis> axes;
+  int variant[2][3] = {{-1, -1, -1}, {-1, -1, -1}};   // line-kernel choice per precision, pass
   SweepBufs sw[2];
   double *wA = nullptr, *wB = nullptr, *praw = nullptr, *phiA = nullptr, *phiB = nullptr;
   double* cw = nullptr;          // quadrature weights (Ns + 2)
@@ -497,6 +503,36 @@ std::vector<double> axis_emul(const Axis& a, double cell, double ds, double b) {
   return e;
 }
 
+// Field descriptors [t][pass][2] of both sweeps for every contour step.
+std::vector<PairIO> fio_table(const psx_scft* e, int prec) {
+  const SweepBufs& s = e->sw[prec];
+  const size_t tb = prec_bytes(prec);
+  const int na = (int)e->axes.size();
+  std::vector<PairIO> tab((size_t)e->Ns * na * 2);
+  const char* F = (const char*)s.F;
+  const char* G = (const char*)s.G;
+  for (int t = 0; t < e->Ns; ++t) {
+    const bool fa = t < e->sA;   // forward step t lies in block A
+    const bool gb = t < e->sB;   // backward step t lies in block B
+    for (int i = 0; i < na; ++i) {
+      const bool first = i == 0, last = i == na - 1;
+      PairIO& f0 = tab[((size_t)t * na + i) * 2];
+      PairIO& f1 = tab[((size_t)t * na + i) * 2 + 1];
+      f0.in = first ? (const void*)(F + (size_t)t * e->N * tb) : s.tmpF;
+      f0.out = last ? (void*)(F + (size_t)(t + 1) * e->N * tb) : s.tmpF;
+      f0.pre = first ? (fa ? s.wfA : s.wfB) : nullptr;
+      f0.post = last ? (fa ? s.wfA : s.wfB) : nullptr;
+      f0.emul = s.emul[fa ? 0 : 1][i];
+      f1.in = first ? (const void*)(G + (size_t)t * e->N * tb) : s.tmpG;
+      f1.out = last ? (void*)(G + (size_t)(t + 1) * e->N * tb) : s.tmpG;
+      f1.pre = first ? (gb ? s.wfB : s.wfA) : nullptr;
+      f1.post = last ? (gb ? s.wfB : s.wfA) : nullptr;
+      f1.emul = s.emul[gb ? 1 : 0][i];
+    }
+  }
+  return tab;
+}
+
 psx_status ensure_sweep(psx_scft* e, int prec) {
   SweepBufs& s = e->sw[prec];
   if (s.ready) return PSX_OK;
@@ -522,6 +558,11 @@ psx_status ensure_sweep(psx_scft* e, int prec) {
       }
     }
   }
+  // field table of every pass of every contour step (static: graphs capture it)
+  {
+    std::vector<PairIO> tab = fio_table(e, prec);
+    PSX_TRY(upload((void**)&s.fio, tab.data(), tab.size() * sizeof(PairIO)));
+  }
   // free ends: q(head of A) = q+(tail of B) = 1 (pspolymer/PsBlock.cpp:146-147)
   const int g = grid_1d(e->N);
   if (prec == PSX_FP64) {
@@ -539,6 +580,7 @@ psx_status ensure_sweep(psx_scft* e, int prec) {
 
 void free_sweep(SweepBufs& s) {
   dfree(s.F); dfree(s.G); dfree(s.tmpF); dfree(s.tmpG); dfree(s.wfA); dfree(s.wfB);
+  dfree((void*&)s.fio);
   for (auto& b : s.emul) for (auto& p : b) dfree(p);
   s.ready = false;
 }
@@ -563,27 +605,11 @@ psx_status compute_wfac(psx_scft* e) {
 template <typename T>
 psx_status dual_step(psx_scft* e, int t, int only_pass = -1) {
   SweepBufs& s = e->sw[e->prec];
-  const size_t tb = sizeof(T);
-  const char* F = (const char*)s.F;
-  const char* G = (const char*)s.G;
-  const bool fa = t < e->sA;   // forward step t lies in block A
-  const bool gb = t < e->sB;   // backward step t lies in block B
   const int na = (int)e->axes.size();
   for (int i = 0; i < na; ++i) {
     if (only_pass >= 0 && i != only_pass) continue;
-    const bool first = i == 0, last = i == na - 1;
-    PairIO f0, f1;
-    f0.in = first ? (const void*)(F + (size_t)t * e->N * tb) : s.tmpF;
-    f0.out = last ? (void*)(F + (size_t)(t + 1) * e->N * tb) : s.tmpF;
-    f0.pre = first ? (fa ? s.wfA : s.wfB) : nullptr;
-    f0.post = last ? (fa ? s.wfA : s.wfB) : nullptr;
-    f0.emul = s.emul[fa ? 0 : 1][i];
-    f1.in = first ? (const void*)(G + (size_t)t * e->N * tb) : s.tmpG;
-    f1.out = last ? (void*)(G + (size_t)(t + 1) * e->N * tb) : s.tmpG;
-    f1.pre = first ? (gb ? s.wfB : s.wfA) : nullptr;
-    f1.post = last ? (gb ? s.wfB : s.wfA) : nullptr;
-    f1.emul = s.emul[gb ? 1 : 0][i];
-    PSX_TRY(launch_pair<T>(e->axes[i], e->prec, f0, f1, 2, e->stream));
+    PSX_TRY(launch_pair<T>(e->axes[i], e->prec, s.fio + ((size_t)t * na + i) * 2, 2, e->stream,
+                           e->variant[e->prec][i]));
   }
   return PSX_OK;
 }
@@ -673,9 +699,16 @@ psx_status ensure_ring(psx_scft* e, int depth) {
   return PSX_OK;
 }
 
+psx_status enqueue_post_sweep(psx_scft* e, const psx_mixing* mix);
+
 // Enqueue one full SCFT iteration on e->stream (capturable).
 psx_status enqueue_iteration(psx_scft* e, const psx_mixing* mix) {
   PSX_TRY(sweep(e));
+  return enqueue_post_sweep(e, mix);
+}
+
+// Q, quadrature and the field update after both sweeps (capturable).
+psx_status enqueue_post_sweep(psx_scft* e, const psx_mixing* mix) {
   PSX_TRY(launch_Q(e));
   if (mix->kind == 0) {
     PSX_TRY(launch_quad(e, QM_SD, mix));
@@ -1045,12 +1078,7 @@ extern "C" psx_status psx_scft_reset_history(psx_scft* e) {
   return PSX_OK;
 }
 
-extern "C" psx_status psx_scft_iterate(psx_scft* e, const psx_mixing* mix, int n_iters, double tol,
-                                       int use_graph, double* hF, double* hQ, double* herr,
-                                       int* done) {
-  if (!e || !mix) return fail(PSX_ERR_INVALID, "null argument");
-  if (done) *done = 0;
-  if (n_iters <= 0) return PSX_OK;
+static psx_status prepare_iterate(psx_scft* e, const psx_mixing* mix, int n_iters) {
   if (mix->kind != 0 && mix->kind != 1) return fail(PSX_ERR_INVALID, "mixing kind must be 0 or 1");
   if (mix->kind == 1 && (mix->depth < 0 || mix->depth > kMaxDepth))
     return fail(PSX_ERR_INVALID, "Anderson depth must be in [0, 32]");
@@ -1064,14 +1092,48 @@ extern "C" psx_status psx_scft_iterate(psx_scft* e, const psx_mixing* mix, int n
   if (!e->sw[e->prec].wfac_valid) PSX_TRY(compute_wfac(e));
   hipLaunchKernelGGL(k_state_set_call, dim3(1), dim3(1), 0, e->stream, e->st);
   PSX_HIP(hipGetLastError());
+  return PSX_OK;
+}
+
+static GraphKey graph_key(int prec, const psx_mixing* mix) {
+  GraphKey key;
+  std::memset(&key, 0, sizeof(key));
+  key.prec = prec; key.kind = mix->kind; key.depth = mix->kind ? mix->depth : 0;
+  key.f_order = mix->f_order; key.lam0 = mix->lambda0; key.lam1 = mix->lambda1;
+  key.beta = mix->kind ? mix->beta : 0.0; key.start = mix->kind ? mix->anderson_start : 0.0;
+  return key;
+}
+
+// Copy the first `it` history rows to the host and update the validity flags.
+static psx_status finish_iterate(psx_scft* e, int it, double* hF, double* hQ, double* herr) {
+  if (hF || hQ || herr) {
+    std::vector<double> hh((size_t)it * 3);
+    if (it > 0)
+      PSX_HIP(hipMemcpy(hh.data(), e->hist, hh.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < it; ++i) {
+      if (hF) hF[i] = hh[3 * i];
+      if (hQ) hQ[i] = hh[3 * i + 1];
+      if (herr) herr[i] = hh[3 * i + 2];
+    }
+  }
+  e->sw[e->prec].wfac_valid = true;
+  e->sw[e->prec ^ 1].wfac_valid = false;
+  e->swept = false;
+  e->dens_valid = true;   // phi of the last iteration's sweep
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_scft_iterate(psx_scft* e, const psx_mixing* mix, int n_iters, double tol,
+                                       int use_graph, double* hF, double* hQ, double* herr,
+                                       int* done) {
+  if (!e || !mix) return fail(PSX_ERR_INVALID, "null argument");
+  if (done) *done = 0;
+  if (n_iters <= 0) return PSX_OK;
+  PSX_TRY(prepare_iterate(e, mix, n_iters));
 
   hipGraphExec_t exec = nullptr;
   if (use_graph) {
-    GraphKey key;
-    std::memset(&key, 0, sizeof(key));
-    key.prec = e->prec; key.kind = mix->kind; key.depth = mix->kind ? mix->depth : 0;
-    key.f_order = mix->f_order; key.lam0 = mix->lambda0; key.lam1 = mix->lambda1;
-    key.beta = mix->kind ? mix->beta : 0.0; key.start = mix->kind ? mix->anderson_start : 0.0;
+    GraphKey key = graph_key(e->prec, mix);
     auto it = e->graphs.find(key);
     if (it != e->graphs.end()) exec = it->second;
     else {
@@ -1106,19 +1168,7 @@ extern "C" psx_status psx_scft_iterate(psx_scft* e, const psx_mixing* mix, int n
     (void)hipMemcpy(&e->st->nanflag, &z, sizeof(int), hipMemcpyHostToDevice);
   }
   if (done) *done = it;
-  if (hF || hQ || herr) {
-    std::vector<double> hh((size_t)it * 3);
-    PSX_HIP(hipMemcpy(hh.data(), e->hist, hh.size() * sizeof(double), hipMemcpyDeviceToHost));
-    for (int i = 0; i < it; ++i) {
-      if (hF) hF[i] = hh[3 * i];
-      if (hQ) hQ[i] = hh[3 * i + 1];
-      if (herr) herr[i] = hh[3 * i + 2];
-    }
-  }
-  e->sw[e->prec].wfac_valid = true;
-  e->sw[e->prec ^ 1].wfac_valid = false;
-  e->swept = false;
-  e->dens_valid = true;   // phi of the last iteration's sweep
+  PSX_TRY(finish_iterate(e, it, hF, hQ, herr));
   return result;
 }
 
@@ -1322,4 +1372,254 @@ extern "C" psx_status psx_scft_time_step(psx_scft* e, int reps, double* ms_step,
   (void)hipEventDestroy(b);
   e->swept = false;
   return PSX_OK;
+}
+
+// ===========================================================================
+// Line-kernel autotuner and batched parameter sweeps
+// ===========================================================================
+namespace {
+
+// Candidate line-kernel variants for length n: -1 default split, 1 LDS
+// Stockham, 2 alternative register split (n = 64: 16x4, n = 128: 32x4).
+std::vector<int> variant_candidates(int n) {
+  std::vector<int> v{-1, 1};
+  if (n == 64 || n == 128) v.push_back(2);
+  return v;
+}
+
+// Time `reps` launches of pass `i` over the field table `fio` (nfields) for
+// each candidate; return the fastest variant.
+template <typename T>
+psx_status tune_pass(const Axis& a, int prec, const PairIO* fio, int nfields, hipStream_t s,
+                     int reps, int* best) {
+  hipEvent_t e0, e1;
+  PSX_HIP(hipEventCreate(&e0));
+  PSX_HIP(hipEventCreate(&e1));
+  float best_ms = 1e30f;
+  *best = -1;
+  for (int v : variant_candidates(a.n)) {
+    PSX_TRY(launch_pair<T>(a, prec, fio, nfields, s, v));   // warm
+    PSX_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) PSX_TRY(launch_pair<T>(a, prec, fio, nfields, s, v));
+    PSX_HIP(hipEventRecord(e1, s));
+    PSX_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    PSX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best_ms) { best_ms = ms; *best = v; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return PSX_OK;
+}
+
+struct BatchClass {
+  int dims[3];
+  std::vector<int> members;         // engine indices
+  PairIO* fio[2] = {nullptr, nullptr};   // [t][pass][2 * members] per precision
+  int variant[2][3] = {{-1, -1, -1}, {-1, -1, -1}};
+};
+
+}  // namespace
+
+extern "C" psx_status psx_scft_autotune(psx_scft* e, int reps) {
+  if (!e || reps < 1) return fail(PSX_ERR_INVALID, "bad argument");
+  if (!e->sw[e->prec].wfac_valid) PSX_TRY(compute_wfac(e));
+  const int na = (int)e->axes.size();
+  for (int i = 0; i < na; ++i) {
+    const PairIO* fio = e->sw[e->prec].fio + (size_t)i * 2;
+    if (e->prec == PSX_FP64) PSX_TRY(tune_pass<double>(e->axes[i], e->prec, fio, 2, e->stream, reps, &e->variant[e->prec][i]));
+    else PSX_TRY(tune_pass<float>(e->axes[i], e->prec, fio, 2, e->stream, reps, &e->variant[e->prec][i]));
+  }
+  drop_graphs(e);
+  e->swept = false;
+  return PSX_OK;
+}
+
+struct psx_batch {
+  std::vector<psx_scft*> eng;
+  std::vector<BatchClass> cls;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int prec = PSX_FP64;
+  int Ns = 0;
+  std::map<GraphKey, hipGraphExec_t> graphs;
+};
+
+static void batch_drop_graphs(psx_batch* b) {
+  for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+  b->graphs.clear();
+}
+
+static psx_status batch_tables(psx_batch* b, int prec) {
+  for (auto& c : b->cls) {
+    if (c.fio[prec]) continue;
+    psx_scft* e0 = b->eng[c.members[0]];
+    const int na = (int)e0->axes.size();
+    const int K = (int)c.members.size();
+    std::vector<std::vector<PairIO>> per;
+    for (int m : c.members) per.push_back(fio_table(b->eng[m], prec));
+    std::vector<PairIO> tab((size_t)b->Ns * na * 2 * K);
+    for (int t = 0; t < b->Ns; ++t)
+      for (int i = 0; i < na; ++i)
+        for (int k = 0; k < K; ++k)
+          for (int h = 0; h < 2; ++h)
+            tab[(((size_t)t * na + i) * K + k) * 2 + h] = per[k][((size_t)t * na + i) * 2 + h];
+    PSX_TRY(upload((void**)&c.fio[prec], tab.data(), tab.size() * sizeof(PairIO)));
+  }
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_batch_destroy(psx_batch* b) {
+  if (!b) return PSX_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_drop_graphs(b);
+  for (auto& c : b->cls) { dfree((void*&)c.fio[0]); dfree((void*&)c.fio[1]); }
+  for (auto* e : b->eng) psx_scft_destroy(e);
+  if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_batch_create(int n, const psx_grid* grids, const psx_diblock* chains,
+                                       psx_prec prec, void* stream, psx_batch** out) {
+  if (!out || !grids || !chains || n < 1) return fail(PSX_ERR_INVALID, "bad argument");
+  *out = nullptr;
+  psx_batch* b = new psx_batch();
+  psx_status st = make_stream(stream, &b->stream, &b->own_stream);
+  b->prec = prec;
+  for (int i = 0; i < n && st == PSX_OK; ++i) {
+    psx_scft* e = nullptr;
+    st = psx_scft_create(&grids[i], &chains[i], prec, b->stream, &e);
+    if (st != PSX_OK) break;
+    b->eng.push_back(e);
+    if (i == 0) b->Ns = e->Ns;
+    else if (e->Ns != b->Ns)
+      st = fail(PSX_ERR_INVALID, "all batch instances need the same number of contour steps");
+  }
+  if (st == PSX_OK) {
+    for (int i = 0; i < n; ++i) {
+      const psx_scft* e = b->eng[i];
+      BatchClass* found = nullptr;
+      for (auto& c : b->cls)
+        if (c.dims[0] == e->dims[0] && c.dims[1] == e->dims[1] && c.dims[2] == e->dims[2]) found = &c;
+      if (!found) {
+        b->cls.emplace_back();
+        found = &b->cls.back();
+        for (int d = 0; d < 3; ++d) found->dims[d] = e->dims[d];
+      }
+      found->members.push_back(i);
+    }
+    st = batch_tables(b, prec);
+  }
+  if (st != PSX_OK) { psx_batch_destroy(b); return st; }
+  *out = b;
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_batch_size(const psx_batch* b, int* n, int* nclasses) {
+  if (!b) return fail(PSX_ERR_INVALID, "null batch");
+  if (n) *n = (int)b->eng.size();
+  if (nclasses) *nclasses = (int)b->cls.size();
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_batch_instance(psx_batch* b, int i, psx_scft** out) {
+  if (!b || !out || i < 0 || i >= (int)b->eng.size()) return fail(PSX_ERR_INVALID, "bad instance");
+  *out = b->eng[i];
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_batch_set_precision(psx_batch* b, psx_prec prec) {
+  if (!b) return fail(PSX_ERR_INVALID, "null batch");
+  for (auto* e : b->eng) PSX_TRY(psx_scft_set_precision(e, prec));
+  PSX_TRY(batch_tables(b, prec));
+  b->prec = prec;
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_batch_autotune(psx_batch* b, int reps, int* h_choice) {
+  if (!b || reps < 1) return fail(PSX_ERR_INVALID, "bad argument");
+  for (auto* e : b->eng)
+    if (!e->sw[b->prec].wfac_valid) PSX_TRY(compute_wfac(e));
+  for (size_t ci = 0; ci < b->cls.size(); ++ci) {
+    BatchClass& c = b->cls[ci];
+    psx_scft* e0 = b->eng[c.members[0]];
+    const int na = (int)e0->axes.size();
+    const int K = (int)c.members.size();
+    for (int i = 0; i < na; ++i) {
+      const PairIO* fio = c.fio[b->prec] + (size_t)i * 2 * K;
+      int* v = &c.variant[b->prec][i];
+      if (b->prec == PSX_FP64) PSX_TRY(tune_pass<double>(e0->axes[i], b->prec, fio, 2 * K, b->stream, reps, v));
+      else PSX_TRY(tune_pass<float>(e0->axes[i], b->prec, fio, 2 * K, b->stream, reps, v));
+      if (h_choice) h_choice[ci * 3 + i] = *v;
+    }
+  }
+  batch_drop_graphs(b);
+  for (auto* e : b->eng) e->swept = false;
+  return PSX_OK;
+}
+
+template <typename T>
+static psx_status batch_sweep(psx_batch* b) {
+  for (int t = 0; t < b->Ns; ++t)
+    for (auto& c : b->cls) {
+      psx_scft* e0 = b->eng[c.members[0]];
+      const int na = (int)e0->axes.size();
+      const int K = (int)c.members.size();
+      for (int i = 0; i < na; ++i)
+        PSX_TRY(launch_pair<T>(e0->axes[i], b->prec, c.fio[b->prec] + ((size_t)t * na + i) * 2 * K,
+                               2 * K, b->stream, c.variant[b->prec][i]));
+    }
+  return PSX_OK;
+}
+
+static psx_status batch_enqueue(psx_batch* b, const psx_mixing* mix) {
+  if (b->prec == PSX_FP64) PSX_TRY(batch_sweep<double>(b));
+  else PSX_TRY(batch_sweep<float>(b));
+  for (auto* e : b->eng) PSX_TRY(enqueue_post_sweep(e, mix));
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_batch_iterate(psx_batch* b, const psx_mixing* mix, int n_iters, int use_graph,
+                                        double* hF, double* herr, int* done) {
+  if (!b || !mix) return fail(PSX_ERR_INVALID, "null argument");
+  if (done) *done = 0;
+  if (n_iters <= 0) return PSX_OK;
+  for (auto* e : b->eng) PSX_TRY(prepare_iterate(e, mix, n_iters));
+  hipGraphExec_t exec = nullptr;
+  if (use_graph) {
+    GraphKey key = graph_key(b->prec, mix);
+    auto it = b->graphs.find(key);
+    if (it != b->graphs.end()) exec = it->second;
+    else {
+      hipGraph_t graph;
+      PSX_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      psx_status cs = batch_enqueue(b, mix);
+      hipError_t ce = hipStreamEndCapture(b->stream, &graph);
+      if (cs != PSX_OK) { if (ce == hipSuccess) (void)hipGraphDestroy(graph); return cs; }
+      if (ce != hipSuccess) return fail(PSX_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+      hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ie != hipSuccess) return fail(PSX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+      b->graphs[key] = exec;
+    }
+  }
+  for (int it = 0; it < n_iters; ++it) {
+    if (exec) PSX_HIP(hipGraphLaunch(exec, b->stream));
+    else PSX_TRY(batch_enqueue(b, mix));
+  }
+  PSX_HIP(hipStreamSynchronize(b->stream));
+  psx_status result = PSX_OK;
+  const size_t n = b->eng.size();
+  for (size_t i = 0; i < n; ++i) {
+    psx_scft* e = b->eng[i];
+    DevState h;
+    PSX_TRY(read_state(e, &h));
+    if (h.nanflag && result == PSX_OK)
+      result = fail(PSX_ERR_NAN, "Q is NaN/Inf in batch instance " + std::to_string(i));
+    PSX_TRY(finish_iterate(e, n_iters, hF ? hF + i * n_iters : nullptr, nullptr,
+                           herr ? herr + i * n_iters : nullptr));
+  }
+  if (done) *done = n_iters;
+  return result;
 }

@@ -229,9 +229,93 @@ class DiblockSCFT:
                                          ctypes.cast(passes, _dp)))
         return step.value, list(passes)
 
+    def autotune(self, reps: int = 10) -> None:
+        """Pick the fastest line kernel per pass on this engine's data."""
+        check(self._L.psx_scft_autotune(self._h, int(reps)))
+
+    @classmethod
+    def _borrow(cls, handle, shape, cell, fA, ds, chiN):
+        """View of an engine owned by someone else (a batch)."""
+        self = cls.__new__(cls)
+        self._L = _abi.load()
+        self._h = handle
+        self._owned = False
+        self.shape = tuple(int(x) for x in shape)
+        self.cell = tuple(float(c) for c in cell)
+        self.N = int(np.prod(self.shape))
+        self.fA, self.ds, self.chiN = float(fA), float(ds), float(chiN)
+        sA, sB = ctypes.c_int(), ctypes.c_int()
+        check(self._L.psx_scft_block_steps(handle, ctypes.byref(sA), ctypes.byref(sB)))
+        self.sA, self.sB = sA.value, sB.value
+        self.Ns = self.sA + self.sB
+        return self
+
     def close(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and getattr(self, "_owned", True):
             self._L.psx_scft_destroy(self._h)
+        self._h = None
+
+    __del__ = close
+
+
+class BatchSCFT:
+    """Independent diblock instances (parameter sweep) advanced together.
+
+    specs: list of dicts with shape, cell, fA, ds, chiN (and optional bA, bB).
+    All instances need the same contour step count (same ds)."""
+
+    def __init__(self, specs, prec="fp32", stream=None):
+        self._L = _abi.load()
+        n = len(specs)
+        grids = (_abi.Grid * n)()
+        chains = (_abi.Diblock * n)()
+        for i, sp in enumerate(specs):
+            grids[i] = _abi.Grid(*[int(x) for x in sp["shape"]], *[float(c) for c in sp["cell"]])
+            chains[i] = _abi.Diblock(float(sp["fA"]), float(sp["ds"]), float(sp.get("bA", 1.0)),
+                                     float(sp.get("bB", 1.0)), float(sp["chiN"]))
+        h = ctypes.c_void_p()
+        check(self._L.psx_batch_create(n, grids, chains, _prec(prec), stream, ctypes.byref(h)))
+        self._h = h
+        self.specs = [dict(sp) for sp in specs]
+        nc = ctypes.c_int()
+        check(self._L.psx_batch_size(h, None, ctypes.byref(nc)))
+        self.nclasses = nc.value
+        self.instances = []
+        for i, sp in enumerate(specs):
+            eh = ctypes.c_void_p()
+            check(self._L.psx_batch_instance(h, i, ctypes.byref(eh)))
+            self.instances.append(DiblockSCFT._borrow(eh, sp["shape"], sp["cell"], sp["fA"],
+                                                      sp["ds"], sp["chiN"]))
+
+    def __len__(self):
+        return len(self.instances)
+
+    def setPrecision(self, prec) -> None:
+        check(self._L.psx_batch_set_precision(self._h, _prec(prec)))
+
+    def autotune(self, reps: int = 10):
+        """Per grid size and pass: fastest line kernel (-1 register default,
+        1 Stockham, 2 alternative register split)."""
+        ch = (ctypes.c_int * (3 * self.nclasses))()
+        check(self._L.psx_batch_autotune(self._h, int(reps), ch))
+        return [list(ch[3 * c:3 * c + 3]) for c in range(self.nclasses)]
+
+    def iterate(self, n_iters: int, mixing: Mixing | None = None, use_graph: bool = True) -> dict:
+        mixing = mixing or Mixing()
+        n = len(self.instances)
+        F = np.zeros(n * n_iters)
+        err = np.zeros(n * n_iters)
+        done = ctypes.c_int()
+        m = mixing.c()
+        check(self._L.psx_batch_iterate(self._h, ctypes.byref(m), int(n_iters), int(bool(use_graph)),
+                                        _ptr(F), _ptr(err), ctypes.byref(done)))
+        return dict(F=F.reshape(n, n_iters), err=err.reshape(n, n_iters), iters=done.value)
+
+    def close(self):
+        for inst in getattr(self, "instances", []):
+            inst._h = None
+        if getattr(self, "_h", None):
+            self._L.psx_batch_destroy(self._h)
             self._h = None
 
     __del__ = close
