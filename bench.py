@@ -39,6 +39,18 @@ def algorithmic_bytes_iter(N: int, Ns: int, bp: int, m: int) -> int:
     return N * (6 * Ns * bp + (Ns + 1) * bp + 80 + ((6 * m + 4) * 8 if m > 0 else 0))
 
 
+def counter_traffic():
+    """HBM bytes per dual contour step from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (calibrated on a copy of known size), or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", "k_line_reg_128cubed_hbm_counters.md")
+    try:
+        m = re.search(r"traffic_per_dual_step_bytes: fp64=(\d+) fp32=(\d+)", open(path).read())
+    except OSError:
+        return None
+    return None if not m else {"fp64": int(m.group(1)), "fp32": int(m.group(2))}
+
+
 def make_engine(prec, w):
     from polyswift_plusplus_amd import DiblockSCFT, fields
     shape = w["shape"]
@@ -221,6 +233,7 @@ def main():
         print("[bench] cpu baseline (oracle, 1 thread)", file=sys.stderr, flush=True)
         cpu = cpu_baseline(w)
 
+    traffic = counter_traffic()
     m = legs["fp32"]
     d = legs["fp64"]
     bp_m, bp_d = 4, 8
@@ -250,9 +263,12 @@ def main():
             "mixed": algorithmic_bytes_iter(N, 100, 4, args.depth) * m["it_s"] / (HBM_PEAK_GBS * 1e9)},
         "roofline": {"kernel": "k_line_reg: one dual contour step = z, y, x round-trip line passes",
                      "bound": "hbm", "achieved": ach_m, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach_m / HBM_PEAK_GBS, "traffic": None,
+                     "frac": ach_m / HBM_PEAK_GBS,
+                     "traffic": (traffic or {}).get("fp32") if args.grid == 128 else None,
+                     "traffic_source": "profiles/k_line_reg_128cubed_hbm_counters.md (bytes per dual step)",
                      "step_us": m["step_ms"] * 1e3, "pass_us": [p * 1e3 for p in m["pass_ms"]],
                      "fp64": {"achieved": ach_d, "frac": ach_d / HBM_PEAK_GBS,
+                              "traffic": (traffic or {}).get("fp64") if args.grid == 128 else None,
                               "step_us": d["step_ms"] * 1e3, "pass_us": [p * 1e3 for p in d["pass_ms"]]}},
         "accuracy": acc,
         "batch": batch,
