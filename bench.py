@@ -189,7 +189,7 @@ def main():
         out = {}
         for prec in ("fp64", "fp32"):
             e = make_engine(prec, w)
-            step_ms, pass_ms = e.timeStep(5)
+            step_ms, pass_ms = e.timeStep(2)
             if prec == "fp64":
                 e.slice(False, 1)
             out[prec] = dict(step_us=step_ms * 1e3, pass_us=[p * 1e3 for p in pass_ms])
@@ -205,7 +205,7 @@ def main():
         e = make_engine(prec, w)
         print(f"[bench] {prec}: warmup {args.warmup}, timing {args.steps} iterations", file=sys.stderr, flush=True)
         dt, h = time_iters(e, mix, args.warmup, args.steps)
-        step_ms, pass_ms = e.timeStep(50)
+        step_ms, pass_ms = e.timeStep(10)
         Ns = e.Ns
         legs[prec] = dict(it_s=args.steps / dt, ms_per_iter=1e3 * dt / args.steps,
                           step_ms=step_ms, pass_ms=pass_ms, F_last=float(h["F"][-1]),

@@ -415,8 +415,10 @@ extern "C" double orc_sd_update(const orc_params* P, long N, double* wA,
  *   c = U^{-1} V,  xbar = x_k + sum_i c_i (x_{k-i} - x_k),
  *   fbar = f_k + sum_i c_i (f_{k-i} - f_k),  x_{k+1} = xbar + beta fbar,
  * with x = (wA, wB), f = (dH_A, dH_B) and <a,b> = sum_r (a_A b_A + a_B b_B).
- * Depth 0 (or residual RMS above anderson_start) is the Picard step
- * x_{k+1} = x_k + lambda0 f_k, identical to steepest descent [lambda0, 0]. */
+ * The final step couples A and B like the reference's relaxlambdas[1]:
+ * x_{k+1} = xbar + beta (fbar_A - lambda1 fbar_B, fbar_B - lambda1 fbar_A).
+ * Depth 0 (or residual RMS above anderson_start) is that step with lambda0
+ * and no history: steepest descent [lambda0, lambda1]. */
 /* ------------------------------------------------------------------------ */
 namespace {
 
@@ -509,17 +511,21 @@ extern "C" int orc_scft_run(const orc_params* P, double* wA, double* wB,
         }
         ok = solve_dense(mk, U, V, c);
       }
+      // x_bar, f_bar (Anderson combination; plain x, f when no history is used)
+      std::vector<double> xb(x), fb(f);
       if (ok) {
-        for (long r = 0; r < 2 * N; ++r) {
-          double xb = x[r], fbv = f[r];
+        for (long r = 0; r < 2 * N; ++r)
           for (int i = 0; i < mk; ++i) {
-            xb += c[i] * (hx[k - 1 - i][r] - x[r]);
-            fbv += c[i] * (hf[k - 1 - i][r] - f[r]);
+            xb[r] += c[i] * (hx[k - 1 - i][r] - x[r]);
+            fb[r] += c[i] * (hf[k - 1 - i][r] - f[r]);
           }
-          xn[r] = xb + P->beta * fbv;
-        }
-      } else {
-        for (long r = 0; r < 2 * N; ++r) xn[r] = x[r] + P->lambda0 * f[r];
+      }
+      // step with the reference's A/B coupling (pseffhamil/PsSteepDUpdater.cpp:265-278):
+      // dwA = s (fA - lambda1 fB), dwB = s (fB - lambda1 fA), s = beta or lambda0
+      const double step = ok ? P->beta : P->lambda0;
+      for (long r = 0; r < N; ++r) {
+        xn[r] = xb[r] + (fb[r] + (-P->lambda1) * fb[N + r]) * step;
+        xn[N + r] = xb[N + r] + (fb[N + r] + (-P->lambda1) * fb[r]) * step;
       }
       if (m > 0) {
         hx.push_back(x);

@@ -86,107 +86,19 @@ __device__ __forceinline__ void dft_reg(C* v, const C* __restrict__ tw, int twst
   }
 }
 
-// Round trip out = post * IFFT(emul * FFT(pre * in)) along one axis, one
-// wave per 2*64/N2 real lines.  VEC: lines of a wave are consecutive in
-// memory (inner % lines-per-wave == 0), so a real pair is one aligned
-// complex load.
-template <typename T, int N1, int N2, bool CONTIG, bool VEC>
-__global__ void __launch_bounds__(64)
-k_line_reg(LineGeom g, const PairIO* __restrict__ fio) {
-  typedef typename Cx<T>::type C;
-  constexpr int n = N1 * N2;
-  constexpr int CL = 64 / N2;     // complex lines per wave
-  constexpr int RL = 2 * CL;      // real lines per wave
-  constexpr int U = N1 / N2;      // DFT_N2 per lane after the transpose
-  static_assert(N1 % N2 == 0 && 64 % N2 == 0, "bad split");
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  C* lds = (C*)smem_raw;
-  int tile, field;
-  xcd_tile(g.ntiles, g.nfields, &tile, &field);
-
-  const PairIO fd = fio[field];
-  const T* __restrict__ in = (const T*)fd.in;
-  T* __restrict__ out = (T*)fd.out;
-  const T* __restrict__ pre = (const T*)fd.pre;
-  const T* __restrict__ post = (const T*)fd.post;
-  const T* __restrict__ emul = (const T*)fd.emul;
-  const C* __restrict__ tw = (const C*)g.tw;
-
-  const int lane = threadIdx.x;
-  const int p = lane / N2, t = lane % N2;
-  const int la = tile * RL + 2 * p;   // real lines la, la + 1
-  const bool va = la < g.nlines, vb = la + 1 < g.nlines;
-  const int inner = g.inner;
-  // element k of line l: base(l) + k * stride
-  int ba, bb, stride;
-  if (CONTIG) { ba = la * n; bb = ba + n; stride = 1; }
-  else {
-    const int oa = la / inner, ob = (la + 1) / inner;
-    ba = oa * n * inner + (la - oa * inner);
-    bb = ob * n * inner + (la + 1 - ob * inner);
-    stride = inner;
-  }
-
-  C v[N1];
-  // Contiguous lines: the wave's RL rows are one contiguous block of RL*n
-  // elements.  Stage it through LDS with 16-byte loads, rows padded by
-  // N2/2 elements so that the FFT-layout reads hit distinct banks.
-  constexpr int PAD = N2 / 2;
-  constexpr int RS = n + PAD;                    // staged row stride
-  constexpr int VW = 16 / sizeof(T);             // elements per 16-byte chunk
-  if (CONTIG) {
-    T* S = (T*)smem_raw;
-    const int row0 = tile * RL;
-    const int nrows = min(RL, g.nlines - row0);
-    const T* src = in + (size_t)row0 * n;
-    const T* psrc = pre ? pre + (size_t)row0 * n : nullptr;
-    const int nchunk = nrows * n / VW;
-#pragma unroll 4
-    for (int c = lane; c < nchunk; c += 64) {
-      const int e = c * VW, row = e / n, col = e - row * n;
-      typedef typename Vec16<T>::type V;
-      V x = *(const V*)(src + e);
-      if (psrc) { const V w = *(const V*)(psrc + e); vmul(x, w); }
-      vstore_lds(S + row * RS + col, x);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < N1; ++r) {
-      const int k = t + N2 * r;
-      C x;
-      x.x = (2 * p < nrows) ? S[(2 * p) * RS + k] : (T)0;
-      x.y = (2 * p + 1 < nrows) ? S[(2 * p + 1) * RS + k] : (T)0;
-      v[r] = x;
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int r = 0; r < N1 && !CONTIG; ++r) {
-    const int k = t + N2 * r;
-    C x;
-    if (VEC) {
-      if (va) {
-        x = *(const C*)(in + ba + k * stride);
-        if (pre) { const C w = *(const C*)(pre + ba + k * stride); x.x *= w.x; x.y *= w.y; }
-      } else { x.x = (T)0; x.y = (T)0; }
-    } else {
-      x.x = va ? in[ba + k * stride] : (T)0;
-      x.y = vb ? in[bb + k * stride] : (T)0;
-      if (pre) {
-        if (va) x.x *= pre[ba + k * stride];
-        if (vb) x.y *= pre[bb + k * stride];
-      }
-    }
-    v[r] = x;
-  }
-
+// One round trip of a wave's lines held in registers (lane t of line p owns
+// points t + N2 r):  v <- IFFT(emul * FFT(v)), result in the same layout.
+// L is the wave's LDS transpose buffer (p-th line at L + p n).
+template <typename T, int N1, int N2, typename C>
+__device__ __forceinline__ void reg_roundtrip(C* v, C* L, const C* __restrict__ tw,
+                                              const T* __restrict__ emul, int t) {
+  constexpr int U = N1 / N2;
   // forward: DFT_N1 in registers, twiddle W_n^{t k1}
   dft_reg<N1, -1, T>(v, tw, N2);
 #pragma unroll
   for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], twiddle<-1>(tw, t * k1));
-
-  // transpose: element (k1, t) of line p at p*n + k1*N2 + (t ^ (k1 % N2))
-  C* L = lds + p * n;
+  // transpose: element (k1, t) at k1*N2 + (t ^ (k1 % N2)) (XOR swizzle)
+  __syncthreads();
 #pragma unroll
   for (int k1 = 0; k1 < N1; ++k1) L[k1 * N2 + (t ^ (k1 % N2))] = v[k1];
   __syncthreads();
@@ -195,7 +107,6 @@ k_line_reg(LineGeom g, const PairIO* __restrict__ fio) {
   for (int u = 0; u < U; ++u)
 #pragma unroll
     for (int s = 0; s < N2; ++s) v[u * N2 + s] = L[(t + N2 * u) * N2 + (s ^ t)];
-
   // DFT_N2, spectral multiply at k = k1 + N1 k2, inverse DFT_N2
 #pragma unroll
   for (int u = 0; u < U; ++u) {
@@ -217,54 +128,188 @@ k_line_reg(LineGeom g, const PairIO* __restrict__ fio) {
   __syncthreads();
 #pragma unroll
   for (int k1 = 0; k1 < N1; ++k1) v[k1] = L[k1 * N2 + (t ^ (k1 % N2))];
-
   // inverse: conjugate twiddle, inverse DFT_N1; lanes hold x[t + N2 r]
 #pragma unroll
   for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], twiddle<+1>(tw, t * k1));
   dft_reg<N1, +1, T>(v, tw, N2);
+}
 
+// Round trip(s) along one axis for 2*64/N2 real lines per wave:
+//   first:  u = post * IFFT(emul * FFT(pre * in))
+//   fused:  mid = u;  out = post2 * IFFT(emul2 * FFT(pre2 * u))   (emul2 != null)
+//   plain:  out = u                                               (emul2 == null)
+// The fused form joins the last pass of one contour step and the first pass
+// of the next when both run along the same axis: the step's result is stored
+// (mid) and carried on in registers.  VEC: the lines of a wave are
+// consecutive in memory (inner % lines-per-wave == 0), so a real pair is one
+// aligned complex load.
+template <typename T, int N1, int N2, bool CONTIG, bool VEC>
+__global__ void __launch_bounds__(64)
+k_line_reg(LineGeom g, const PairIO* __restrict__ fio) {
+  typedef typename Cx<T>::type C;
+  typedef typename Vec16<T>::type V;
+  constexpr int n = N1 * N2;
+  constexpr int CL = 64 / N2;     // complex lines per wave
+  constexpr int RL = 2 * CL;      // real lines per wave
+  static_assert(N1 % N2 == 0 && 64 % N2 == 0, "bad split");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* lds = (C*)smem_raw;
+  int tile, field;
+  xcd_tile(g.ntiles, g.nfields, &tile, &field);
+
+  const PairIO fd = fio[field];
+  const C* __restrict__ tw = (const C*)g.tw;
+  const bool fused = fd.emul2 != nullptr;
+
+  const int lane = threadIdx.x;
+  const int p = lane / N2, t = lane % N2;
+  const int la = tile * RL + 2 * p;   // real lines la, la + 1
+  const bool va = la < g.nlines, vb = la + 1 < g.nlines;
+  const int inner = g.inner;
+  // element k of line l: base(l) + k * stride
+  int ba, bb, stride;
+  if (CONTIG) { ba = la * n; bb = ba + n; stride = 1; }
+  else {
+    const int oa = la / inner, ob = (la + 1) / inner;
+    ba = oa * n * inner + (la - oa * inner);
+    bb = ob * n * inner + (la + 1 - ob * inner);
+    stride = inner;
+  }
+  C* L = lds + p * n;
+
+  // Contiguous lines: the wave's RL rows are one contiguous block of RL*n
+  // elements, moved through LDS with 16-byte accesses; rows are padded by
+  // N2/2 elements so that the FFT-layout accesses hit distinct banks.
+  constexpr int PAD = N2 / 2;
+  constexpr int RS = n + PAD;
+  constexpr int VW = 16 / sizeof(T);
+  T* S = (T*)smem_raw;
+  const int row0 = tile * RL;
+  const int nrows = min(RL, g.nlines - row0);
+  const int nchunk = nrows * n / VW;
+
+  C v[N1];
+  // ---- load (times pre)
   if (CONTIG) {
-    T* S = (T*)smem_raw;
-    const int row0 = tile * RL;
-    const int nrows = min(RL, g.nlines - row0);
+    const T* src = (const T*)fd.in + (size_t)row0 * n;
+    const T* psrc = fd.pre ? (const T*)fd.pre + (size_t)row0 * n : nullptr;
+#pragma unroll 4
+    for (int c = lane; c < nchunk; c += 64) {
+      const int e = c * VW, row = e / n, col = e - row * n;
+      V x = *(const V*)(src + e);
+      if (psrc) { const V w = *(const V*)(psrc + e); vmul(x, w); }
+      vstore_lds(S + row * RS + col, x);
+    }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < N1; ++r) {
       const int k = t + N2 * r;
-      if (2 * p < nrows) S[(2 * p) * RS + k] = v[r].x;
-      if (2 * p + 1 < nrows) S[(2 * p + 1) * RS + k] = v[r].y;
+      C x;
+      x.x = (2 * p < nrows) ? S[(2 * p) * RS + k] : (T)0;
+      x.y = (2 * p + 1 < nrows) ? S[(2 * p + 1) * RS + k] : (T)0;
+      v[r] = x;
     }
-    __syncthreads();
-    T* dst = out + (size_t)row0 * n;
-    const T* pp = post ? post + (size_t)row0 * n : nullptr;
-    const int nchunk = nrows * n / VW;
-#pragma unroll 4
-    for (int c = lane; c < nchunk; c += 64) {
-      const int e = c * VW, row = e / n, col = e - row * n;
-      typedef typename Vec16<T>::type V;
-      V x = vload_lds<T>(S + row * RS + col);
-      if (pp) { const V w = *(const V*)(pp + e); vmul(x, w); }
-      *(V*)(dst + e) = x;
-    }
-    return;
-  }
+  } else {
+    const T* __restrict__ in = (const T*)fd.in;
+    const T* __restrict__ pre = (const T*)fd.pre;
 #pragma unroll
-  for (int r = 0; r < N1; ++r) {
-    const int k = t + N2 * r;
-    C x = v[r];
-    if (VEC) {
-      if (va) {
-        if (post) { const C w = *(const C*)(post + ba + k * stride); x.x *= w.x; x.y *= w.y; }
-        *(C*)(out + ba + k * stride) = x;
+    for (int r = 0; r < N1; ++r) {
+      const int k = t + N2 * r;
+      C x;
+      if (VEC) {
+        if (va) {
+          x = *(const C*)(in + ba + k * stride);
+          if (pre) { const C w = *(const C*)(pre + ba + k * stride); x.x *= w.x; x.y *= w.y; }
+        } else { x.x = (T)0; x.y = (T)0; }
+      } else {
+        x.x = va ? in[ba + k * stride] : (T)0;
+        x.y = vb ? in[bb + k * stride] : (T)0;
+        if (pre) {
+          if (va) x.x *= pre[ba + k * stride];
+          if (vb) x.y *= pre[bb + k * stride];
+        }
+      }
+      v[r] = x;
+    }
+  }
+
+  reg_roundtrip<T, N1, N2>(v, L, tw, (const T*)fd.emul, t);
+
+  // ---- store: the step result (post) to mid and on through the second
+  // round trip, or straight to out
+  for (int stage = 0; stage < 2; ++stage) {
+    const bool last = stage == 1 || !fused;
+    T* __restrict__ dst = (T*)(last ? fd.out : fd.mid);
+    const T* __restrict__ post = (const T*)(stage == 0 ? fd.post : fd.post2);
+    const T* __restrict__ pre2 = last ? nullptr : (const T*)fd.pre2;
+    if (CONTIG) {
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < N1; ++r) {
+        const int k = t + N2 * r;
+        if (2 * p < nrows) S[(2 * p) * RS + k] = v[r].x;
+        if (2 * p + 1 < nrows) S[(2 * p + 1) * RS + k] = v[r].y;
+      }
+      __syncthreads();
+      T* d = dst + (size_t)row0 * n;
+      const T* pp = post ? post + (size_t)row0 * n : nullptr;
+      const T* qq = pre2 ? pre2 + (size_t)row0 * n : nullptr;
+#pragma unroll 4
+      for (int c = lane; c < nchunk; c += 64) {
+        const int e = c * VW, row = e / n, col = e - row * n;
+        V x = vload_lds<T>(S + row * RS + col);
+        if (pp) { const V w = *(const V*)(pp + e); vmul(x, w); }
+        *(V*)(d + e) = x;
+        if (!last) {
+          if (qq) { const V w = (qq == pp) ? *(const V*)(pp + e) : *(const V*)(qq + e); vmul(x, w); }
+          vstore_lds(S + row * RS + col, x);
+        }
+      }
+      if (!last) {
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < N1; ++r) {
+          const int k = t + N2 * r;
+          v[r].x = (2 * p < nrows) ? S[(2 * p) * RS + k] : (T)0;
+          v[r].y = (2 * p + 1 < nrows) ? S[(2 * p + 1) * RS + k] : (T)0;
+        }
       }
     } else {
-      if (va) out[ba + k * stride] = post ? x.x * post[ba + k * stride] : x.x;
-      if (vb) out[bb + k * stride] = post ? x.y * post[bb + k * stride] : x.y;
+#pragma unroll
+      for (int r = 0; r < N1; ++r) {
+        const int k = t + N2 * r;
+        C x = v[r];
+        if (VEC) {
+          if (va) {
+            C w;
+            if (post) { w = *(const C*)(post + ba + k * stride); x.x *= w.x; x.y *= w.y; }
+            *(C*)(dst + ba + k * stride) = x;
+            if (!last && pre2) {
+              if (pre2 != post) w = *(const C*)(pre2 + ba + k * stride);
+              x.x *= w.x; x.y *= w.y;
+            }
+          }
+        } else {
+          if (post) {
+            if (va) x.x *= post[ba + k * stride];
+            if (vb) x.y *= post[bb + k * stride];
+          }
+          if (va) dst[ba + k * stride] = x.x;
+          if (vb) dst[bb + k * stride] = x.y;
+          if (!last && pre2) {
+            if (va) x.x *= pre2[ba + k * stride];
+            if (vb) x.y *= pre2[bb + k * stride];
+          }
+        }
+        v[r] = x;
+      }
     }
+    if (last) break;
+    reg_roundtrip<T, N1, N2>(v, L, tw, (const T*)fd.emul2, t);
   }
 }
 
-// Whole line in one lane (n <= 8): no transposes.
+// Whole line in one lane (n <= 8): no transposes.  Same stages as k_line_reg.
 template <typename T, int NN, bool CONTIG>
 __global__ void __launch_bounds__(64)
 k_line_lane(LineGeom g, const PairIO* __restrict__ fio) {
@@ -272,11 +317,6 @@ k_line_lane(LineGeom g, const PairIO* __restrict__ fio) {
   int tile, field;
   xcd_tile(g.ntiles, g.nfields, &tile, &field);
   const PairIO fd = fio[field];
-  const T* __restrict__ in = (const T*)fd.in;
-  T* __restrict__ out = (T*)fd.out;
-  const T* __restrict__ pre = (const T*)fd.pre;
-  const T* __restrict__ post = (const T*)fd.post;
-  const T* __restrict__ emul = (const T*)fd.emul;
   const C* __restrict__ tw = (const C*)g.tw;
   const int la = (tile * 64 + threadIdx.x) * 2;
   const bool va = la < g.nlines, vb = la + 1 < g.nlines;
@@ -289,6 +329,8 @@ k_line_lane(LineGeom g, const PairIO* __restrict__ fio) {
     bb = ob * NN * inner + (la + 1 - ob * inner);
     stride = inner;
   }
+  const T* __restrict__ in = (const T*)fd.in;
+  const T* __restrict__ pre = (const T*)fd.pre;
   C v[NN];
 #pragma unroll
   for (int k = 0; k < NN; ++k) {
@@ -301,14 +343,31 @@ k_line_lane(LineGeom g, const PairIO* __restrict__ fio) {
     }
     v[k] = x;
   }
-  dft_reg<NN, -1, T>(v, tw, 1);
+  const bool fused = fd.emul2 != nullptr;
+  for (int stage = 0; stage < 2; ++stage) {
+    const T* __restrict__ emul = (const T*)(stage == 0 ? fd.emul : fd.emul2);
+    dft_reg<NN, -1, T>(v, tw, 1);
 #pragma unroll
-  for (int k = 0; k < NN; ++k) { const T m = emul[k]; v[k].x *= m; v[k].y *= m; }
-  dft_reg<NN, +1, T>(v, tw, 1);
+    for (int k = 0; k < NN; ++k) { const T m = emul[k]; v[k].x *= m; v[k].y *= m; }
+    dft_reg<NN, +1, T>(v, tw, 1);
+    const bool last = stage == 1 || !fused;
+    T* __restrict__ dst = (T*)(last ? fd.out : fd.mid);
+    const T* __restrict__ post = (const T*)(stage == 0 ? fd.post : fd.post2);
+    const T* __restrict__ pre2 = last ? nullptr : (const T*)fd.pre2;
 #pragma unroll
-  for (int k = 0; k < NN; ++k) {
-    if (va) out[ba + k * stride] = post ? v[k].x * post[ba + k * stride] : v[k].x;
-    if (vb) out[bb + k * stride] = post ? v[k].y * post[bb + k * stride] : v[k].y;
+    for (int k = 0; k < NN; ++k) {
+      if (post) {
+        if (va) v[k].x *= post[ba + k * stride];
+        if (vb) v[k].y *= post[bb + k * stride];
+      }
+      if (va) dst[ba + k * stride] = v[k].x;
+      if (vb) dst[bb + k * stride] = v[k].y;
+      if (pre2) {
+        if (va) v[k].x *= pre2[ba + k * stride];
+        if (vb) v[k].y *= pre2[bb + k * stride];
+      }
+    }
+    if (last) break;
   }
 }
 

@@ -39,13 +39,22 @@ __device__ __forceinline__ void xcd_tile(int ntiles, int nfields, int* tile, int
   *tile = logical - *field * ntiles;
 }
 
-// One field of a pair round-trip pass.
+// One field of a round-trip pass (T = FP64 or FP32 elements):
+//   u = post * IFFT(emul * FFT(pre * in))
+//   emul2 == null:  out = u
+//   emul2 != null:  mid = u, out = post2 * IFFT(emul2 * FFT(pre2 * u))
+// Null multipliers are 1.  out may equal in (each tile reads, then writes,
+// only its own elements).
 struct PairIO {
-  const void* in;     // T[N]
-  void* out;          // T[N]  (may equal in)
-  const void* pre;    // T[N] multiplied into the input, or null
-  const void* post;   // T[N] multiplied into the output, or null
+  const void* in;
+  void* out;
+  const void* pre;
+  const void* post;
   const void* emul;   // T[n] spectral multiplier (includes 1/n)
+  void* mid;          // fused pass: the first round trip's result
+  const void* pre2;
+  const void* emul2;
+  const void* post2;
 };
 
 __device__ __forceinline__ int line_addr(int l, int k, int n, int inner) {
@@ -67,9 +76,7 @@ k_pair_roundtrip(LineGeom g, const PairIO* __restrict__ fio) {
   xcd_tile(g.ntiles, g.nfields, &tile, &field);
   const PairIO fd = fio[field];
   const T* __restrict__ in = (const T*)fd.in;
-  T* __restrict__ out = (T*)fd.out;
   const T* __restrict__ pre = (const T*)fd.pre;
-  const T* __restrict__ post = (const T*)fd.post;
   const T* __restrict__ emul = (const T*)fd.emul;
   const C* __restrict__ tw = (const C*)g.tw;
 
@@ -108,18 +115,43 @@ k_pair_roundtrip(LineGeom g, const PairIO* __restrict__ fio) {
   C* other = (res == A) ? B : A;
   res = fft_lines<+1, T>(res, other, g.pl, P, ls, tw, tid, nt);
 
-  const T* Rr = (const T*)res;
-  for (int e = tid; e < tot; e += nt) {
-    int li, k;
-    if (CONTIG) { li = e / n; k = e - li * n; }
-    else { k = e / L; li = e - k * L; }
-    const int l = l0 + li;
-    if (l < g.nlines) {
-      const int addr = CONTIG ? l * n + k : line_addr(l, k, n, g.inner);
-      T v = Rr[2 * ((li >> 1) * ls + k) + (li & 1)];
-      if (post) v *= post[addr];
-      out[addr] = v;
+  const bool fused = fd.emul2 != nullptr;
+  for (int stage = 0; stage < 2; ++stage) {
+    const bool last = stage == 1 || !fused;
+    T* __restrict__ dst = (T*)(last ? fd.out : fd.mid);
+    const T* __restrict__ pst = (const T*)(stage == 0 ? fd.post : fd.post2);
+    const T* __restrict__ pre2 = last ? nullptr : (const T*)fd.pre2;
+    T* Rr = (T*)res;
+    for (int e = tid; e < tot; e += nt) {
+      int li, k;
+      if (CONTIG) { li = e / n; k = e - li * n; }
+      else { k = e / L; li = e - k * L; }
+      const int l = l0 + li;
+      T* slot = Rr + 2 * ((li >> 1) * ls + k) + (li & 1);
+      T v = *slot;
+      if (l < g.nlines) {
+        const int addr = CONTIG ? l * n + k : line_addr(l, k, n, g.inner);
+        if (pst) v *= pst[addr];
+        dst[addr] = v;
+        if (pre2) v *= pre2[addr];
+      }
+      if (!last) *slot = v;
     }
+    if (last) break;
+    __syncthreads();
+    const T* __restrict__ emul2 = (const T*)fd.emul2;
+    other = (res == A) ? B : A;
+    res = fft_lines<-1, T>(res, other, g.pl, P, ls, tw, tid, nt);
+    for (int e = tid; e < P * n; e += nt) {
+      const int line = e / n, k = e - line * n;
+      C v = res[line * ls + k];
+      const T m = emul2[k];
+      v.x *= m; v.y *= m;
+      res[line * ls + k] = v;
+    }
+    __syncthreads();
+    other = (res == A) ? B : A;
+    res = fft_lines<+1, T>(res, other, g.pl, P, ls, tw, tid, nt);
   }
 }
 
@@ -542,21 +574,26 @@ k_mix(QuadArgs a) {
   double acc[kNPart] = {0.0, 0.0, 0.0, 0.0};
   for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < N;
        r += (long long)gridDim.x * blockDim.x) {
-    double nw[2], old[2];
+    double xb[2], fb[2], old[2];
     for (int h = 0; h < 2; ++h) {
       const long long o = h * N + r;
       const double x = h == 0 ? a.wA[r] : a.wB[r];
       const double f = fc[o];
-      double xb = x, fb = f;
+      double xs = x, fs = f;
       for (int i = 0; i < mk; ++i) {
         const double ci = st->coef[i];
         const long long sl = (cur - 1 - i + m1) % m1;
-        xb += ci * (a.ringx[sl * 2 * N + o] - x);
-        fb += ci * (a.ringf[sl * 2 * N + o] - f);
+        xs += ci * (a.ringx[sl * 2 * N + o] - x);
+        fs += ci * (a.ringf[sl * 2 * N + o] - f);
       }
       old[h] = x;
-      nw[h] = xb + beta * fb;
+      xb[h] = xs;
+      fb[h] = fs;
     }
+    // step with the reference's A/B coupling: dwA = beta (fA - lam1 fB)
+    double nw[2];
+    nw[0] = xb[0] + (fb[0] + (-a.lam1) * fb[1]) * beta;
+    nw[1] = xb[1] + (fb[1] + (-a.lam1) * fb[0]) * beta;
     const double pr = (nw[0] + nw[1]) * 0.5;
     a.wA[r] = nw[0]; a.wB[r] = nw[1]; a.praw[r] = pr;
     ((T*)a.wfA)[r] = (T)exp(-a.halfds * nw[0]);

@@ -7,6 +7,7 @@
 // forward (q) and backward (q+) sweeps advance together in the same launches.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -429,7 +430,9 @@ struct SweepBufs {
   void* wfA = nullptr;
   void* wfB = nullptr;
   void* emul[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};  // [block][axis idx]
-  PairIO* fio = nullptr;   // device table [t][pass][2]: both sweeps of every contour step
+  PairIO* fio = nullptr;   // device table [launch][2]: both sweeps (see sweep_launches)
+  std::vector<int> launch_axis;   // axis index of every launch
+  std::vector<int> launch_nops;   // 1 plain, 2 fused
   bool wfac_valid = false;
 };
 
@@ -503,31 +506,68 @@ std::vector<double> axis_emul(const Axis& a, double cell, double ds, double b) {
   return e;
 }
 
-// Field descriptors [t][pass][2] of both sweeps for every contour step.
+// Launch list of one sweep.  Contour step s applies the active axes in the
+// order 0..na-1 when s is even and na-1..0 when s is odd (the axis operators
+// commute), so the last pass of step s and the first of step s+1 run along
+// the same axis and are fused into one launch (two round trips, the step's
+// result stored in between).  Both sweeps share the list.
+struct SweepOp { int axis, step; bool first, last; };
+
+std::vector<std::pair<SweepOp, int>> sweep_launches(int na, int Ns) {
+  std::vector<SweepOp> ops;
+  for (int st = 0; st < Ns; ++st)
+    for (int p = 0; p < na; ++p) {
+      const int a = (st % 2 == 0) ? p : na - 1 - p;
+      ops.push_back(SweepOp{a, st, p == 0, p == na - 1});
+    }
+  std::vector<std::pair<SweepOp, int>> out;   // (first op, number of ops: 1 or 2)
+  for (size_t i = 0; i < ops.size();) {
+    if (ops[i].last && i + 1 < ops.size() && ops[i + 1].axis == ops[i].axis) {
+      out.push_back({ops[i], 2});
+      i += 2;
+    } else {
+      out.push_back({ops[i], 1});
+      i += 1;
+    }
+  }
+  return out;
+}
+
+// Field descriptors [launch][2] (forward sweep, backward sweep).
 std::vector<PairIO> fio_table(const psx_scft* e, int prec) {
   const SweepBufs& s = e->sw[prec];
   const size_t tb = prec_bytes(prec);
   const int na = (int)e->axes.size();
-  std::vector<PairIO> tab((size_t)e->Ns * na * 2);
-  const char* F = (const char*)s.F;
-  const char* G = (const char*)s.G;
-  for (int t = 0; t < e->Ns; ++t) {
-    const bool fa = t < e->sA;   // forward step t lies in block A
-    const bool gb = t < e->sB;   // backward step t lies in block B
-    for (int i = 0; i < na; ++i) {
-      const bool first = i == 0, last = i == na - 1;
-      PairIO& f0 = tab[((size_t)t * na + i) * 2];
-      PairIO& f1 = tab[((size_t)t * na + i) * 2 + 1];
-      f0.in = first ? (const void*)(F + (size_t)t * e->N * tb) : s.tmpF;
-      f0.out = last ? (void*)(F + (size_t)(t + 1) * e->N * tb) : s.tmpF;
-      f0.pre = first ? (fa ? s.wfA : s.wfB) : nullptr;
-      f0.post = last ? (fa ? s.wfA : s.wfB) : nullptr;
-      f0.emul = s.emul[fa ? 0 : 1][i];
-      f1.in = first ? (const void*)(G + (size_t)t * e->N * tb) : s.tmpG;
-      f1.out = last ? (void*)(G + (size_t)(t + 1) * e->N * tb) : s.tmpG;
-      f1.pre = first ? (gb ? s.wfB : s.wfA) : nullptr;
-      f1.post = last ? (gb ? s.wfB : s.wfA) : nullptr;
-      f1.emul = s.emul[gb ? 1 : 0][i];
+  const auto launches = sweep_launches(na, e->Ns);
+  std::vector<PairIO> tab(launches.size() * 2);
+  for (int h = 0; h < 2; ++h) {
+    const char* base = (const char*)(h == 0 ? s.F : s.G);
+    void* tmp = h == 0 ? s.tmpF : s.tmpG;
+    // block of contour step st: forward A then B, backward B then A
+    auto inA = [&](int st) { return h == 0 ? st < e->sA : !(st < e->sB); };
+    auto W = [&](int st) { return inA(st) ? s.wfA : s.wfB; };
+    auto E = [&](int a, int st) { return s.emul[inA(st) ? 0 : 1][a]; };
+    auto slice = [&](int st) { return (void*)(base + (size_t)st * e->N * tb); };
+    for (size_t j = 0; j < launches.size(); ++j) {
+      const SweepOp& o = launches[j].first;
+      PairIO& f = tab[j * 2 + h];
+      std::memset(&f, 0, sizeof(f));
+      f.in = o.first ? slice(o.step) : tmp;
+      f.pre = o.first ? W(o.step) : nullptr;
+      f.emul = E(o.axis, o.step);
+      if (launches[j].second == 1) {
+        f.post = o.last ? W(o.step) : nullptr;
+        f.out = o.last ? slice(o.step + 1) : tmp;
+      } else {
+        const int s2 = o.step + 1;
+        const bool last2 = na == 1;   // the second op is also the last of its step
+        f.post = W(o.step);
+        f.mid = slice(s2);
+        f.pre2 = W(s2);
+        f.emul2 = E(o.axis, s2);
+        f.post2 = last2 ? W(s2) : nullptr;
+        f.out = last2 ? slice(s2 + 1) : tmp;
+      }
     }
   }
   return tab;
@@ -562,6 +602,12 @@ psx_status ensure_sweep(psx_scft* e, int prec) {
   {
     std::vector<PairIO> tab = fio_table(e, prec);
     PSX_TRY(upload((void**)&s.fio, tab.data(), tab.size() * sizeof(PairIO)));
+    s.launch_axis.clear();
+    s.launch_nops.clear();
+    for (const auto& l : sweep_launches((int)e->axes.size(), e->Ns)) {
+      s.launch_axis.push_back(l.first.axis);
+      s.launch_nops.push_back(l.second);
+    }
   }
   // free ends: q(head of A) = q+(tail of B) = 1 (pspolymer/PsBlock.cpp:146-147)
   const int g = grid_1d(e->N);
@@ -602,24 +648,23 @@ psx_status compute_wfac(psx_scft* e) {
 }
 
 // One contour step of both sweeps: F[t] -> F[t+1], G[t] -> G[t+1].
+// Enqueue launches [j0, j1) of the sweep (both directions in every launch);
+// only_axis >= 0 keeps the launches along that axis only (timing).
 template <typename T>
-psx_status dual_step(psx_scft* e, int t, int only_pass = -1) {
+psx_status sweep_range(psx_scft* e, size_t j0, size_t j1, int only_axis = -1) {
   SweepBufs& s = e->sw[e->prec];
-  const int na = (int)e->axes.size();
-  for (int i = 0; i < na; ++i) {
-    if (only_pass >= 0 && i != only_pass) continue;
-    PSX_TRY(launch_pair<T>(e->axes[i], e->prec, s.fio + ((size_t)t * na + i) * 2, 2, e->stream,
-                           e->variant[e->prec][i]));
+  for (size_t j = j0; j < j1; ++j) {
+    const int a = s.launch_axis[j];
+    if (only_axis >= 0 && a != only_axis) continue;
+    PSX_TRY(launch_pair<T>(e->axes[a], e->prec, s.fio + j * 2, 2, e->stream, e->variant[e->prec][a]));
   }
   return PSX_OK;
 }
 
 psx_status sweep(psx_scft* e) {
-  for (int t = 0; t < e->Ns; ++t) {
-    if (e->prec == PSX_FP64) PSX_TRY(dual_step<double>(e, t));
-    else PSX_TRY(dual_step<float>(e, t));
-  }
-  return PSX_OK;
+  const size_t nl = e->sw[e->prec].launch_axis.size();
+  if (e->prec == PSX_FP64) return sweep_range<double>(e, 0, nl);
+  return sweep_range<float>(e, 0, nl);
 }
 
 psx_status launch_Q(psx_scft* e) {
@@ -1334,17 +1379,30 @@ extern "C" psx_status psx_scft_param_grad(psx_scft* e, double* dFdchiN, double* 
   return PSX_OK;
 }
 
+// The launch that stands for axis a in tuning: a fused one if the sweep has one.
+static size_t representative_launch(const SweepBufs& s, int a) {
+  size_t best = 0;
+  bool found = false;
+  for (size_t j = 0; j < s.launch_axis.size(); ++j) {
+    if (s.launch_axis[j] != a) continue;
+    if (!found || (s.launch_nops[j] == 2 && s.launch_nops[best] != 2)) { best = j; found = true; }
+  }
+  return best;
+}
+
 extern "C" psx_status psx_scft_time_step(psx_scft* e, int reps, double* ms_step, double* ms_pass) {
   if (!e || reps < 1) return fail(PSX_ERR_INVALID, "bad argument");
   if (!e->sw[e->prec].wfac_valid) PSX_TRY(compute_wfac(e));
   hipEvent_t a, b;
   PSX_HIP(hipEventCreate(&a));
   PSX_HIP(hipEventCreate(&b));
+  // reps whole sweeps; per-step figures divide by reps * Ns
   const int na = (int)e->axes.size();
-  auto run = [&](int pass) -> psx_status {
+  const size_t nl = e->sw[e->prec].launch_axis.size();
+  auto run = [&](int axis) -> psx_status {
     for (int r = 0; r < reps; ++r) {
-      if (e->prec == PSX_FP64) PSX_TRY(dual_step<double>(e, 0, pass));
-      else PSX_TRY(dual_step<float>(e, 0, pass));
+      if (e->prec == PSX_FP64) PSX_TRY(sweep_range<double>(e, 0, nl, axis));
+      else PSX_TRY(sweep_range<float>(e, 0, nl, axis));
     }
     return PSX_OK;
   };
@@ -1355,7 +1413,8 @@ extern "C" psx_status psx_scft_time_step(psx_scft* e, int reps, double* ms_step,
   PSX_HIP(hipEventSynchronize(b));
   float ms = 0.f;
   PSX_HIP(hipEventElapsedTime(&ms, a, b));
-  if (ms_step) *ms_step = ms / reps;
+  const double per = 1.0 / ((double)reps * e->Ns);
+  if (ms_step) *ms_step = ms * per;
   for (int i = 0; i < 3; ++i) {
     double v = 0.0;
     if (i < na) {
@@ -1364,7 +1423,7 @@ extern "C" psx_status psx_scft_time_step(psx_scft* e, int reps, double* ms_step,
       PSX_HIP(hipEventRecord(b, e->stream));
       PSX_HIP(hipEventSynchronize(b));
       PSX_HIP(hipEventElapsedTime(&ms, a, b));
-      v = ms / reps;
+      v = ms * per;
     }
     if (ms_pass) ms_pass[i] = v;
   }
@@ -1426,7 +1485,7 @@ extern "C" psx_status psx_scft_autotune(psx_scft* e, int reps) {
   if (!e->sw[e->prec].wfac_valid) PSX_TRY(compute_wfac(e));
   const int na = (int)e->axes.size();
   for (int i = 0; i < na; ++i) {
-    const PairIO* fio = e->sw[e->prec].fio + (size_t)i * 2;
+    const PairIO* fio = e->sw[e->prec].fio + representative_launch(e->sw[e->prec], i) * 2;
     if (e->prec == PSX_FP64) PSX_TRY(tune_pass<double>(e->axes[i], e->prec, fio, 2, e->stream, reps, &e->variant[e->prec][i]));
     else PSX_TRY(tune_pass<float>(e->axes[i], e->prec, fio, 2, e->stream, reps, &e->variant[e->prec][i]));
   }
@@ -1458,12 +1517,12 @@ static psx_status batch_tables(psx_batch* b, int prec) {
     const int K = (int)c.members.size();
     std::vector<std::vector<PairIO>> per;
     for (int m : c.members) per.push_back(fio_table(b->eng[m], prec));
-    std::vector<PairIO> tab((size_t)b->Ns * na * 2 * K);
-    for (int t = 0; t < b->Ns; ++t)
-      for (int i = 0; i < na; ++i)
-        for (int k = 0; k < K; ++k)
-          for (int h = 0; h < 2; ++h)
-            tab[(((size_t)t * na + i) * K + k) * 2 + h] = per[k][((size_t)t * na + i) * 2 + h];
+    (void)na;
+    const size_t nl = per[0].size() / 2;
+    std::vector<PairIO> tab(nl * 2 * K);
+    for (size_t j = 0; j < nl; ++j)
+      for (int k = 0; k < K; ++k)
+        for (int h = 0; h < 2; ++h) tab[(j * K + k) * 2 + h] = per[k][j * 2 + h];
     PSX_TRY(upload((void**)&c.fio[prec], tab.data(), tab.size() * sizeof(PairIO)));
   }
   return PSX_OK;
@@ -1547,7 +1606,7 @@ extern "C" psx_status psx_batch_autotune(psx_batch* b, int reps, int* h_choice) 
     const int na = (int)e0->axes.size();
     const int K = (int)c.members.size();
     for (int i = 0; i < na; ++i) {
-      const PairIO* fio = c.fio[b->prec] + (size_t)i * 2 * K;
+      const PairIO* fio = c.fio[b->prec] + representative_launch(e0->sw[b->prec], i) * 2 * K;
       int* v = &c.variant[b->prec][i];
       if (b->prec == PSX_FP64) PSX_TRY(tune_pass<double>(e0->axes[i], b->prec, fio, 2 * K, b->stream, reps, v));
       else PSX_TRY(tune_pass<float>(e0->axes[i], b->prec, fio, 2 * K, b->stream, reps, v));
@@ -1561,14 +1620,17 @@ extern "C" psx_status psx_batch_autotune(psx_batch* b, int reps, int* h_choice) 
 
 template <typename T>
 static psx_status batch_sweep(psx_batch* b) {
-  for (int t = 0; t < b->Ns; ++t)
+  size_t nmax = 0;
+  for (auto& c : b->cls) nmax = std::max(nmax, b->eng[c.members[0]]->sw[b->prec].launch_axis.size());
+  for (size_t j = 0; j < nmax; ++j)
     for (auto& c : b->cls) {
       psx_scft* e0 = b->eng[c.members[0]];
-      const int na = (int)e0->axes.size();
+      const SweepBufs& s = e0->sw[b->prec];
+      if (j >= s.launch_axis.size()) continue;
+      const int a = s.launch_axis[j];
       const int K = (int)c.members.size();
-      for (int i = 0; i < na; ++i)
-        PSX_TRY(launch_pair<T>(e0->axes[i], b->prec, c.fio[b->prec] + ((size_t)t * na + i) * 2 * K,
-                               2 * K, b->stream, c.variant[b->prec][i]));
+      PSX_TRY(launch_pair<T>(e0->axes[a], b->prec, c.fio[b->prec] + j * 2 * K, 2 * K, b->stream,
+                             c.variant[b->prec][a]));
     }
   return PSX_OK;
 }
