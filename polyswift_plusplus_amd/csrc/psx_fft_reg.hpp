@@ -89,14 +89,16 @@ __device__ __forceinline__ void dft_reg(C* v, const C* __restrict__ tw, int twst
 // One round trip of a wave's lines held in registers (lane t of line p owns
 // points t + N2 r):  v <- IFFT(emul * FFT(v)), result in the same layout.
 // L is the wave's LDS transpose buffer (p-th line at L + p n).
+// tw: global twiddle table (uniform indices, scalar loads); twl, emul: LDS
+// copies of the twiddles and the spectral multiplier (per-lane indices).
 template <typename T, int N1, int N2, typename C>
 __device__ __forceinline__ void reg_roundtrip(C* v, C* L, const C* __restrict__ tw,
-                                              const T* __restrict__ emul, int t) {
+                                              const C* twl, const T* emul, int t) {
   constexpr int U = N1 / N2;
   // forward: DFT_N1 in registers, twiddle W_n^{t k1}
   dft_reg<N1, -1, T>(v, tw, N2);
 #pragma unroll
-  for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], twiddle<-1>(tw, t * k1));
+  for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], twiddle<-1>(twl, t * k1));
   // transpose: element (k1, t) at k1*N2 + (t ^ (k1 % N2)) (XOR swizzle)
   __syncthreads();
 #pragma unroll
@@ -130,7 +132,7 @@ __device__ __forceinline__ void reg_roundtrip(C* v, C* L, const C* __restrict__ 
   for (int k1 = 0; k1 < N1; ++k1) v[k1] = L[k1 * N2 + (t ^ (k1 % N2))];
   // inverse: conjugate twiddle, inverse DFT_N1; lanes hold x[t + N2 r]
 #pragma unroll
-  for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], twiddle<+1>(tw, t * k1));
+  for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], twiddle<+1>(twl, t * k1));
   dft_reg<N1, +1, T>(v, tw, N2);
 }
 
@@ -176,6 +178,29 @@ k_line_reg(LineGeom g, const PairIO* __restrict__ fio) {
     stride = inner;
   }
   C* L = lds + p * n;
+  // FP32: LDS copies of the per-lane tables after the transpose/staging
+  // buffer (shorter latency than L2 in the middle of the round trip).  FP64
+  // keeps them in global memory: the extra 4 KB would drop the wave below
+  // 8 waves per CU of LDS, and a 128^3 pass needs exactly 8.
+  constexpr bool TLDS = sizeof(T) == 4;
+  constexpr int BUF = (64 * N1 * 2 * (int)sizeof(T) > 2 * 64 / N2 * (n + N2 / 2) * (int)sizeof(T))
+                          ? 64 * N1 * 2 * (int)sizeof(T)
+                          : 2 * 64 / N2 * (n + N2 / 2) * (int)sizeof(T);
+  const C* twl = tw;
+  const T* e1 = (const T*)fd.emul;
+  const T* e2 = (const T*)fd.emul2;
+  if (TLDS) {
+    C* tws = (C*)(smem_raw + BUF);
+    T* e1s = (T*)(tws + n);
+    T* e2s = e1s + n;
+    for (int i = lane; i < n; i += 64) {
+      tws[i] = tw[i];
+      e1s[i] = e1[i];
+      if (fused) e2s[i] = e2[i];
+    }
+    __syncthreads();
+    twl = tws; e1 = e1s; e2 = e2s;
+  }
 
   // Contiguous lines: the wave's RL rows are one contiguous block of RL*n
   // elements, moved through LDS with 16-byte accesses; rows are padded by
@@ -233,7 +258,7 @@ k_line_reg(LineGeom g, const PairIO* __restrict__ fio) {
     }
   }
 
-  reg_roundtrip<T, N1, N2>(v, L, tw, (const T*)fd.emul, t);
+  reg_roundtrip<T, N1, N2>(v, L, tw, twl, e1, t);
 
   // ---- store: the step result (post) to mid and on through the second
   // round trip, or straight to out
@@ -305,7 +330,7 @@ k_line_reg(LineGeom g, const PairIO* __restrict__ fio) {
       }
     }
     if (last) break;
-    reg_roundtrip<T, N1, N2>(v, L, tw, (const T*)fd.emul2, t);
+    reg_roundtrip<T, N1, N2>(v, L, tw, twl, e2, t);
   }
 }
 

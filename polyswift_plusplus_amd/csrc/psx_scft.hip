@@ -192,7 +192,8 @@ void launch_reg(const Axis& a, int prec, const PairIO* fio, int nfields,
   // transpose buffer (64 N1 complex) or the staged rows (RL (n + N2/2) reals)
   const size_t tbuf = (size_t)64 * N1 * 2 * sizeof(T);
   const size_t sbuf = (size_t)RL * (N1 * N2 + N2 / 2) * sizeof(T);
-  const size_t smem = tbuf > sbuf ? tbuf : sbuf;
+  // FP32: + LDS copies of the twiddles (n complex) and two spectral multipliers
+  const size_t smem = (tbuf > sbuf ? tbuf : sbuf) + (sizeof(T) == 4 ? (size_t)N1 * N2 * 4 * sizeof(T) : 0);
   dim3 grid(g.ntiles * nfields);
   if (a.contig)
     hipLaunchKernelGGL((k_line_reg<T, N1, N2, true, false>), grid, dim3(64), smem, s, g, fio);
