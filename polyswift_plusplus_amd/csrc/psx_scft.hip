@@ -451,6 +451,13 @@ struct psx_scft {
   int dev = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  // The forward and backward sweeps are independent: they run as two
+  // branches (second stream, forked and joined with events) so that their
+  // kernels drift out of phase and one sweep's loads overlap the other's
+  // arithmetic.  Default priority like the main stream.
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool split_sweeps = true;
   int dims[3];
   long long N;
   double cell[3];
@@ -654,11 +661,25 @@ psx_status compute_wfac(psx_scft* e) {
 template <typename T>
 psx_status sweep_range(psx_scft* e, size_t j0, size_t j1, int only_axis = -1) {
   SweepBufs& s = e->sw[e->prec];
+  if (!e->split_sweeps) {
+    for (size_t j = j0; j < j1; ++j) {
+      const int a = s.launch_axis[j];
+      if (only_axis >= 0 && a != only_axis) continue;
+      PSX_TRY(launch_pair<T>(e->axes[a], e->prec, s.fio + j * 2, 2, e->stream, e->variant[e->prec][a]));
+    }
+    return PSX_OK;
+  }
+  // forward sweep on the engine stream, backward sweep on stream2
+  PSX_HIP(hipEventRecord(e->ev_fork, e->stream));
+  PSX_HIP(hipStreamWaitEvent(e->stream2, e->ev_fork, 0));
   for (size_t j = j0; j < j1; ++j) {
     const int a = s.launch_axis[j];
     if (only_axis >= 0 && a != only_axis) continue;
-    PSX_TRY(launch_pair<T>(e->axes[a], e->prec, s.fio + j * 2, 2, e->stream, e->variant[e->prec][a]));
+    PSX_TRY(launch_pair<T>(e->axes[a], e->prec, s.fio + j * 2, 1, e->stream, e->variant[e->prec][a]));
+    PSX_TRY(launch_pair<T>(e->axes[a], e->prec, s.fio + j * 2 + 1, 1, e->stream2, e->variant[e->prec][a]));
   }
+  PSX_HIP(hipEventRecord(e->ev_join, e->stream2));
+  PSX_HIP(hipStreamWaitEvent(e->stream, e->ev_join, 0));
   return PSX_OK;
 }
 
@@ -813,6 +834,9 @@ extern "C" psx_status psx_scft_destroy(psx_scft* e) {
   for (auto& p : e->kd2) dfree((void*&)p);
   for (auto& b : e->kf1) for (auto& p : b) dfree((void*&)p);
   for (auto& a : e->caxes) { dfree(a.tw[0]); dfree(a.tw[1]); }
+  if (e->stream2) { (void)hipStreamSynchronize(e->stream2); (void)hipStreamDestroy(e->stream2); }
+  if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
+  if (e->ev_join) (void)hipEventDestroy(e->ev_join);
   if (e->own_stream && e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
   return PSX_OK;
@@ -883,6 +907,12 @@ extern "C" psx_status psx_scft_create(const psx_grid* g, const psx_diblock* c,
 #define E_TRY(x) do { if (st == PSX_OK) st = (x); } while (0)
   E_TRY((hipGetDevice(&e->dev) == hipSuccess) ? PSX_OK : fail(PSX_ERR_HIP, "hipGetDevice failed"));
   E_TRY(make_stream(stream, &e->stream, &e->own_stream));
+  E_TRY(hipStreamCreateWithFlags(&e->stream2, hipStreamNonBlocking) == hipSuccess
+            ? PSX_OK : fail(PSX_ERR_HIP, "stream creation failed"));
+  E_TRY(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming) == hipSuccess &&
+                hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming) == hipSuccess
+            ? PSX_OK : fail(PSX_ERR_HIP, "event creation failed"));
+  if (const char* v = getenv("PSX_SPLIT_SWEEPS")) e->split_sweeps = v[0] != '0';
   E_TRY(make_axes(e->dims, &e->axes));
   const size_t fb = e->N * sizeof(double);
   E_TRY(dalloc((void**)&e->wA, fb));
