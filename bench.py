@@ -261,7 +261,7 @@ def main():
         "algorithmic_hbm_frac": {
             "fp64": algorithmic_bytes_iter(N, 100, 8, args.depth) * d["it_s"] / (HBM_PEAK_GBS * 1e9),
             "mixed": algorithmic_bytes_iter(N, 100, 4, args.depth) * m["it_s"] / (HBM_PEAK_GBS * 1e9)},
-        "roofline": {"kernel": "k_line_reg: one dual contour step = z, y, x round-trip line passes",
+        "roofline": {"kernel": "k_line_reg: one dual contour step (both sweeps; two line launches per sweep step: strided + fused boundary pair)",
                      "bound": "hbm", "achieved": ach_m, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach_m / HBM_PEAK_GBS,
                      "traffic": (traffic or {}).get("fp32") if args.grid == 128 else None,
