@@ -23,6 +23,38 @@ _lib = None
 _dp = ctypes.POINTER(ctypes.c_double)
 
 
+ORC_MAX_JOIN = 8
+
+
+class OrcBlock(ctypes.Structure):
+    _fields_ = [
+        ("species", ctypes.c_int), ("frac", ctypes.c_double), ("b", ctypes.c_double),
+        ("nhead", ctypes.c_int), ("ntail", ctypes.c_int),
+        ("head", ctypes.c_int * ORC_MAX_JOIN), ("tail", ctypes.c_int * ORC_MAX_JOIN),
+    ]
+
+
+@dataclass
+class Block:
+    """One block of a block graph: species 0 = A, 1 = B; lengthfrac; bSegRatio;
+    indices of the blocks joined at its head and tail (empty = freeEnd)."""
+    species: int
+    frac: float
+    b: float = 1.0
+    head: tuple = ()
+    tail: tuple = ()
+
+    def c(self) -> OrcBlock:
+        ob = OrcBlock()
+        ob.species, ob.frac, ob.b = self.species, self.frac, self.b
+        ob.nhead, ob.ntail = len(self.head), len(self.tail)
+        for i, j in enumerate(self.head):
+            ob.head[i] = j
+        for i, j in enumerate(self.tail):
+            ob.tail[i] = j
+        return ob
+
+
 class OrcParams(ctypes.Structure):
     _fields_ = [
         ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
@@ -35,6 +67,8 @@ class OrcParams(ctypes.Structure):
         ("anderson_start", ctypes.c_double),
         ("f_order", ctypes.c_int),
         ("quad", ctypes.c_int),
+        ("nblocks", ctypes.c_int),
+        ("blocks", ctypes.POINTER(OrcBlock)),
     ]
 
 
@@ -60,6 +94,7 @@ class Params:
     anderson_start: float = 1e300
     f_order: int = 0        # 0 reference (F after update), 1 consistent
     quad: int = 0           # 0 Simpson 0.33333 (reference), 1 trapezoid
+    blocks: tuple = ()      # Block graph; empty = the diblock of fA, bA, bB
 
     @property
     def N(self) -> int:
@@ -70,10 +105,16 @@ class Params:
         return (self.nx, self.ny, self.nz)
 
     def c(self) -> OrcParams:
-        return OrcParams(self.nx, self.ny, self.nz, self.dx, self.dy, self.dz,
-                         self.ds, self.fA, self.chiN, self.bA, self.bB, self.mix,
-                         self.lambda0, self.lambda1, self.depth, self.beta,
-                         self.anderson_start, self.f_order, self.quad)
+        cp = OrcParams(self.nx, self.ny, self.nz, self.dx, self.dy, self.dz,
+                       self.ds, self.fA, self.chiN, self.bA, self.bB, self.mix,
+                       self.lambda0, self.lambda1, self.depth, self.beta,
+                       self.anderson_start, self.f_order, self.quad, 0, None)
+        if self.blocks:
+            arr = (OrcBlock * len(self.blocks))(*[b.c() for b in self.blocks])
+            cp.nblocks = len(self.blocks)
+            cp.blocks = ctypes.cast(arr, ctypes.POINTER(OrcBlock))
+            cp._keep = arr      # the array lives as long as the struct
+        return cp
 
 
 def lib():
@@ -94,6 +135,10 @@ def lib():
         L.orc_quadrature.argtypes = [ctypes.c_long, ctypes.c_int, ctypes.c_double,
                                      _dp, _dp, ctypes.c_double, _dp]
         L.orc_diblock_densities.argtypes = [ctypes.POINTER(OrcParams)] + [_dp] * 5
+        L.orc_graph_densities.argtypes = [ctypes.POINTER(OrcParams)] + [_dp] * 5
+        L.orc_graph_slice_count.argtypes = [ctypes.POINTER(OrcParams)]
+        L.orc_graph_slice_count.restype = ctypes.c_long
+        L.orc_species_fracs.argtypes = [ctypes.POINTER(OrcParams), _dp, _dp]
         L.orc_sd_update.argtypes = [ctypes.POINTER(OrcParams), ctypes.c_long] + [_dp] * 5 + [ctypes.c_double]
         L.orc_sd_update.restype = ctypes.c_double
         L.orc_free_energy.argtypes = [ctypes.POINTER(OrcParams), ctypes.c_long] + [_dp] * 4 + [ctypes.c_double]
@@ -199,6 +244,43 @@ def diblock_densities(P: Params, wA, wB, want_slices: bool = False):
         qtA = sl[i0:i0 + sA + 1]
         return phiA, phiB, Q.value, (qA, qB, qtB, qtA)
     return phiA, phiB, Q.value
+
+
+def graph_densities(P: Params, wA, wB, want_slices: bool = False):
+    """Densities and Q of the block graph (pspolymer/PsBlockCopolymer.cpp:189-313);
+    with want_slices, also [(q from head, q from tail)] per block, each
+    (steps+1, *shape) and indexed from its own starting end."""
+    wA, wB = _f64(wA), _f64(wB)
+    phiA, phiB = np.empty_like(wA), np.empty_like(wB)
+    Q = ctypes.c_double()
+    cp = P.c()
+    total = lib().orc_graph_slice_count(ctypes.byref(cp))
+    if total < 0:
+        raise ValueError("bad block graph")
+    sl = np.empty((total,) + wA.shape) if want_slices else None
+    rc = lib().orc_graph_densities(ctypes.byref(cp), _p(wA), _p(wB), _p(phiA), _p(phiB),
+                                   ctypes.cast(ctypes.byref(Q), _dp),
+                                   _p(sl) if want_slices else None)
+    if rc:
+        raise RuntimeError(f"oracle graph densities failed rc={rc}")
+    if not want_slices:
+        return phiA, phiB, Q.value
+    out, i0 = [], 0
+    blocks = P.blocks or (Block(0, P.fA, P.bA, (), (1,)), Block(1, 1.0 - P.fA, P.bB, (0,), ()))
+    for b in blocks:
+        M = block_steps(b.frac, P.ds) + 1
+        out.append((sl[i0:i0 + M], sl[i0 + M:i0 + 2 * M]))
+        i0 += 2 * M
+    return phiA, phiB, Q.value, out
+
+
+def species_fracs(P: Params):
+    """(f_A, f_B): lengthfrac summed over each species' blocks."""
+    a, b = ctypes.c_double(), ctypes.c_double()
+    cp = P.c()
+    lib().orc_species_fracs(ctypes.byref(cp), ctypes.cast(ctypes.byref(a), _dp),
+                            ctypes.cast(ctypes.byref(b), _dp))
+    return a.value, b.value
 
 
 def sd_update(P: Params, wA, wB, p, phiA, phiB, Q):

@@ -364,13 +364,190 @@ extern "C" int orc_diblock_densities(const orc_params* P, const double* wA,
   return 0;
 }
 
+/* ------------------------------------------------------------------------ */
+/* General block graphs (pspolymer/PsBlockCopolymer.cpp:132-313,
+ * pspolymer/PsBlock.cpp:136-147,154-237,387-472, psbase/PsBlockBase.cpp:205-240,
+ * 352-420).  Each block has a propagator from its head and one from its
+ * tail.  A free end starts at q0 = 1; a joined end starts at the product of
+ * the final propagators that the blocks joined there publish to it, taken in
+ * ascending block index (the std::map order of qHeadJnts/qTailJnts).  Blocks
+ * are swept in index order, head end then tail end, until every block has
+ * both finals. */
+/* ------------------------------------------------------------------------ */
+namespace {
+
+enum { kHead = 0, kTail = 1 };
+
+struct GBlock {
+  int species = 0;
+  double frac = 0.0, b = 1.0;
+  int steps = 0;
+  std::vector<std::pair<int, int>> cnt[2];      // (block, its end) joined at each end
+  double* q[2] = {nullptr, nullptr};            // M slices propagated from end e
+  bool initSet[2] = {false, false};             // initial q at end e
+  bool finalSet[2] = {false, false};            // final q arriving at end e
+  std::map<int, const double*> jnt[2];          // published finals at end e, by block
+};
+
+std::vector<orc_block> graph_desc(const orc_params* P) {
+  if (P->nblocks > 0) return std::vector<orc_block>(P->blocks, P->blocks + P->nblocks);
+  orc_block a{}, b{};
+  a.species = 0; a.frac = P->fA; a.b = P->bA; a.nhead = 0; a.ntail = 1; a.tail[0] = 1;
+  b.species = 1; b.frac = 1.0 - P->fA; b.b = P->bB; b.nhead = 1; b.head[0] = 0; b.ntail = 0;
+  return {a, b};
+}
+
+// Blocks, steps and resolved junctions; false on a bad description.
+bool build_graph(const orc_params* P, std::vector<GBlock>& B) {
+  const std::vector<orc_block> d = graph_desc(P);
+  const int nb = (int)d.size();
+  B.assign(nb, GBlock());
+  for (int i = 0; i < nb; ++i) {
+    if (d[i].species != 0 && d[i].species != 1) return false;
+    if (d[i].nhead < 0 || d[i].ntail < 0 || d[i].nhead > ORC_MAX_JOIN || d[i].ntail > ORC_MAX_JOIN)
+      return false;
+    B[i].species = d[i].species;
+    B[i].frac = d[i].frac;
+    B[i].b = d[i].b;
+    B[i].steps = orc_block_steps(d[i].frac, P->ds);
+    if (B[i].steps < 0) return false;
+  }
+  // A block j named at end e of block i is joined at j's tail when j's tail
+  // list names i, else at j's head (PsBlockCopolymer::buildCntBlockType).
+  for (int i = 0; i < nb; ++i)
+    for (int e = 0; e < 2; ++e) {
+      const int n = e == kHead ? d[i].nhead : d[i].ntail;
+      const int* lst = e == kHead ? d[i].head : d[i].tail;
+      for (int k = 0; k < n; ++k) {
+        const int j = lst[k];
+        if (j < 0 || j >= nb || j == i) return false;
+        for (const auto& c : B[i].cnt[e])
+          if (c.first == j) return false;
+        bool at_tail = false;
+        for (int m = 0; m < d[j].ntail; ++m)
+          if (d[j].tail[m] == i) at_tail = true;
+        B[i].cnt[e].push_back({j, at_tail ? kTail : kHead});
+      }
+    }
+  return true;
+}
+
+}  // namespace
+
+extern "C" void orc_species_fracs(const orc_params* P, double* fA, double* fB) {
+  // f_alpha = sum of lengthfrac * volfrac over the species' blocks
+  // (pspolymer/PsPolymerHldr.cpp:69-83 with volfrac = 1)
+  double a = 0.0, b = 0.0;
+  for (const orc_block& d : graph_desc(P)) (d.species == 0 ? a : b) += d.frac;
+  if (fA) *fA = a;
+  if (fB) *fB = b;
+}
+
+extern "C" long orc_graph_slice_count(const orc_params* P) {
+  std::vector<GBlock> B;
+  if (!build_graph(P, B)) return -1;
+  long t = 0;
+  for (const GBlock& g : B) t += 2L * (g.steps + 1);
+  return t;
+}
+
+extern "C" int orc_graph_densities(const orc_params* P, const double* wA,
+                                   const double* wB, double* phiA, double* phiB,
+                                   double* Qout, double* slices) {
+  const int nx = P->nx, ny = P->ny, nz = P->nz;
+  const long N = ncells(P);
+  std::vector<GBlock> B;
+  if (!build_graph(P, B)) return -1;
+  const int nb = (int)B.size();
+  long total = 0;
+  for (const GBlock& g : B) total += 2L * (g.steps + 1);
+  std::vector<double> own;
+  double* S = slices;
+  if (!S) { own.resize(total * N); S = own.data(); }
+  for (int i = 0; i < nb; ++i) {
+    const long M = B[i].steps + 1;
+    B[i].q[kHead] = S;
+    B[i].q[kTail] = S + M * N;
+    S += 2 * M * N;
+  }
+  // kfac per block (pspolymer/PsFlexPseudoSpec.cpp:188-206), w-factor per species
+  std::vector<std::vector<double>> kf(nb, std::vector<double>(N));
+  for (int i = 0; i < nb; ++i)
+    orc_kfac(nx, ny, nz, P->dx, P->dy, P->dz, P->ds, B[i].b, kf[i].data());
+  std::vector<double> wf[2] = {std::vector<double>(N), std::vector<double>(N)};
+  const double ds2 = -0.5 * P->ds;
+  for (long n = 0; n < N; ++n) { wf[0][n] = std::exp(ds2 * wA[n]); wf[1][n] = std::exp(ds2 * wB[n]); }
+  // reset: free ends start at q0 = 1 (pspolymer/PsBlock.cpp:136-147)
+  for (GBlock& g : B)
+    for (int e = 0; e < 2; ++e)
+      if (g.cnt[e].empty()) {
+        for (long r = 0; r < N; ++r) g.q[e][r] = 1.0;
+        g.initSet[e] = true;
+      }
+  // PsBlockCopolymer::update / updateBlockQ / publishQFrom
+  for (;;) {
+    bool all = true, progress = false;
+    for (int n = 0; n < nb; ++n)
+      for (int e = 0; e < 2; ++e) {
+        GBlock& g = B[n];
+        const int o = 1 - e;
+        if (!g.initSet[e]) {
+          bool ready = true;
+          for (const auto& c : g.cnt[e])
+            if (!g.jnt[e].count(c.first)) ready = false;
+          if (ready) {
+            // combineSetJnt: 1 * q_j1 * q_j2 * ... in block-index order
+            double* q0 = g.q[e];
+            for (long r = 0; r < N; ++r) q0[r] = 1.0;
+            for (const auto& kv : g.jnt[e])
+              for (long r = 0; r < N; ++r) q0[r] *= kv.second[r];
+            g.initSet[e] = true;
+          }
+        }
+        if (g.initSet[e] && !g.finalSet[o]) {
+          if (orc_solve_q(nx, ny, nz, kf[n].data(), wf[g.species].data(), g.q[e], g.steps, g.q[e]))
+            return -1;
+          g.finalSet[o] = true;
+          progress = true;
+          const double* fin = g.q[e] + (long)g.steps * N;
+          for (const auto& c : g.cnt[o]) B[c.first].jnt[c.second][n] = fin;
+        }
+      }
+    for (const GBlock& g : B)
+      if (!g.finalSet[0] || !g.finalSet[1]) all = false;
+    if (all) break;
+    if (!progress) return -1;   // a cycle or a junction nothing reaches
+  }
+  // Q from block 0's head: qHeadInitial * qHeadFinal (pspolymer/PsBlock.cpp:205-237)
+  const GBlock& b0 = B[0];
+  const double* qhi = b0.q[kHead];
+  const double* qhf = b0.q[kTail] + (long)b0.steps * N;
+  double sum = 0.0;
+  for (long n = 0; n < N; ++n) sum += qhi[n] * qhf[n];
+  const double Q = sum / (double)N;
+  if (Q != Q) return -2;
+  *Qout = Q;
+  // densities: each block adds its qqt into its species field
+  // (pspolymer/PsBlock.cpp:154-196, densWt = vf = lengthRatio = 1)
+  for (long r = 0; r < N; ++r) { phiA[r] = 0.0; phiB[r] = 0.0; }
+  std::vector<double> tmp(N);
+  for (const GBlock& g : B) {
+    if (P->quad == 1) orc_quadrature_trap(N, g.steps, P->ds, g.q[kHead], g.q[kTail], Q, tmp.data());
+    else orc_quadrature(N, g.steps, P->ds, g.q[kHead], g.q[kTail], Q, tmp.data());
+    double* phi = g.species == 0 ? phiA : phiB;
+    for (long r = 0; r < N; ++r) phi[r] += tmp[r];
+  }
+  return 0;
+}
+
 extern "C" double orc_free_energy(const orc_params* P, long N,
                                   const double* wA, const double* wB,
                                   const double* phiA, const double* phiB,
                                   double Q) {
   // pseffhamil/PsPolymerUpdater.cpp:173-226, calcUpdateFieldProd 303-341,
   // pseffhamil/PsFloryInteraction.cpp:275-321 (calcDisorder = true), bulk.
-  const double fA = P->fA, fB = 1.0 - P->fA;
+  double fA, fB;
+  orc_species_fracs(P, &fA, &fB);
   const double vol = (double)N;
   double prod = 0.0, inter = 0.0;
   for (long r = 0; r < N; ++r) {
@@ -384,7 +561,8 @@ extern "C" double orc_sd_update(const orc_params* P, long N, double* wA,
                                 double* wB, double* p, const double* phiA,
                                 const double* phiB, double Q) {
   // pseffhamil/PsSteepDUpdater.cpp:159-203 with update_set2Fields 250-333
-  const double fA = P->fA, fB = 1.0 - P->fA;
+  double fA, fB;
+  orc_species_fracs(P, &fA, &fB);
   std::vector<double> dHA(N), dHB(N);
   for (long r = 0; r < N; ++r) {
     dHA[r] = p[r] - wA[r] + P->chiN * (phiB[r] - fB);
@@ -462,7 +640,8 @@ extern "C" int orc_scft_run(const orc_params* P, double* wA, double* wB,
                             double* phiA_out, double* phiB_out) {
   const long N = ncells(P);
   std::vector<double> phiA(N), phiB(N);
-  const double fA = P->fA, fB = 1.0 - P->fA;
+  double fA, fB;
+  orc_species_fracs(P, &fA, &fB);
   const int m = P->mix == 1 ? P->depth : 0;
   // history ring (x, f) of the last m iterates
   std::vector<std::vector<double>> hx, hf;
@@ -470,7 +649,8 @@ extern "C" int orc_scft_run(const orc_params* P, double* wA, double* wB,
   for (int it = 0; it < n_iters; ++it) {
     double Q;
     // pstrol/PsDomain.cpp:332-372: densities reset, propagate, Q, densities
-    int rc = orc_diblock_densities(P, wA, wB, phiA.data(), phiB.data(), &Q, nullptr);
+    int rc = P->nblocks > 0 ? orc_graph_densities(P, wA, wB, phiA.data(), phiB.data(), &Q, nullptr)
+                            : orc_diblock_densities(P, wA, wB, phiA.data(), phiB.data(), &Q, nullptr);
     if (rc) return -1;
     // residual f = dH (pseffhamil/PsSteepDUpdater.cpp:174-184, Flory calcDfD)
     std::vector<double> f(2 * N), x(2 * N);
@@ -571,27 +751,17 @@ extern "C" int orc_box_stress(const orc_params* P, const double* wA,
                               const double* wB, double* dFdL) {
   const int nx = P->nx, ny = P->ny, nz = P->nz;
   const long N = ncells(P);
-  const int sA = orc_block_steps(P->fA, P->ds);
-  const int sB = orc_block_steps(1.0 - P->fA, P->ds);
-  if (sA < 0 || sB < 0) return -1;
-  const long total = (long)(sA + 1) * 2 + (long)(sB + 1) * 2;
+  std::vector<GBlock> B;
+  if (!build_graph(P, B)) return -1;
+  const long total = orc_graph_slice_count(P);
   std::vector<double> S(total * N), phiA(N), phiB(N);
   double Q;
-  if (orc_diblock_densities(P, wA, wB, phiA.data(), phiB.data(), &Q, S.data())) return -1;
-  const double* qA = S.data();
-  const double* qB = qA + (long)(sA + 1) * N;
-  const double* qtB = qB + (long)(sB + 1) * N;
-  const double* qtA = qtB + (long)(sB + 1) * N;
-  const int Ns = sA + sB;
+  if (orc_graph_densities(P, wA, wB, phiA.data(), phiB.data(), &Q, S.data())) return -1;
   const double twopi = 2.0 * std::acos(-1.0);
   const int g[3] = {nx, ny, nz};
   const double h[3] = {P->dx, P->dy, P->dz};
-  // per-mode k_d^2 and the two kfac tables
   std::vector<double> kd2[3];
   for (int d = 0; d < 3; ++d) kd2[d].resize(N);
-  std::vector<double> KA(N), KB(N);
-  orc_kfac(nx, ny, nz, P->dx, P->dy, P->dz, P->ds, P->bA, KA.data());
-  orc_kfac(nx, ny, nz, P->dx, P->dy, P->dz, P->ds, P->bB, KB.data());
   long n = 0;
   for (int i = 0; i < nx; ++i)
     for (int j = 0; j < ny; ++j)
@@ -604,25 +774,34 @@ extern "C" int orc_box_stress(const orc_params* P, const double* wA,
           kd2[d][n] = kv * kv;
         }
       }
-  std::vector<double> wfA(N), wfB(N);
-  for (long r = 0; r < N; ++r) { wfA[r] = std::exp(-0.5 * P->ds * wA[r]); wfB[r] = std::exp(-0.5 * P->ds * wB[r]); }
+  std::vector<double> wf[2] = {std::vector<double>(N), std::vector<double>(N)};
+  for (long r = 0; r < N; ++r) {
+    wf[0][r] = std::exp(-0.5 * P->ds * wA[r]);
+    wf[1][r] = std::exp(-0.5 * P->ds * wB[r]);
+  }
   double acc[3] = {0, 0, 0};
   std::vector<cd> a(N), b(N);
-  for (int t = 0; t < Ns; ++t) {
-    // forward slice q_t along the whole chain, backward slice q+_{Ns-1-t}
-    const bool inA = t < sA;
-    const double* qt = inA ? qA + (long)t * N : qB + (long)(t - sA) * N;
-    const int u = Ns - 1 - t;  // backward steps taken (from the B tail)
-    const double* qb = (u < sB) ? qtB + (long)u * N : qtA + (long)(u - sB) * N;
-    const double* wf = inA ? wfA.data() : wfB.data();
-    const double* K = inA ? KA.data() : KB.data();
-    const double bb = inA ? P->bA * P->bA : P->bB * P->bB;
-    for (long r = 0; r < N; ++r) { a[r] = cd(wf[r] * qt[r], 0.0); b[r] = cd(wf[r] * qb[r], 0.0); }
-    fft3(a.data(), nx, ny, nz, false);
-    fft3(b.data(), nx, ny, nz, false);
-    for (long r = 0; r < N; ++r) {
-      const double s = (std::conj(b[r]) * a[r]).real() * K[r] * P->ds * bb * 2.0;
-      for (int d = 0; d < 3; ++d) acc[d] += s * kd2[d][r];
+  std::vector<double> K(N);
+  const double* base = S.data();
+  for (const GBlock& blk : B) {
+    // step t of the block: q_t from the head, q+ after steps-1-t steps from the tail
+    const long M = blk.steps + 1;
+    const double* qh = base;
+    const double* qt = base + M * N;
+    base += 2 * M * N;
+    orc_kfac(nx, ny, nz, P->dx, P->dy, P->dz, P->ds, blk.b, K.data());
+    const double* w = wf[blk.species].data();
+    const double bb = blk.b * blk.b;
+    for (int t = 0; t < blk.steps; ++t) {
+      const double* q1 = qh + (long)t * N;
+      const double* q2 = qt + (long)(blk.steps - 1 - t) * N;
+      for (long r = 0; r < N; ++r) { a[r] = cd(w[r] * q1[r], 0.0); b[r] = cd(w[r] * q2[r], 0.0); }
+      fft3(a.data(), nx, ny, nz, false);
+      fft3(b.data(), nx, ny, nz, false);
+      for (long r = 0; r < N; ++r) {
+        const double s = (std::conj(b[r]) * a[r]).real() * K[r] * P->ds * bb * 2.0;
+        for (int d = 0; d < 3; ++d) acc[d] += s * kd2[d][r];
+      }
     }
   }
   for (int d = 0; d < 3; ++d) {

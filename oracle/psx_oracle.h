@@ -35,7 +35,21 @@
 extern "C" {
 #endif
 
-/* Run parameters of one AB diblock melt (bulk, no walls, no solvent). */
+/* One block of a general block graph (pspolymer/PsBlockCopolymer.cpp,
+ * psbase/PsBlockBase.cpp:110-131,205-240): species 0 = A, 1 = B (the Block's
+ * scfield), lengthfrac, bSegRatio, and the indices of the blocks joined at
+ * its head and tail (headjoined / tailjoined; none = freeEnd). */
+#define ORC_MAX_JOIN 8
+typedef struct orc_block {
+  int species;
+  double frac;
+  double b;
+  int nhead, ntail;
+  int head[ORC_MAX_JOIN], tail[ORC_MAX_JOIN];
+} orc_block;
+
+/* Run parameters of one AB melt (bulk, no walls, no solvent): the diblock of
+ * fA, bA, bB, or the block graph `blocks` when nblocks > 0. */
 typedef struct orc_params {
   int nx, ny, nz;        /* grid cells; nz = 1 (and ny = 1) for 2D (1D) */
   double dx, dy, dz;     /* cell sizes in Rg (reference "cellSizes") */
@@ -50,6 +64,8 @@ typedef struct orc_params {
   double anderson_start; /* use Picard while residual RMS > this */
   int f_order;           /* 0: F after the w update (reference); 1: consistent */
   int quad;              /* 0: Simpson 0.33333 (reference); 1: trapezoid */
+  int nblocks;           /* 0: diblock of fA; > 0: the block graph below */
+  const orc_block* blocks;
 } orc_params;
 
 /* Thread count used by the FFT line loops and element-wise loops (OpenMP). */
@@ -93,6 +109,23 @@ void orc_quadrature_trap(long n, int steps, double ds, const double* qs,
 int orc_diblock_densities(const orc_params* P, const double* wA,
                           const double* wB, double* phiA, double* phiB,
                           double* Q, double* slices);
+
+/* Full propagation of a block graph (pspolymer/PsBlockCopolymer.cpp:189-313,
+ * pspolymer/PsBlock.cpp:146-147,154-237,387-472): fills the species densities
+ * and Q.  If slices != NULL it receives, block by block, the M = steps+1
+ * slices propagated from the head (index from the head) followed by the M
+ * slices propagated from the tail (index from the tail).  Returns -1 on a bad
+ * graph (block steps, indices, a cycle), -2 on NaN Q.  Works for the diblock
+ * (nblocks = 0) too, with blocks A = 0 and B = 1. */
+int orc_graph_densities(const orc_params* P, const double* wA,
+                        const double* wB, double* phiA, double* phiB,
+                        double* Q, double* slices);
+
+/* Total slices that orc_graph_densities writes (in fields of N), or -1. */
+long orc_graph_slice_count(const orc_params* P);
+
+/* Species totals f_A, f_B (sum of lengthfrac over the species' blocks). */
+void orc_species_fracs(const orc_params* P, double* fA, double* fB);
 
 /* One steepest-descent update + pressure shift + F, in reference order
  * (pseffhamil/PsSteepDUpdater.cpp:159-203).  Updates wA, wB, p in place. */
