@@ -84,6 +84,24 @@ typedef struct psx_diblock {
   double chiN;           /* chi * scaleLength (PsFloryInteraction.cpp:160-161) */
 } psx_diblock;
 
+/* One block of a general block copolymer (a Block of a blockCopolymer
+ * Polymer, pspolymer/PsBlockCopolymer.cpp:60-150, psbase/PsBlockBase.cpp:
+ * 110-131): species 0 = A, 1 = B (the Block's scfield), lengthfrac,
+ * bSegRatio, and the blocks joined at its head and tail ("headjoined" /
+ * "tailjoined" by index; none = freeEnd).  A block j named at an end of
+ * block i is joined at j's tail if j's tail list names i, else at j's head
+ * (PsBlockCopolymer::buildCntBlockType, pspolymer/PsBlockCopolymer.cpp:
+ * 157-182).  Trees only: linear multiblocks, stars, combs. */
+#define PSX_MAX_BLOCKS 16
+#define PSX_MAX_JOIN 8
+typedef struct psx_block {
+  int species;
+  double lengthfrac;
+  double bSegRatio;
+  int nhead, ntail;
+  int head[PSX_MAX_JOIN], tail[PSX_MAX_JOIN];
+} psx_block;
+
 typedef struct psx_scft psx_scft;
 
 /* Build: block steps (psbase/PsBlockBase.cpp:252-326, error
@@ -92,10 +110,34 @@ typedef struct psx_scft psx_scft;
  * contour slices.  Fields start at w = 0, p = 0. */
 psx_status psx_scft_create(const psx_grid* grid, const psx_diblock* chain,
                            psx_prec prec, void* stream, psx_scft** out);
+/* Same for a general block graph of A and B blocks sharing one ds
+ * (PsBlockCopolymer with flexPseudoSpec blocks; volfrac = 1, the polymer's
+ * length is the scale length).  Propagation follows PsBlockCopolymer::update
+ * (pspolymer/PsBlockCopolymer.cpp:189-313): free ends start at q = 1, a
+ * joined end at the product of the connected blocks' final propagators
+ * (PsBlock::combineSetJnt, pspolymer/PsBlock.cpp:387-417); Q is block 0's
+ * mean(qHeadInitial qHeadFinal).  PSX_ERR_STEPS for a non-integral block,
+ * PSX_ERR_INVALID for a bad index, a repeated join, or a graph with a cycle
+ * or an unreachable junction. */
+psx_status psx_scft_create_blocks(const psx_grid* grid, int nblocks,
+                                  const psx_block* blocks, double ds,
+                                  double chiN, psx_prec prec, void* stream,
+                                  psx_scft** out);
 psx_status psx_scft_destroy(psx_scft* e);
+/* Block count, propagation paths (stored slice arrays) and rounds (sets of
+ * paths that run concurrently; a path in round r starts from finals of
+ * paths of earlier rounds). */
+psx_status psx_scft_graph_info(const psx_scft* e, int* h_nblocks,
+                               int* h_npaths, int* h_nrounds);
+/* Slice s of block b's propagator from its head (from_tail = 0, s counted
+ * from the head) or from its tail (s counted from the tail), FP64 out
+ * (the reference's qs / qts of PsBlock, pspolymer/PsBlock.cpp:256-300). */
+psx_status psx_scft_get_block_slice_host(psx_scft* e, int block,
+                                         int from_tail, int s, double* h_out);
 /* Switch the sweep precision (allocates the other slice set on first use). */
 psx_status psx_scft_set_precision(psx_scft* e, psx_prec prec);
-/* Block steps of A and B. */
+/* Contour steps of the A blocks and of the B blocks (for a diblock, the
+ * block steps of A and B). */
 psx_status psx_scft_block_steps(const psx_scft* e, int* sA, int* sB);
 
 /* Conjugate fields w_A, w_B and pressure p, FP64 device arrays of N cells
@@ -123,9 +165,11 @@ psx_status psx_scft_densities(psx_scft* e, double* d_phiA, double* d_phiB);
 /* Host copies of the densities of the last densities()/iterate() call. */
 psx_status psx_scft_get_densities_host(psx_scft* e, double* h_phiA,
                                        double* h_phiB);
-/* Copy one stored contour slice (FP64 out; slices of the forward sweep are
- * q_A[0..sA] then q_B[1..sB]; backward: q+_B[0..sB] then q+_A[1..sA]). */
-psx_status psx_scft_get_slice_host(psx_scft* e, int backward, int t,
+/* Copy slice t of propagation path `path` (FP64 out).  For a linear chain,
+ * path 0 is the forward sweep (q of block 0 from its head, then on through
+ * the chain) and path 1 the backward sweep; for the diblock: path 0 holds
+ * q_A[0..sA] then q_B[1..sB], path 1 q+_B[0..sB] then q+_A[1..sA]. */
+psx_status psx_scft_get_slice_host(psx_scft* e, int path, int t,
                                    double* h_out);
 
 /* One steepest-descent update from the current densities
@@ -184,7 +228,9 @@ psx_status psx_scft_set_cell(psx_scft* e, double dx, double dy, double dz);
  * and dF/dfA = -(1/Q) mean(q(fA) q+(fA) (wB - wA)), the derivative with
  * respect to the junction position (the discrete F moves in whole contour
  * steps, so finite differences agree to O(ds^2)).  Requires a preceding
- * propagate and densities.  No reference counterpart. */
+ * propagate and densities.  dF/dfA is defined for the AB diblock only
+ * (PSX_ERR_INVALID otherwise; pass NULL to get dF/dchiN of any graph).
+ * No reference counterpart. */
 psx_status psx_scft_param_grad(psx_scft* e, double* h_dFdchiN,
                                double* h_dFdfA);
 

@@ -34,6 +34,16 @@ class Diblock(ctypes.Structure):
                 ("bB", ctypes.c_double), ("chiN", ctypes.c_double)]
 
 
+PSX_MAX_BLOCKS = 16
+PSX_MAX_JOIN = 8
+
+
+class BlockDesc(ctypes.Structure):
+    _fields_ = [("species", ctypes.c_int), ("lengthfrac", ctypes.c_double),
+                ("bSegRatio", ctypes.c_double), ("nhead", ctypes.c_int), ("ntail", ctypes.c_int),
+                ("head", ctypes.c_int * PSX_MAX_JOIN), ("tail", ctypes.c_int * PSX_MAX_JOIN)]
+
+
 class Mixing(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("lambda0", ctypes.c_double),
                 ("lambda1", ctypes.c_double), ("depth", ctypes.c_int),
@@ -52,7 +62,12 @@ EXPORTS = {
     "psx_fft_scaled_pair_host": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "psx_scft_create": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Diblock), ctypes.c_int,
                                        _vp, ctypes.POINTER(_vp)]),
+    "psx_scft_create_blocks": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.c_int, ctypes.POINTER(BlockDesc),
+                                              ctypes.c_double, ctypes.c_double, ctypes.c_int, _vp,
+                                              ctypes.POINTER(_vp)]),
     "psx_scft_destroy": (ctypes.c_int, [_vp]),
+    "psx_scft_graph_info": (ctypes.c_int, [_vp, _ip, _ip, _ip]),
+    "psx_scft_get_block_slice_host": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
     "psx_scft_set_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
     "psx_scft_block_steps": (ctypes.c_int, [_vp, _ip, _ip]),
     "psx_scft_set_fields": (ctypes.c_int, [_vp, _vp, _vp, _vp]),

@@ -284,7 +284,43 @@ k_partial_sum(const T* __restrict__ x, long long N, double* __restrict__ part) {
   if (threadIdx.x == 0) part[blockIdx.x] = v[0];
 }
 
-// Q = sum(part) / N  (PsBlock::calcBigQ with qHeadInitial = 1).
+// part[block] = sum of x y over a grid-stride range, in FP64.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_partial_dot(const T* __restrict__ x, const T* __restrict__ y, long long N,
+              double* __restrict__ part) {
+  double v[1] = {0.0};
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long long)gridDim.x * blockDim.x)
+    v[0] += (double)x[i] * (double)y[i];
+  block_sum<1>(v);
+  if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+// Initial q of a joined block end: the product of the final propagators the
+// blocks joined there publish, in ascending block index
+// (pspolymer/PsBlock.cpp:387-417).
+constexpr int kMaxJoin = 8;
+struct JunctionArgs {
+  void* out;
+  const void* in[kMaxJoin];
+  int nin;
+  long long N;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_junction(JunctionArgs a) {
+  T* __restrict__ out = (T*)a.out;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < a.N;
+       i += (long long)gridDim.x * blockDim.x) {
+    T v = (T)1;
+    for (int k = 0; k < a.nin; ++k) v *= ((const T*)a.in[k])[i];
+    out[i] = v;
+  }
+}
+
+// Q = sum(part) / N  (PsBlock::calcBigQ: mean of qHeadInitial qHeadFinal).
 __global__ void k_finish_Q(const double* __restrict__ part, int nb, double invN,
                            DevState* st) {
   double v[1] = {0.0};
@@ -339,12 +375,21 @@ enum QuadMode {
 };
 constexpr int kNPart = 4;   // sum p_raw, sum(-phi w), sum(chiN phiA phiB), sum dH^2
 
+// One block of the contour quadrature: M = steps + 1 slices of the
+// propagator from the head (H[s], s from the head) and from the tail (T[s],
+// s from the tail), both of element type T, N apart; weights cw[woff ..].
+constexpr int kMaxBlocks = 16;
+struct QuadBlock {
+  const void* H;
+  const void* T;
+  int M, woff, species;
+};
+
 struct QuadArgs {
-  const void* F;       // forward slices, (Ns+1)*N of T
-  const void* G;       // backward slices, (Ns+1)*N of T
+  QuadBlock blk[kMaxBlocks];
+  int nblk;
   long long N;
-  int Ns, sA;
-  const double* cw;    // quadrature weights: A at [0..sA], B at [sA+1..Ns+1]
+  const double* cw;    // per-block quadrature weights, block b at blk[b].woff
   double* phiA; double* phiB;
   double* wA; double* wB; double* praw;
   void* wfA; void* wfB;          // T
@@ -359,10 +404,7 @@ struct QuadArgs {
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_quad_update(QuadArgs a) {
-  const T* __restrict__ F = (const T*)a.F;
-  const T* __restrict__ G = (const T*)a.G;
   const long long N = a.N;
-  const int Ns = a.Ns, sA = a.sA;
   const double* __restrict__ cw = a.cw;
   const double invQ = 1.0 / a.st->Q;
   const double pbar = a.st->pbar;
@@ -375,18 +417,21 @@ k_quad_update(QuadArgs a) {
       phA = a.phiA[r];
       phB = a.phiB[r];
     } else {
+      // per block: sum_s c_s q_head[s] q_tail[M-1-s] (pspolymer/PsBlock.cpp:256-380),
+      // added into the block's species (pspolymer/PsBlock.cpp:154-196)
       double qa = 0.0, qb = 0.0;
+      for (int b = 0; b < a.nblk; ++b) {
+        const T* __restrict__ H = (const T*)a.blk[b].H + r;
+        const T* __restrict__ Tq = (const T*)a.blk[b].T + r;
+        const double* __restrict__ c = cw + a.blk[b].woff;
+        const int M = a.blk[b].M;
+        double acc = 0.0;
 #pragma unroll 8
-      for (int t = 0; t < sA; ++t)
-        qa += cw[t] * ((double)F[(long long)t * N + r] * (double)G[(long long)(Ns - t) * N + r]);
-      {
-        const double j = (double)F[(long long)sA * N + r] * (double)G[(long long)(Ns - sA) * N + r];
-        qa += cw[sA] * j;
-        qb += cw[sA + 1] * j;
+        for (int t = 0; t < M; ++t)
+          acc += c[t] * ((double)H[(long long)t * N] * (double)Tq[(long long)(M - 1 - t) * N]);
+        if (a.blk[b].species == 0) qa += acc;
+        else qb += acc;
       }
-#pragma unroll 8
-      for (int t = sA + 1; t <= Ns; ++t)
-        qb += cw[t + 1] * ((double)F[(long long)t * N + r] * (double)G[(long long)(Ns - t) * N + r]);
       phA = qa * invQ * a.dens;
       phB = qb * invQ * a.dens;
       a.phiA[r] = phA;

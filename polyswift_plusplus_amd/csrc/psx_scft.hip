@@ -1,13 +1,17 @@
 // psx_scft.hip -- host side of the C ABI declared in include/psx_scft.h.
 //
-// One engine = one bulk AB diblock on one HIP device and one stream.  All
-// fields stay resident in HBM; a contour step is one launch per active grid
-// axis, each a batched round-trip line FFT (psx_kernels.hpp) with the
-// exp(-ds w / 2) multiplies fused into the first and last launch.  The
-// forward (q) and backward (q+) sweeps advance together in the same launches.
+// One engine = one bulk AB block-copolymer melt (a diblock, or any tree of A
+// and B blocks) on one HIP device.  All fields stay resident in HBM.  The
+// block propagators are grouped into paths (contiguous slice arrays); a
+// contour step of a path is one round-trip line FFT launch per active axis
+// (psx_fft_reg.hpp), with the exp(-ds w / 2) multiplies fused in and the
+// boundary launches of consecutive steps fused in pairs.  Independent paths
+// (the forward and backward sweeps of a chain, the arms of a star) run on
+// their own streams.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -422,18 +426,42 @@ extern "C" psx_status psx_fft_scaled_pair_host(psx_fft* p, const void* h_data,
 // ===========================================================================
 namespace {
 
+// One block of the engine's block graph.
+struct BlockInfo {
+  int species = 0;                 // 0 = A, 1 = B
+  double frac = 0.0, b = 1.0;
+  int steps = 0;
+  int bclass = 0;                  // index of the spectral tables for this b
+  std::vector<std::pair<int, int>> cnt[2];   // (block, its end 0 head / 1 tail) joined at head, tail
+  int path[2] = {-1, -1};          // path of the propagator from the head (0) / tail (1)
+  int off[2] = {0, 0};             // its slice 0 in that path
+};
+
+// A propagation path: block propagators in sequence, each starting at the
+// final slice of the one before (a junction with a single partner), stored
+// as one contiguous slice array.  A path starts at a free end (slice 0 = 1)
+// or at a junction product of finals of paths of earlier rounds.
+struct PathInfo {
+  std::vector<int> step_block;     // block of every contour step
+  int len = 0;
+  int round = 0;
+  int slot = 0;                    // stream slot within the round
+  std::vector<std::pair<int, int>> init;   // (path, slice) factors of slice 0; empty = free end
+  std::vector<int> launch_axis;    // axis index of every launch (see sweep_launches)
+  std::vector<int> launch_nops;    // 1 plain, 2 fused
+};
+
+struct PathBufs {
+  void* P = nullptr;               // (len + 1) slices of T
+  void* tmp = nullptr;             // intermediate of a step
+};
+
 struct SweepBufs {
   bool ready = false;
-  void* F = nullptr;     // (Ns+1) slices, forward (head of A -> tail of B)
-  void* G = nullptr;     // (Ns+1) slices, backward (tail of B -> head of A)
-  void* tmpF = nullptr;
-  void* tmpG = nullptr;
-  void* wfA = nullptr;
-  void* wfB = nullptr;
-  void* emul[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};  // [block][axis idx]
-  PairIO* fio = nullptr;   // device table [launch][2]: both sweeps (see sweep_launches)
-  std::vector<int> launch_axis;   // axis index of every launch
-  std::vector<int> launch_nops;   // 1 plain, 2 fused
+  std::vector<PathBufs> path;
+  void* wf[2] = {nullptr, nullptr};          // exp(-ds w / 2) of species A, B (T)
+  std::vector<std::array<void*, 3>> emul;    // [b class][axis idx] spectral factors (T)
+  std::vector<PairIO*> rtab;       // per round: device table [launch][path slot]
   bool wfac_valid = false;
 };
 
@@ -455,14 +483,20 @@ struct psx_scft {
   // branches (second stream, forked and joined with events) so that their
   // kernels drift out of phase and one sweep's loads overlap the other's
   // arithmetic.  Default priority like the main stream.
-  hipStream_t stream2 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  std::vector<hipStream_t> pstream;     // stream slot k >= 1 of a round (slot 0 = stream)
+  hipEvent_t ev_fork = nullptr;
+  std::vector<hipEvent_t> ev_join;      // per slot
   bool split_sweeps = true;
   int dims[3];
   long long N;
   double cell[3];
-  double fA, fB, ds, bA, bB, chiN;
-  int sA, sB, Ns;
+  double fA, fB, ds, chiN;              // fA, fB: lengthfrac summed per species
+  int Ns;                               // contour steps summed over the blocks
+  std::vector<BlockInfo> blocks;
+  std::vector<double> bval;             // distinct bSegRatio values (b classes)
+  std::vector<PathInfo> paths;
+  std::vector<std::vector<int>> rounds; // path ids per round, slot order
+  bool diblock = false;                 // two blocks A-B joined head to tail
   int prec;
   std::vector<Axis> axes;
   int variant[2][3] = {{-1, -1, -1}, {-1, -1, -1}};   // line-kernel choice per precision, pass
@@ -484,7 +518,7 @@ struct psx_scft {
   // box stress: FP64 complex work grid, per-axis k^2 and kfac tables
   void* zbuf = nullptr;
   double* kd2[3] = {nullptr, nullptr, nullptr};
-  double* kf1[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  std::vector<std::array<double*, 3>> kf1;   // per b class: exp(-ds b^2 k_d^2)
   double* sacc = nullptr;
   std::vector<Axis> caxes;     // axes of the complex stress transforms
   // cached host-visible scratch for the host API
@@ -541,90 +575,108 @@ std::vector<std::pair<SweepOp, int>> sweep_launches(int na, int Ns) {
   return out;
 }
 
-// Field descriptors [launch][2] (forward sweep, backward sweep).
-std::vector<PairIO> fio_table(const psx_scft* e, int prec) {
+// Field descriptors of every launch of path p (see sweep_launches): the
+// w-factor and spectral factors of each contour step are those of the
+// step's block.
+std::vector<PairIO> path_fio(const psx_scft* e, int prec, int p) {
   const SweepBufs& s = e->sw[prec];
+  const PathInfo& pi = e->paths[p];
   const size_t tb = prec_bytes(prec);
   const int na = (int)e->axes.size();
-  const auto launches = sweep_launches(na, e->Ns);
-  std::vector<PairIO> tab(launches.size() * 2);
-  for (int h = 0; h < 2; ++h) {
-    const char* base = (const char*)(h == 0 ? s.F : s.G);
-    void* tmp = h == 0 ? s.tmpF : s.tmpG;
-    // block of contour step st: forward A then B, backward B then A
-    auto inA = [&](int st) { return h == 0 ? st < e->sA : !(st < e->sB); };
-    auto W = [&](int st) { return inA(st) ? s.wfA : s.wfB; };
-    auto E = [&](int a, int st) { return s.emul[inA(st) ? 0 : 1][a]; };
-    auto slice = [&](int st) { return (void*)(base + (size_t)st * e->N * tb); };
-    for (size_t j = 0; j < launches.size(); ++j) {
-      const SweepOp& o = launches[j].first;
-      PairIO& f = tab[j * 2 + h];
-      std::memset(&f, 0, sizeof(f));
-      f.in = o.first ? slice(o.step) : tmp;
-      f.pre = o.first ? W(o.step) : nullptr;
-      f.emul = E(o.axis, o.step);
-      if (launches[j].second == 1) {
-        f.post = o.last ? W(o.step) : nullptr;
-        f.out = o.last ? slice(o.step + 1) : tmp;
-      } else {
-        const int s2 = o.step + 1;
-        const bool last2 = na == 1;   // the second op is also the last of its step
-        f.post = W(o.step);
-        f.mid = slice(s2);
-        f.pre2 = W(s2);
-        f.emul2 = E(o.axis, s2);
-        f.post2 = last2 ? W(s2) : nullptr;
-        f.out = last2 ? slice(s2 + 1) : tmp;
-      }
+  const auto launches = sweep_launches(na, pi.len);
+  std::vector<PairIO> tab(launches.size());
+  const char* base = (const char*)s.path[p].P;
+  void* tmp = s.path[p].tmp;
+  auto blk = [&](int st) -> const BlockInfo& { return e->blocks[pi.step_block[st]]; };
+  auto W = [&](int st) { return (const void*)s.wf[blk(st).species]; };
+  auto E = [&](int a, int st) { return (const void*)s.emul[blk(st).bclass][a]; };
+  auto slice = [&](int st) { return (void*)(base + (size_t)st * e->N * tb); };
+  for (size_t j = 0; j < launches.size(); ++j) {
+    const SweepOp& o = launches[j].first;
+    PairIO& f = tab[j];
+    std::memset(&f, 0, sizeof(f));
+    f.in = o.first ? slice(o.step) : tmp;
+    f.pre = o.first ? W(o.step) : nullptr;
+    f.emul = E(o.axis, o.step);
+    if (launches[j].second == 1) {
+      f.post = o.last ? W(o.step) : nullptr;
+      f.out = o.last ? slice(o.step + 1) : tmp;
+    } else {
+      const int s2 = o.step + 1;
+      const bool last2 = na == 1;   // the second op is also the last of its step
+      f.post = W(o.step);
+      f.mid = slice(s2);
+      f.pre2 = W(s2);
+      f.emul2 = E(o.axis, s2);
+      f.post2 = last2 ? W(s2) : nullptr;
+      f.out = last2 ? slice(s2 + 1) : tmp;
     }
   }
   return tab;
+}
+
+// Paths of round r have identical launch lists (equal lengths).
+bool round_uniform(const psx_scft* e, int r) {
+  const std::vector<int>& rp = e->rounds[r];
+  for (int p : rp)
+    if (e->paths[p].len != e->paths[rp[0]].len) return false;
+  return true;
+}
+
+// Device table of round r: entry [j * np + slot] is launch j of the path in
+// that slot (rows past a shorter path's launch count stay zero, never used).
+psx_status upload_round_table(psx_scft* e, int prec, int r) {
+  const std::vector<int>& rp = e->rounds[r];
+  const size_t np = rp.size();
+  size_t rows = 0;
+  std::vector<std::vector<PairIO>> per;
+  for (int p : rp) {
+    per.push_back(path_fio(e, prec, p));
+    rows = std::max(rows, per.back().size());
+  }
+  std::vector<PairIO> tab(rows * np);
+  std::memset(tab.data(), 0, tab.size() * sizeof(PairIO));
+  for (size_t k = 0; k < np; ++k)
+    for (size_t j = 0; j < per[k].size(); ++j) tab[j * np + k] = per[k][j];
+  return upload((void**)&e->sw[prec].rtab[r], tab.data(), tab.size() * sizeof(PairIO));
 }
 
 psx_status ensure_sweep(psx_scft* e, int prec) {
   SweepBufs& s = e->sw[prec];
   if (s.ready) return PSX_OK;
   const size_t tb = prec_bytes(prec);
-  const size_t slices = (size_t)(e->Ns + 1) * e->N * tb;
-  PSX_TRY(dalloc(&s.F, slices));
-  PSX_TRY(dalloc(&s.G, slices));
-  PSX_TRY(dalloc(&s.tmpF, e->N * tb));
-  PSX_TRY(dalloc(&s.tmpG, e->N * tb));
-  PSX_TRY(dalloc(&s.wfA, e->N * tb));
-  PSX_TRY(dalloc(&s.wfB, e->N * tb));
+  s.path.assign(e->paths.size(), PathBufs());
+  for (size_t p = 0; p < e->paths.size(); ++p) {
+    PSX_TRY(dalloc(&s.path[p].P, (size_t)(e->paths[p].len + 1) * e->N * tb));
+    PSX_TRY(dalloc(&s.path[p].tmp, e->N * tb));
+  }
+  PSX_TRY(dalloc(&s.wf[0], e->N * tb));
+  PSX_TRY(dalloc(&s.wf[1], e->N * tb));
+  s.emul.assign(e->bval.size(), std::array<void*, 3>{nullptr, nullptr, nullptr});
   for (size_t i = 0; i < e->axes.size(); ++i) {
     Axis& a = e->axes[i];
     PSX_TRY(axis_tw(a, prec));
-    for (int blk = 0; blk < 2; ++blk) {
-      const double b = blk == 0 ? e->bA : e->bB;
+    for (size_t c = 0; c < e->bval.size(); ++c) {
       const double h = a.n == 1 ? 1.0 : e->cell[a.axis];
-      std::vector<double> em = axis_emul(a, h, e->ds, b);
-      if (prec == PSX_FP64) PSX_TRY(upload(&s.emul[blk][i], em.data(), em.size() * 8));
+      std::vector<double> em = axis_emul(a, h, e->ds, e->bval[c]);
+      if (prec == PSX_FP64) PSX_TRY(upload(&s.emul[c][i], em.data(), em.size() * 8));
       else {
         std::vector<float> f = cast_vec<float>(em);
-        PSX_TRY(upload(&s.emul[blk][i], f.data(), f.size() * 4));
+        PSX_TRY(upload(&s.emul[c][i], f.data(), f.size() * 4));
       }
     }
   }
-  // field table of every pass of every contour step (static: graphs capture it)
-  {
-    std::vector<PairIO> tab = fio_table(e, prec);
-    PSX_TRY(upload((void**)&s.fio, tab.data(), tab.size() * sizeof(PairIO)));
-    s.launch_axis.clear();
-    s.launch_nops.clear();
-    for (const auto& l : sweep_launches((int)e->axes.size(), e->Ns)) {
-      s.launch_axis.push_back(l.first.axis);
-      s.launch_nops.push_back(l.second);
-    }
-  }
-  // free ends: q(head of A) = q+(tail of B) = 1 (pspolymer/PsBlock.cpp:146-147)
+  // field tables of every launch of every path (static: graphs capture them)
+  s.rtab.assign(e->rounds.size(), nullptr);
+  for (size_t r = 0; r < e->rounds.size(); ++r) PSX_TRY(upload_round_table(e, prec, (int)r));
+  // free ends: q = 1 (pspolymer/PsBlock.cpp:146-147)
   const int g = grid_1d(e->N);
-  if (prec == PSX_FP64) {
-    hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kThreads), 0, e->stream, (double*)s.F, e->N, 1.0);
-    hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kThreads), 0, e->stream, (double*)s.G, e->N, 1.0);
-  } else {
-    hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kThreads), 0, e->stream, (float*)s.F, e->N, 1.0f);
-    hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kThreads), 0, e->stream, (float*)s.G, e->N, 1.0f);
+  for (size_t p = 0; p < e->paths.size(); ++p) {
+    if (!e->paths[p].init.empty()) continue;
+    if (prec == PSX_FP64)
+      hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kThreads), 0, e->stream, (double*)s.path[p].P, e->N, 1.0);
+    else
+      hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kThreads), 0, e->stream, (float*)s.path[p].P, e->N, 1.0f);
   }
   PSX_HIP(hipGetLastError());
   s.ready = true;
@@ -633,9 +685,13 @@ psx_status ensure_sweep(psx_scft* e, int prec) {
 }
 
 void free_sweep(SweepBufs& s) {
-  dfree(s.F); dfree(s.G); dfree(s.tmpF); dfree(s.tmpG); dfree(s.wfA); dfree(s.wfB);
-  dfree((void*&)s.fio);
+  for (auto& pb : s.path) { dfree(pb.P); dfree(pb.tmp); }
+  s.path.clear();
+  dfree(s.wf[0]); dfree(s.wf[1]);
+  for (auto& t : s.rtab) dfree((void*&)t);
+  s.rtab.clear();
   for (auto& b : s.emul) for (auto& p : b) dfree(p);
+  s.emul.clear();
   s.ready = false;
 }
 
@@ -644,60 +700,109 @@ psx_status compute_wfac(psx_scft* e) {
   const int g = grid_1d(e->N);
   const double h = 0.5 * e->ds;
   if (e->prec == PSX_FP64) {
-    hipLaunchKernelGGL(k_wfac<double>, dim3(g), dim3(kThreads), 0, e->stream, e->wA, e->N, h, (double*)s.wfA);
-    hipLaunchKernelGGL(k_wfac<double>, dim3(g), dim3(kThreads), 0, e->stream, e->wB, e->N, h, (double*)s.wfB);
+    hipLaunchKernelGGL(k_wfac<double>, dim3(g), dim3(kThreads), 0, e->stream, e->wA, e->N, h, (double*)s.wf[0]);
+    hipLaunchKernelGGL(k_wfac<double>, dim3(g), dim3(kThreads), 0, e->stream, e->wB, e->N, h, (double*)s.wf[1]);
   } else {
-    hipLaunchKernelGGL(k_wfac<float>, dim3(g), dim3(kThreads), 0, e->stream, e->wA, e->N, h, (float*)s.wfA);
-    hipLaunchKernelGGL(k_wfac<float>, dim3(g), dim3(kThreads), 0, e->stream, e->wB, e->N, h, (float*)s.wfB);
+    hipLaunchKernelGGL(k_wfac<float>, dim3(g), dim3(kThreads), 0, e->stream, e->wA, e->N, h, (float*)s.wf[0]);
+    hipLaunchKernelGGL(k_wfac<float>, dim3(g), dim3(kThreads), 0, e->stream, e->wB, e->N, h, (float*)s.wf[1]);
   }
   PSX_HIP(hipGetLastError());
   s.wfac_valid = true;
   return PSX_OK;
 }
 
-// One contour step of both sweeps: F[t] -> F[t+1], G[t] -> G[t+1].
-// Enqueue launches [j0, j1) of the sweep (both directions in every launch);
+// Device address of slice t of path p in precision prec.
+const void* slice_ptr(const psx_scft* e, int prec, int p, int t) {
+  return (const char*)e->sw[prec].path[p].P + (size_t)t * e->N * prec_bytes(prec);
+}
+
+// Junction products for the paths of round r that start at a joined end.
+template <typename T>
+psx_status launch_junctions(psx_scft* e, int r) {
+  for (int p : e->rounds[r]) {
+    const PathInfo& pi = e->paths[p];
+    if (pi.init.empty()) continue;
+    JunctionArgs ja;
+    std::memset(&ja, 0, sizeof(ja));
+    ja.out = (void*)slice_ptr(e, e->prec, p, 0);
+    ja.nin = (int)pi.init.size();
+    ja.N = e->N;
+    for (int k = 0; k < ja.nin; ++k) ja.in[k] = slice_ptr(e, e->prec, pi.init[k].first, pi.init[k].second);
+    hipLaunchKernelGGL(k_junction<T>, dim3(grid_1d(e->N)), dim3(kThreads), 0, e->stream, ja);
+  }
+  PSX_HIP(hipGetLastError());
+  return PSX_OK;
+}
+
+// All propagators, round by round.  Within a round every path advances on
+// its own stream slot (forked from and joined back into the engine stream;
+// inside a capture these become parallel graph branches), or, with
+// split_sweeps off and equal path lengths, all paths share each launch.
 // only_axis >= 0 keeps the launches along that axis only (timing).
 template <typename T>
-psx_status sweep_range(psx_scft* e, size_t j0, size_t j1, int only_axis = -1) {
+psx_status sweep_range(psx_scft* e, int only_axis = -1) {
   SweepBufs& s = e->sw[e->prec];
-  if (!e->split_sweeps) {
-    for (size_t j = j0; j < j1; ++j) {
-      const int a = s.launch_axis[j];
-      if (only_axis >= 0 && a != only_axis) continue;
-      PSX_TRY(launch_pair<T>(e->axes[a], e->prec, s.fio + j * 2, 2, e->stream, e->variant[e->prec][a]));
+  for (size_t r = 0; r < e->rounds.size(); ++r) {
+    const std::vector<int>& rp = e->rounds[r];
+    const int np = (int)rp.size();
+    if (only_axis < 0) PSX_TRY(launch_junctions<T>(e, (int)r));
+    const PairIO* tab = s.rtab[r];
+    if (!e->split_sweeps && round_uniform(e, (int)r)) {
+      const PathInfo& p0 = e->paths[rp[0]];
+      for (size_t j = 0; j < p0.launch_axis.size(); ++j) {
+        const int a = p0.launch_axis[j];
+        if (only_axis >= 0 && a != only_axis) continue;
+        PSX_TRY(launch_pair<T>(e->axes[a], e->prec, tab + j * np, np, e->stream, e->variant[e->prec][a]));
+      }
+      continue;
     }
-    return PSX_OK;
+    const bool fork = e->split_sweeps && np > 1;
+    if (fork) {
+      PSX_HIP(hipEventRecord(e->ev_fork, e->stream));
+      for (int k = 1; k < np; ++k) PSX_HIP(hipStreamWaitEvent(e->pstream[k], e->ev_fork, 0));
+    }
+    // launches interleaved across the slots, so that every stream has work
+    // queued early when the sweep is enqueued eagerly
+    size_t rows = 0;
+    for (int p : rp) rows = std::max(rows, e->paths[p].launch_axis.size());
+    for (size_t j = 0; j < rows; ++j)
+      for (int k = 0; k < np; ++k) {
+        const PathInfo& pi = e->paths[rp[k]];
+        if (j >= pi.launch_axis.size()) continue;
+        const int a = pi.launch_axis[j];
+        if (only_axis >= 0 && a != only_axis) continue;
+        hipStream_t st = (fork && k > 0) ? e->pstream[k] : e->stream;
+        PSX_TRY(launch_pair<T>(e->axes[a], e->prec, tab + j * np + k, 1, st, e->variant[e->prec][a]));
+      }
+    if (fork) {
+      for (int k = 1; k < np; ++k) {
+        PSX_HIP(hipEventRecord(e->ev_join[k], e->pstream[k]));
+        PSX_HIP(hipStreamWaitEvent(e->stream, e->ev_join[k], 0));
+      }
+    }
   }
-  // forward sweep on the engine stream, backward sweep on stream2
-  PSX_HIP(hipEventRecord(e->ev_fork, e->stream));
-  PSX_HIP(hipStreamWaitEvent(e->stream2, e->ev_fork, 0));
-  for (size_t j = j0; j < j1; ++j) {
-    const int a = s.launch_axis[j];
-    if (only_axis >= 0 && a != only_axis) continue;
-    PSX_TRY(launch_pair<T>(e->axes[a], e->prec, s.fio + j * 2, 1, e->stream, e->variant[e->prec][a]));
-    PSX_TRY(launch_pair<T>(e->axes[a], e->prec, s.fio + j * 2 + 1, 1, e->stream2, e->variant[e->prec][a]));
-  }
-  PSX_HIP(hipEventRecord(e->ev_join, e->stream2));
-  PSX_HIP(hipStreamWaitEvent(e->stream, e->ev_join, 0));
   return PSX_OK;
 }
 
 psx_status sweep(psx_scft* e) {
-  const size_t nl = e->sw[e->prec].launch_axis.size();
-  if (e->prec == PSX_FP64) return sweep_range<double>(e, 0, nl);
-  return sweep_range<float>(e, 0, nl);
+  if (e->prec == PSX_FP64) return sweep_range<double>(e);
+  return sweep_range<float>(e);
 }
 
+// Q = mean(qHeadInitial qHeadFinal) of block 0 (pspolymer/PsBlock.cpp:205-237);
+// a free head has qHeadInitial = 1.
 psx_status launch_Q(psx_scft* e) {
-  SweepBufs& s = e->sw[e->prec];
+  const BlockInfo& b0 = e->blocks[0];
+  const void* fin = slice_ptr(e, e->prec, b0.path[1], b0.off[1] + b0.steps);
+  const void* ini = b0.cnt[0].empty() ? nullptr : slice_ptr(e, e->prec, b0.path[0], b0.off[0]);
   const int g = e->nb;
-  const size_t off = (size_t)e->Ns * e->N * prec_bytes(e->prec);
-  const void* last = (const char*)s.G + off;
-  if (e->prec == PSX_FP64)
-    hipLaunchKernelGGL(k_partial_sum<double>, dim3(g), dim3(kThreads), 0, e->stream, (const double*)last, e->N, e->part);
-  else
-    hipLaunchKernelGGL(k_partial_sum<float>, dim3(g), dim3(kThreads), 0, e->stream, (const float*)last, e->N, e->part);
+  if (e->prec == PSX_FP64) {
+    if (ini) hipLaunchKernelGGL(k_partial_dot<double>, dim3(g), dim3(kThreads), 0, e->stream, (const double*)ini, (const double*)fin, e->N, e->part);
+    else hipLaunchKernelGGL(k_partial_sum<double>, dim3(g), dim3(kThreads), 0, e->stream, (const double*)fin, e->N, e->part);
+  } else {
+    if (ini) hipLaunchKernelGGL(k_partial_dot<float>, dim3(g), dim3(kThreads), 0, e->stream, (const float*)ini, (const float*)fin, e->N, e->part);
+    else hipLaunchKernelGGL(k_partial_sum<float>, dim3(g), dim3(kThreads), 0, e->stream, (const float*)fin, e->N, e->part);
+  }
   hipLaunchKernelGGL(k_finish_Q, dim3(1), dim3(kThreads), 0, e->stream, (const double*)e->part, g,
                      1.0 / (double)e->N, e->st);
   PSX_HIP(hipGetLastError());
@@ -707,11 +812,23 @@ psx_status launch_Q(psx_scft* e) {
 QuadArgs quad_args(psx_scft* e, int mode, const psx_mixing* mix) {
   SweepBufs& s = e->sw[e->prec];
   QuadArgs a;
-  a.F = s.F; a.G = s.G; a.N = e->N; a.Ns = e->Ns; a.sA = e->sA;
+  std::memset(&a, 0, sizeof(a));
+  a.nblk = (int)e->blocks.size();
+  int woff = 0;
+  for (int b = 0; b < a.nblk; ++b) {
+    const BlockInfo& bi = e->blocks[b];
+    a.blk[b].H = slice_ptr(e, e->prec, bi.path[0], bi.off[0]);
+    a.blk[b].T = slice_ptr(e, e->prec, bi.path[1], bi.off[1]);
+    a.blk[b].M = bi.steps + 1;
+    a.blk[b].woff = woff;
+    a.blk[b].species = bi.species;
+    woff += bi.steps + 1;
+  }
+  a.N = e->N;
   a.cw = e->cw;
   a.phiA = e->phiA; a.phiB = e->phiB;
   a.wA = e->wA; a.wB = e->wB; a.praw = e->praw;
-  a.wfA = s.wfA; a.wfB = s.wfB;
+  a.wfA = s.wf[0]; a.wfB = s.wf[1];
   a.ringx = e->ringx; a.ringf = e->ringf;
   a.m1 = (mix && mix->kind == 1) ? mix->depth + 1 : 1;
   a.chiN = e->chiN; a.fA = e->fA; a.fB = e->fB;
@@ -834,9 +951,10 @@ extern "C" psx_status psx_scft_destroy(psx_scft* e) {
   for (auto& p : e->kd2) dfree((void*&)p);
   for (auto& b : e->kf1) for (auto& p : b) dfree((void*&)p);
   for (auto& a : e->caxes) { dfree(a.tw[0]); dfree(a.tw[1]); }
-  if (e->stream2) { (void)hipStreamSynchronize(e->stream2); (void)hipStreamDestroy(e->stream2); }
+  for (size_t k = 1; k < e->pstream.size(); ++k)
+    if (e->pstream[k]) { (void)hipStreamSynchronize(e->pstream[k]); (void)hipStreamDestroy(e->pstream[k]); }
   if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
-  if (e->ev_join) (void)hipEventDestroy(e->ev_join);
+  for (auto& ev : e->ev_join) if (ev) (void)hipEventDestroy(ev);
   if (e->own_stream && e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
   return PSX_OK;
@@ -880,40 +998,172 @@ static void quad_weights(int steps, double ds, double* w) {
   if (even) { w[M - 2] += 0.5 * ds; w[M - 1] += 0.5 * ds; }
 }
 
-extern "C" psx_status psx_scft_create(const psx_grid* g, const psx_diblock* c,
-                                      psx_prec prec, void* stream, psx_scft** out) {
-  if (!out || !g || !c) return fail(PSX_ERR_INVALID, "null argument");
+// Per-block quadrature weights, blocks in order (mode 0 reference Simpson,
+// 1 trapezoid).
+static std::vector<double> all_weights(const psx_scft* e, int mode) {
+  std::vector<double> w;
+  for (const BlockInfo& b : e->blocks) {
+    const size_t o = w.size();
+    w.resize(o + b.steps + 1);
+    if (mode == 0) quad_weights(b.steps, e->ds, w.data() + o);
+    else trap_weights(b.steps, e->ds, w.data() + o);
+  }
+  return w;
+}
+
+// Blocks, junctions and propagation paths of the graph d[0..nb).
+static psx_status build_graph(psx_scft* e, int nb, const psx_block* d) {
+  if (nb < 1 || nb > PSX_MAX_BLOCKS)
+    return fail(PSX_ERR_INVALID, "block count must be in [1, " + std::to_string(PSX_MAX_BLOCKS) + "]");
+  e->blocks.assign(nb, BlockInfo());
+  e->bval.clear();
+  for (int i = 0; i < nb; ++i) {
+    const psx_block& b = d[i];
+    if (b.species != 0 && b.species != 1) return fail(PSX_ERR_INVALID, "block species must be 0 (A) or 1 (B)");
+    if (!(b.lengthfrac > 0) || !(b.bSegRatio > 0)) return fail(PSX_ERR_INVALID, "need lengthfrac > 0 and bSegRatio > 0");
+    if (b.nhead < 0 || b.ntail < 0 || b.nhead > PSX_MAX_JOIN || b.ntail > PSX_MAX_JOIN)
+      return fail(PSX_ERR_INVALID, "joined-block counts must be in [0, " + std::to_string(PSX_MAX_JOIN) + "]");
+    BlockInfo& bi = e->blocks[i];
+    bi.species = b.species;
+    bi.frac = b.lengthfrac;
+    bi.b = b.bSegRatio;
+    bi.steps = block_steps(b.lengthfrac, e->ds);
+    if (bi.steps < 0)
+      return fail(PSX_ERR_STEPS, "block " + std::to_string(i) + ": non-integral or too few block steps "
+                                 "(lengthfrac/ds = " + std::to_string(b.lengthfrac / e->ds) + ")");
+    bi.bclass = -1;
+    for (size_t c = 0; c < e->bval.size(); ++c)
+      if (e->bval[c] == b.bSegRatio) bi.bclass = (int)c;
+    if (bi.bclass < 0) { bi.bclass = (int)e->bval.size(); e->bval.push_back(b.bSegRatio); }
+  }
+  // junctions (PsBlockCopolymer::buildCntBlockType): block j named at end
+  // E of block i is joined at j's tail when j's tail list names i
+  for (int i = 0; i < nb; ++i)
+    for (int E = 0; E < 2; ++E) {
+      const int n = E == 0 ? d[i].nhead : d[i].ntail;
+      const int* lst = E == 0 ? d[i].head : d[i].tail;
+      for (int k = 0; k < n; ++k) {
+        const int j = lst[k];
+        if (j < 0 || j >= nb || j == i)
+          return fail(PSX_ERR_INVALID, "block " + std::to_string(i) + ": joined block index out of range");
+        for (const auto& c : e->blocks[i].cnt[E])
+          if (c.first == j) return fail(PSX_ERR_INVALID, "block " + std::to_string(i) + " joins block " + std::to_string(j) + " twice");
+        bool at_tail = false;
+        for (int m = 0; m < d[j].ntail; ++m)
+          if (d[j].tail[m] == i) at_tail = true;
+        e->blocks[i].cnt[E].push_back({j, at_tail ? 1 : 0});
+      }
+      // junction products run in ascending block index (std::map order)
+      std::sort(e->blocks[i].cnt[E].begin(), e->blocks[i].cnt[E].end());
+    }
+  // Propagation tasks: task 2b + dir is block b propagated from its head
+  // (dir 0, ends at the tail) or from its tail (dir 1, ends at the head).
+  // Its inputs are the tasks that end at the blocks joined to its start end.
+  const int nt = 2 * nb;
+  auto inputs = [&](int t) {
+    std::vector<int> in;
+    for (const auto& c : e->blocks[t / 2].cnt[t % 2]) in.push_back(2 * c.first + (c.second == 1 ? 0 : 1));
+    return in;
+  };
+  std::vector<int> tpath(nt, -1), toff(nt, 0), plast;
+  e->paths.clear();
+  int left = nt;
+  while (left > 0) {
+    bool progress = false;
+    for (int t = 0; t < nt; ++t) {
+      if (tpath[t] >= 0) continue;
+      const std::vector<int> in = inputs(t);
+      bool ready = true;
+      for (int c : in) ready = ready && tpath[c] >= 0;
+      if (!ready) continue;
+      const int b = t / 2, steps = e->blocks[b].steps;
+      int p;
+      if (in.size() == 1 && plast[tpath[in[0]]] == in[0]) {
+        p = tpath[in[0]];               // continues the path of its single input
+      } else {
+        p = (int)e->paths.size();
+        PathInfo pi;
+        for (int c : in) {
+          pi.init.push_back({tpath[c], toff[c] + e->blocks[c / 2].steps});
+          pi.round = std::max(pi.round, e->paths[tpath[c]].round + 1);
+        }
+        e->paths.push_back(pi);
+        plast.push_back(-1);
+      }
+      PathInfo& pi = e->paths[p];
+      tpath[t] = p;
+      toff[t] = pi.len;
+      for (int k = 0; k < steps; ++k) pi.step_block.push_back(b);
+      pi.len += steps;
+      plast[p] = t;
+      e->blocks[b].path[t % 2] = p;
+      e->blocks[b].off[t % 2] = toff[t];
+      --left;
+      progress = true;
+    }
+    if (!progress)
+      return fail(PSX_ERR_INVALID, "block graph has a cycle or a junction no free end reaches");
+  }
+  int nr = 0;
+  for (const PathInfo& pi : e->paths) nr = std::max(nr, pi.round + 1);
+  e->rounds.assign(nr, {});
+  const int na = (int)e->axes.size();
+  for (size_t p = 0; p < e->paths.size(); ++p) {
+    PathInfo& pi = e->paths[p];
+    pi.slot = (int)e->rounds[pi.round].size();
+    e->rounds[pi.round].push_back((int)p);
+    for (const auto& l : sweep_launches(na, pi.len)) {
+      pi.launch_axis.push_back(l.first.axis);
+      pi.launch_nops.push_back(l.second);
+    }
+  }
+  // species totals, contour steps
+  e->fA = e->fB = 0.0;
+  e->Ns = 0;
+  for (const BlockInfo& b : e->blocks) {
+    (b.species == 0 ? e->fA : e->fB) += b.frac;
+    e->Ns += b.steps;
+  }
+  e->diblock = nb == 2 && e->blocks[0].species == 0 && e->blocks[1].species == 1 &&
+               d[0].nhead == 0 && d[0].ntail == 1 && d[0].tail[0] == 1 &&
+               d[1].nhead == 1 && d[1].head[0] == 0 && d[1].ntail == 0;
+  return PSX_OK;
+}
+
+static psx_status create_engine(const psx_grid* g, int nb, const psx_block* blocks, double ds,
+                                double chiN, psx_prec prec, void* stream, psx_scft** out) {
+  if (!out || !g || !blocks) return fail(PSX_ERR_INVALID, "null argument");
   *out = nullptr;
   if (g->nx < 1 || g->ny < 1 || g->nz < 1) return fail(PSX_ERR_SHAPE, "grid dims must be >= 1");
   if (!(g->dx > 0 && g->dy > 0 && g->dz > 0)) return fail(PSX_ERR_INVALID, "cell sizes must be > 0");
   if ((long long)g->nx * g->ny * g->nz > (1LL << 30)) return fail(PSX_ERR_SHAPE, "grid too large");
   if (prec != PSX_FP64 && prec != PSX_FP32) return fail(PSX_ERR_INVALID, "bad precision");
-  if (!(c->ds > 0) || !(c->fA > 0 && c->fA < 1)) return fail(PSX_ERR_INVALID, "need ds > 0 and 0 < fA < 1");
-  const int sA = block_steps(c->fA, c->ds);
-  const int sB = block_steps(1.0 - c->fA, c->ds);
-  if (sA < 0 || sB < 0)
-    return fail(PSX_ERR_STEPS, "non-integral or too few block steps (fA/ds = " +
-                                   std::to_string(c->fA / c->ds) + ", (1-fA)/ds = " +
-                                   std::to_string((1.0 - c->fA) / c->ds) + ")");
+  if (!(ds > 0)) return fail(PSX_ERR_INVALID, "need ds > 0");
   psx_scft* e = new psx_scft();
   e->dims[0] = g->nx; e->dims[1] = g->ny; e->dims[2] = g->nz;
   e->N = (long long)g->nx * g->ny * g->nz;
   e->cell[0] = g->dx; e->cell[1] = g->dy; e->cell[2] = g->dz;
-  e->fA = c->fA; e->fB = 1.0 - c->fA; e->ds = c->ds;
-  e->bA = c->bA; e->bB = c->bB; e->chiN = c->chiN;
-  e->sA = sA; e->sB = sB; e->Ns = sA + sB;
+  e->ds = ds;
+  e->chiN = chiN;
   e->prec = prec;
-  psx_status st = PSX_OK;
+  psx_status st = make_axes(e->dims, &e->axes);
+  if (st == PSX_OK) st = build_graph(e, nb, blocks);
+  if (st != PSX_OK) { delete e; return st; }
 #define E_TRY(x) do { if (st == PSX_OK) st = (x); } while (0)
   E_TRY((hipGetDevice(&e->dev) == hipSuccess) ? PSX_OK : fail(PSX_ERR_HIP, "hipGetDevice failed"));
   E_TRY(make_stream(stream, &e->stream, &e->own_stream));
-  E_TRY(hipStreamCreateWithFlags(&e->stream2, hipStreamNonBlocking) == hipSuccess
-            ? PSX_OK : fail(PSX_ERR_HIP, "stream creation failed"));
-  E_TRY(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming) == hipSuccess &&
-                hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming) == hipSuccess
+  size_t slots = 1;
+  for (const auto& r : e->rounds) slots = std::max(slots, r.size());
+  e->pstream.assign(slots, nullptr);
+  e->ev_join.assign(slots, nullptr);
+  e->pstream[0] = e->stream;
+  for (size_t k = 1; k < slots; ++k)
+    E_TRY(hipStreamCreateWithFlags(&e->pstream[k], hipStreamNonBlocking) == hipSuccess &&
+                  hipEventCreateWithFlags(&e->ev_join[k], hipEventDisableTiming) == hipSuccess
+              ? PSX_OK : fail(PSX_ERR_HIP, "stream creation failed"));
+  E_TRY(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming) == hipSuccess
             ? PSX_OK : fail(PSX_ERR_HIP, "event creation failed"));
   if (const char* v = getenv("PSX_SPLIT_SWEEPS")) e->split_sweeps = v[0] != '0';
-  E_TRY(make_axes(e->dims, &e->axes));
   const size_t fb = e->N * sizeof(double);
   E_TRY(dalloc((void**)&e->wA, fb));
   E_TRY(dalloc((void**)&e->wB, fb));
@@ -924,10 +1174,10 @@ extern "C" psx_status psx_scft_create(const psx_grid* g, const psx_diblock* c,
   e->part_cap = e->nb * kNPart;
   E_TRY(dalloc((void**)&e->part, e->part_cap * sizeof(double)));
   E_TRY(dalloc((void**)&e->st, sizeof(DevState)));
-  std::vector<double> w(e->Ns + 2);
-  quad_weights(sA, e->ds, w.data());
-  quad_weights(sB, e->ds, w.data() + sA + 1);
-  E_TRY(upload((void**)&e->cw, w.data(), w.size() * sizeof(double)));
+  if (st == PSX_OK) {
+    std::vector<double> w = all_weights(e, 0);
+    st = upload((void**)&e->cw, w.data(), w.size() * sizeof(double));
+  }
   if (st == PSX_OK) {
     DevState h;
     std::memset(&h, 0, sizeof(h));
@@ -948,17 +1198,41 @@ extern "C" psx_status psx_scft_create(const psx_grid* g, const psx_diblock* c,
   return PSX_OK;
 }
 
+extern "C" psx_status psx_scft_create(const psx_grid* g, const psx_diblock* c,
+                                      psx_prec prec, void* stream, psx_scft** out) {
+  if (!out || !g || !c) return fail(PSX_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (!(c->ds > 0) || !(c->fA > 0 && c->fA < 1)) return fail(PSX_ERR_INVALID, "need ds > 0 and 0 < fA < 1");
+  if (block_steps(c->fA, c->ds) < 0 || block_steps(1.0 - c->fA, c->ds) < 0)
+    return fail(PSX_ERR_STEPS, "non-integral or too few block steps (fA/ds = " +
+                                   std::to_string(c->fA / c->ds) + ", (1-fA)/ds = " +
+                                   std::to_string((1.0 - c->fA) / c->ds) + ")");
+  // A (head free) joined at its tail to the head of B (tail free)
+  psx_block b[2];
+  std::memset(b, 0, sizeof(b));
+  b[0].species = 0; b[0].lengthfrac = c->fA; b[0].bSegRatio = c->bA; b[0].ntail = 1; b[0].tail[0] = 1;
+  b[1].species = 1; b[1].lengthfrac = 1.0 - c->fA; b[1].bSegRatio = c->bB; b[1].nhead = 1; b[1].head[0] = 0;
+  return create_engine(g, 2, b, c->ds, c->chiN, prec, stream, out);
+}
+
+extern "C" psx_status psx_scft_create_blocks(const psx_grid* g, int nblocks, const psx_block* blocks,
+                                             double ds, double chiN, psx_prec prec, void* stream,
+                                             psx_scft** out) {
+  return create_engine(g, nblocks, blocks, ds, chiN, prec, stream, out);
+}
+
+extern "C" psx_status psx_scft_graph_info(const psx_scft* e, int* nblocks, int* npaths, int* nrounds) {
+  if (!e) return fail(PSX_ERR_INVALID, "null engine");
+  if (nblocks) *nblocks = (int)e->blocks.size();
+  if (npaths) *npaths = (int)e->paths.size();
+  if (nrounds) *nrounds = (int)e->rounds.size();
+  return PSX_OK;
+}
+
 extern "C" psx_status psx_scft_set_quadrature(psx_scft* e, int mode) {
   if (!e) return fail(PSX_ERR_INVALID, "null engine");
   if (mode != 0 && mode != 1) return fail(PSX_ERR_INVALID, "quadrature mode must be 0 or 1");
-  std::vector<double> w(e->Ns + 2);
-  if (mode == 0) {
-    quad_weights(e->sA, e->ds, w.data());
-    quad_weights(e->sB, e->ds, w.data() + e->sA + 1);
-  } else {
-    trap_weights(e->sA, e->ds, w.data());
-    trap_weights(e->sB, e->ds, w.data() + e->sA + 1);
-  }
+  std::vector<double> w = all_weights(e, mode);
   PSX_HIP(hipStreamSynchronize(e->stream));
   PSX_HIP(hipMemcpy(e->cw, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
   e->dens_valid = false;
@@ -978,7 +1252,8 @@ extern "C" psx_status psx_scft_set_precision(psx_scft* e, psx_prec prec) {
 
 extern "C" psx_status psx_scft_block_steps(const psx_scft* e, int* sA, int* sB) {
   if (!e || !sA || !sB) return fail(PSX_ERR_INVALID, "null argument");
-  *sA = e->sA; *sB = e->sB;
+  *sA = 0; *sB = 0;
+  for (const BlockInfo& b : e->blocks) (b.species == 0 ? *sA : *sB) += b.steps;
   return PSX_OK;
 }
 
@@ -1098,23 +1373,36 @@ extern "C" psx_status psx_scft_get_densities_host(psx_scft* e, double* phiA, dou
   return PSX_OK;
 }
 
-extern "C" psx_status psx_scft_get_slice_host(psx_scft* e, int backward, int t, double* out) {
-  if (!e || !out) return fail(PSX_ERR_INVALID, "null argument");
-  if (t < 0 || t > e->Ns) return fail(PSX_ERR_INVALID, "slice index out of range");
-  SweepBufs& s = e->sw[e->prec];
+static psx_status copy_slice_host(psx_scft* e, const void* src, double* out) {
   const size_t fb = e->N * sizeof(double);
   if (!e->hstage) PSX_TRY(dalloc(&e->hstage, fb));
-  const char* base = (const char*)(backward ? s.G : s.F) + (size_t)t * e->N * prec_bytes(e->prec);
   if (e->prec == PSX_FP64)
     hipLaunchKernelGGL(k_to_f64<double>, dim3(grid_1d(e->N)), dim3(kThreads), 0, e->stream,
-                       (const double*)base, e->N, (double*)e->hstage);
+                       (const double*)src, e->N, (double*)e->hstage);
   else
     hipLaunchKernelGGL(k_to_f64<float>, dim3(grid_1d(e->N)), dim3(kThreads), 0, e->stream,
-                       (const float*)base, e->N, (double*)e->hstage);
+                       (const float*)src, e->N, (double*)e->hstage);
   PSX_HIP(hipGetLastError());
   PSX_HIP(hipMemcpyAsync(out, e->hstage, fb, hipMemcpyDeviceToHost, e->stream));
   PSX_HIP(hipStreamSynchronize(e->stream));
   return PSX_OK;
+}
+
+extern "C" psx_status psx_scft_get_slice_host(psx_scft* e, int path, int t, double* out) {
+  if (!e || !out) return fail(PSX_ERR_INVALID, "null argument");
+  if (path < 0 || path >= (int)e->paths.size()) return fail(PSX_ERR_INVALID, "path index out of range");
+  if (t < 0 || t > e->paths[path].len) return fail(PSX_ERR_INVALID, "slice index out of range");
+  return copy_slice_host(e, slice_ptr(e, e->prec, path, t), out);
+}
+
+extern "C" psx_status psx_scft_get_block_slice_host(psx_scft* e, int block, int from_tail, int t,
+                                                    double* out) {
+  if (!e || !out) return fail(PSX_ERR_INVALID, "null argument");
+  if (block < 0 || block >= (int)e->blocks.size()) return fail(PSX_ERR_INVALID, "block index out of range");
+  if (from_tail != 0 && from_tail != 1) return fail(PSX_ERR_INVALID, "from_tail must be 0 or 1");
+  const BlockInfo& b = e->blocks[block];
+  if (t < 0 || t > b.steps) return fail(PSX_ERR_INVALID, "slice index out of range");
+  return copy_slice_host(e, slice_ptr(e, e->prec, b.path[from_tail], b.off[from_tail] + t), out);
 }
 
 extern "C" psx_status psx_scft_update_steepest(psx_scft* e, double lambda0, double lambda1,
@@ -1253,15 +1541,14 @@ static psx_status write_emul(psx_scft* e, int prec) {
   SweepBufs& s = e->sw[prec];
   for (size_t i = 0; i < e->axes.size(); ++i) {
     const Axis& a = e->axes[i];
-    for (int blk = 0; blk < 2; ++blk) {
-      const double b = blk == 0 ? e->bA : e->bB;
+    for (size_t c = 0; c < e->bval.size(); ++c) {
       const double h = a.n == 1 ? 1.0 : e->cell[a.axis];
-      std::vector<double> em = axis_emul(a, h, e->ds, b);
+      std::vector<double> em = axis_emul(a, h, e->ds, e->bval[c]);
       if (prec == PSX_FP64) {
-        PSX_HIP(hipMemcpy(s.emul[blk][i], em.data(), em.size() * 8, hipMemcpyHostToDevice));
+        PSX_HIP(hipMemcpy(s.emul[c][i], em.data(), em.size() * 8, hipMemcpyHostToDevice));
       } else {
         std::vector<float> f = cast_vec<float>(em);
-        PSX_HIP(hipMemcpy(s.emul[blk][i], f.data(), f.size() * 4, hipMemcpyHostToDevice));
+        PSX_HIP(hipMemcpy(s.emul[c][i], f.data(), f.size() * 4, hipMemcpyHostToDevice));
       }
     }
   }
@@ -1270,25 +1557,23 @@ static psx_status write_emul(psx_scft* e, int prec) {
 
 static psx_status write_stress_tables(psx_scft* e, bool alloc) {
   const double twopi = 2.0 * std::acos(-1.0);
+  if (alloc) e->kf1.assign(e->bval.size(), std::array<double*, 3>{nullptr, nullptr, nullptr});
   for (int d = 0; d < 3; ++d) {
     const int n = e->dims[d];
-    std::vector<double> k2(n), kfA(n), kfB(n);
+    std::vector<double> k2(n);
     const double n2 = (double)n / 2.0;
     for (int i = 0; i < n; ++i) {
       const double nk = n2 - std::fabs((double)i - n2);
       const double kv = twopi * nk / (double)n / e->cell[d];
       k2[i] = kv * kv;
-      kfA[i] = std::exp(-1.0 * e->ds * k2[i] * e->bA * e->bA);
-      kfB[i] = std::exp(-1.0 * e->ds * k2[i] * e->bB * e->bB);
     }
-    if (alloc) {
-      PSX_TRY(upload((void**)&e->kd2[d], k2.data(), n * 8));
-      PSX_TRY(upload((void**)&e->kf1[0][d], kfA.data(), n * 8));
-      PSX_TRY(upload((void**)&e->kf1[1][d], kfB.data(), n * 8));
-    } else {
-      PSX_HIP(hipMemcpy(e->kd2[d], k2.data(), n * 8, hipMemcpyHostToDevice));
-      PSX_HIP(hipMemcpy(e->kf1[0][d], kfA.data(), n * 8, hipMemcpyHostToDevice));
-      PSX_HIP(hipMemcpy(e->kf1[1][d], kfB.data(), n * 8, hipMemcpyHostToDevice));
+    if (alloc) PSX_TRY(upload((void**)&e->kd2[d], k2.data(), n * 8));
+    else PSX_HIP(hipMemcpy(e->kd2[d], k2.data(), n * 8, hipMemcpyHostToDevice));
+    for (size_t c = 0; c < e->bval.size(); ++c) {
+      std::vector<double> kf(n);
+      for (int i = 0; i < n; ++i) kf[i] = std::exp(-1.0 * e->ds * k2[i] * e->bval[c] * e->bval[c]);
+      if (alloc) PSX_TRY(upload((void**)&e->kf1[c][d], kf.data(), n * 8));
+      else PSX_HIP(hipMemcpy(e->kf1[c][d], kf.data(), n * 8, hipMemcpyHostToDevice));
     }
   }
   return PSX_OK;
@@ -1323,35 +1608,37 @@ static psx_status ensure_stress(psx_scft* e) {
   return PSX_OK;
 }
 
+// For every contour step t of every block: q_t from the head and the
+// propagator from the tail after steps-1-t steps meet across the step
+// operator; sum their spectral product weighted by dK/dL.
 template <typename T>
 static psx_status stress_sweep(psx_scft* e) {
   SweepBufs& s = e->sw[e->prec];
-  const size_t tb = sizeof(T);
   const int g = grid_1d(e->N);
   PSX_HIP(hipMemsetAsync(e->sacc, 0, 4 * sizeof(double), e->stream));
-  for (int t = 0; t < e->Ns; ++t) {
-    const bool inA = t < e->sA;
-    const T* f = (const T*)((const char*)s.F + (size_t)t * e->N * tb);
-    const T* gg = (const T*)((const char*)s.G + (size_t)(e->Ns - 1 - t) * e->N * tb);
-    const T* w = (const T*)(inA ? s.wfA : s.wfB);
-    hipLaunchKernelGGL(k_stress_pack<T>, dim3(g), dim3(kThreads), 0, e->stream, f, gg, w, e->N,
-                       (double2*)e->zbuf);
-    for (auto& a : e->caxes) {
-      CplxIO io{};
-      io.in = e->zbuf; io.out = e->zbuf; io.kmul = nullptr;
-      io.in_real = 0; io.out_real = 0; io.do_fwd = 1; io.do_inv = 0;
-      PSX_TRY(launch_cplx<double>(a, PSX_FP64, io, e->stream));
+  for (const BlockInfo& blk : e->blocks) {
+    for (int t = 0; t < blk.steps; ++t) {
+      const T* f = (const T*)slice_ptr(e, e->prec, blk.path[0], blk.off[0] + t);
+      const T* gg = (const T*)slice_ptr(e, e->prec, blk.path[1], blk.off[1] + blk.steps - 1 - t);
+      const T* w = (const T*)s.wf[blk.species];
+      hipLaunchKernelGGL(k_stress_pack<T>, dim3(g), dim3(kThreads), 0, e->stream, f, gg, w, e->N,
+                         (double2*)e->zbuf);
+      for (auto& a : e->caxes) {
+        CplxIO io{};
+        io.in = e->zbuf; io.out = e->zbuf; io.kmul = nullptr;
+        io.in_real = 0; io.out_real = 0; io.do_fwd = 1; io.do_inv = 0;
+        PSX_TRY(launch_cplx<double>(a, PSX_FP64, io, e->stream));
+      }
+      StressArgs sa;
+      sa.z = (const double2*)e->zbuf;
+      sa.nx = e->dims[0]; sa.ny = e->dims[1]; sa.nz = e->dims[2];
+      for (int d = 0; d < 3; ++d) { sa.kf[d] = e->kf1[blk.bclass][d]; sa.kd2[d] = e->kd2[d]; }
+      sa.part = e->part;
+      hipLaunchKernelGGL(k_stress_reduce, dim3(e->nb), dim3(kThreads), 0, e->stream, sa);
+      hipLaunchKernelGGL(k_stress_accum, dim3(1), dim3(kThreads), 0, e->stream, (const double*)e->part,
+                         e->nb, e->ds * blk.b * blk.b, e->sacc);
+      PSX_HIP(hipGetLastError());
     }
-    StressArgs sa;
-    sa.z = (const double2*)e->zbuf;
-    sa.nx = e->dims[0]; sa.ny = e->dims[1]; sa.nz = e->dims[2];
-    for (int d = 0; d < 3; ++d) { sa.kf[d] = e->kf1[inA ? 0 : 1][d]; sa.kd2[d] = e->kd2[d]; }
-    sa.part = e->part;
-    hipLaunchKernelGGL(k_stress_reduce, dim3(e->nb), dim3(kThreads), 0, e->stream, sa);
-    const double b = inA ? e->bA : e->bB;
-    hipLaunchKernelGGL(k_stress_accum, dim3(1), dim3(kThreads), 0, e->stream, (const double*)e->part,
-                       e->nb, e->ds * b * b, e->sacc);
-    PSX_HIP(hipGetLastError());
   }
   return PSX_OK;
 }
@@ -1380,12 +1667,14 @@ extern "C" psx_status psx_scft_param_grad(psx_scft* e, double* dFdchiN, double* 
   if (!e) return fail(PSX_ERR_INVALID, "null argument");
   if (!e->dens_valid) return fail(PSX_ERR_STATE, "densities before the parameter gradient");
   if (!e->swept) return fail(PSX_ERR_STATE, "propagate before the parameter gradient");
+  if (dFdfA && !e->diblock)
+    return fail(PSX_ERR_INVALID, "dF/dfA is defined for the AB diblock only");
   double Q;
   PSX_TRY(psx_scft_Q(e, &Q));
-  SweepBufs& s = e->sw[e->prec];
-  const size_t tb = prec_bytes(e->prec);
-  const void* fj = (const char*)s.F + (size_t)e->sA * e->N * tb;
-  const void* gj = (const char*)s.G + (size_t)(e->Ns - e->sA) * e->N * tb;
+  // junction of the diblock: q of B's head and q+ arriving there
+  const BlockInfo& bb = e->blocks.back();
+  const void* fj = slice_ptr(e, e->prec, bb.path[0], bb.off[0]);
+  const void* gj = slice_ptr(e, e->prec, bb.path[1], bb.off[1] + bb.steps);
   if (e->prec == PSX_FP64)
     hipLaunchKernelGGL(k_param_grad<double>, dim3(e->nb), dim3(kThreads), 0, e->stream,
                        (const double*)e->phiA, (const double*)e->phiB, (const double*)fj,
@@ -1410,13 +1699,14 @@ extern "C" psx_status psx_scft_param_grad(psx_scft* e, double* dFdchiN, double* 
   return PSX_OK;
 }
 
-// The launch that stands for axis a in tuning: a fused one if the sweep has one.
-static size_t representative_launch(const SweepBufs& s, int a) {
+// The launch of path pi that stands for axis a in tuning: a fused one if the
+// path has one.
+static size_t representative_launch(const PathInfo& pi, int a) {
   size_t best = 0;
   bool found = false;
-  for (size_t j = 0; j < s.launch_axis.size(); ++j) {
-    if (s.launch_axis[j] != a) continue;
-    if (!found || (s.launch_nops[j] == 2 && s.launch_nops[best] != 2)) { best = j; found = true; }
+  for (size_t j = 0; j < pi.launch_axis.size(); ++j) {
+    if (pi.launch_axis[j] != a) continue;
+    if (!found || (pi.launch_nops[j] == 2 && pi.launch_nops[best] != 2)) { best = j; found = true; }
   }
   return best;
 }
@@ -1429,11 +1719,10 @@ extern "C" psx_status psx_scft_time_step(psx_scft* e, int reps, double* ms_step,
   PSX_HIP(hipEventCreate(&b));
   // reps whole sweeps; per-step figures divide by reps * Ns
   const int na = (int)e->axes.size();
-  const size_t nl = e->sw[e->prec].launch_axis.size();
   auto run = [&](int axis) -> psx_status {
     for (int r = 0; r < reps; ++r) {
-      if (e->prec == PSX_FP64) PSX_TRY(sweep_range<double>(e, 0, nl, axis));
-      else PSX_TRY(sweep_range<float>(e, 0, nl, axis));
+      if (e->prec == PSX_FP64) PSX_TRY(sweep_range<double>(e, axis));
+      else PSX_TRY(sweep_range<float>(e, axis));
     }
     return PSX_OK;
   };
@@ -1515,10 +1804,15 @@ extern "C" psx_status psx_scft_autotune(psx_scft* e, int reps) {
   if (!e || reps < 1) return fail(PSX_ERR_INVALID, "bad argument");
   if (!e->sw[e->prec].wfac_valid) PSX_TRY(compute_wfac(e));
   const int na = (int)e->axes.size();
+  // tuned on round 0, all of its paths in one launch (each candidate sees
+  // the same work)
+  const int np = (int)e->rounds[0].size();
+  const PathInfo& p0 = e->paths[e->rounds[0][0]];
   for (int i = 0; i < na; ++i) {
-    const PairIO* fio = e->sw[e->prec].fio + representative_launch(e->sw[e->prec], i) * 2;
-    if (e->prec == PSX_FP64) PSX_TRY(tune_pass<double>(e->axes[i], e->prec, fio, 2, e->stream, reps, &e->variant[e->prec][i]));
-    else PSX_TRY(tune_pass<float>(e->axes[i], e->prec, fio, 2, e->stream, reps, &e->variant[e->prec][i]));
+    const PairIO* fio = e->sw[e->prec].rtab[0] + representative_launch(p0, i) * np;
+    const int nf = round_uniform(e, 0) ? np : 1;
+    if (e->prec == PSX_FP64) PSX_TRY(tune_pass<double>(e->axes[i], e->prec, fio, nf, e->stream, reps, &e->variant[e->prec][i]));
+    else PSX_TRY(tune_pass<float>(e->axes[i], e->prec, fio, nf, e->stream, reps, &e->variant[e->prec][i]));
   }
   drop_graphs(e);
   e->swept = false;
@@ -1546,14 +1840,16 @@ static psx_status batch_tables(psx_batch* b, int prec) {
     psx_scft* e0 = b->eng[c.members[0]];
     const int na = (int)e0->axes.size();
     const int K = (int)c.members.size();
+    // members are diblocks: paths 0 and 1 (forward, backward) of equal length
     std::vector<std::vector<PairIO>> per;
-    for (int m : c.members) per.push_back(fio_table(b->eng[m], prec));
+    for (int m : c.members)
+      for (int h = 0; h < 2; ++h) per.push_back(path_fio(b->eng[m], prec, h));
     (void)na;
-    const size_t nl = per[0].size() / 2;
+    const size_t nl = per[0].size();
     std::vector<PairIO> tab(nl * 2 * K);
     for (size_t j = 0; j < nl; ++j)
       for (int k = 0; k < K; ++k)
-        for (int h = 0; h < 2; ++h) tab[(j * K + k) * 2 + h] = per[k][j * 2 + h];
+        for (int h = 0; h < 2; ++h) tab[(j * K + k) * 2 + h] = per[2 * k + h][j];
     PSX_TRY(upload((void**)&c.fio[prec], tab.data(), tab.size() * sizeof(PairIO)));
   }
   return PSX_OK;
@@ -1637,7 +1933,7 @@ extern "C" psx_status psx_batch_autotune(psx_batch* b, int reps, int* h_choice) 
     const int na = (int)e0->axes.size();
     const int K = (int)c.members.size();
     for (int i = 0; i < na; ++i) {
-      const PairIO* fio = c.fio[b->prec] + representative_launch(e0->sw[b->prec], i) * 2 * K;
+      const PairIO* fio = c.fio[b->prec] + representative_launch(e0->paths[0], i) * 2 * K;
       int* v = &c.variant[b->prec][i];
       if (b->prec == PSX_FP64) PSX_TRY(tune_pass<double>(e0->axes[i], b->prec, fio, 2 * K, b->stream, reps, v));
       else PSX_TRY(tune_pass<float>(e0->axes[i], b->prec, fio, 2 * K, b->stream, reps, v));
@@ -1652,13 +1948,13 @@ extern "C" psx_status psx_batch_autotune(psx_batch* b, int reps, int* h_choice) 
 template <typename T>
 static psx_status batch_sweep(psx_batch* b) {
   size_t nmax = 0;
-  for (auto& c : b->cls) nmax = std::max(nmax, b->eng[c.members[0]]->sw[b->prec].launch_axis.size());
+  for (auto& c : b->cls) nmax = std::max(nmax, b->eng[c.members[0]]->paths[0].launch_axis.size());
   for (size_t j = 0; j < nmax; ++j)
     for (auto& c : b->cls) {
       psx_scft* e0 = b->eng[c.members[0]];
-      const SweepBufs& s = e0->sw[b->prec];
-      if (j >= s.launch_axis.size()) continue;
-      const int a = s.launch_axis[j];
+      const PathInfo& pi = e0->paths[0];
+      if (j >= pi.launch_axis.size()) continue;
+      const int a = pi.launch_axis[j];
       const int K = (int)c.members.size();
       PSX_TRY(launch_pair<T>(e0->axes[a], b->prec, c.fio[b->prec] + j * 2 * K, 2 * K, b->stream,
                              c.variant[b->prec][a]));

@@ -9,6 +9,8 @@ Names and argument meanings follow PolySwift++:
   ``PsFlexPseudoSpec`` blocks (pspolymer/PsBlockCopolymer.cpp:189-239),
   the ``flory`` interaction and the ``steepestDescent`` updater
   (pseffhamil/PsSteepDUpdater.cpp:159-333), plus Anderson mixing.
+* ``BlockCopolymerSCFT`` -- the same for any tree of A and B ``Block``s
+  joined by ``headjoined`` / ``tailjoined`` (linear multiblocks, stars).
 
 Every call goes through the C ABI of libpsx_scft.so on the current HIP
 device; errors raise ``PsxError`` the way the reference throws
@@ -113,12 +115,15 @@ class DiblockSCFT:
         check(self._L.psx_scft_create(ctypes.byref(g), ctypes.byref(c), _prec(prec), stream,
                                       ctypes.byref(h)))
         self._h = h
-        sA, sB = ctypes.c_int(), ctypes.c_int()
-        check(self._L.psx_scft_block_steps(h, ctypes.byref(sA), ctypes.byref(sB)))
-        self.sA, self.sB = sA.value, sB.value
-        self.Ns = self.sA + self.sB
+        self._steps()
         if quadrature != "reference":
             self.setQuadrature(quadrature)
+
+    def _steps(self):
+        sA, sB = ctypes.c_int(), ctypes.c_int()
+        check(self._L.psx_scft_block_steps(self._h, ctypes.byref(sA), ctypes.byref(sB)))
+        self.sA, self.sB = sA.value, sB.value
+        self.Ns = self.sA + self.sB
 
     # -- fields -------------------------------------------------------------
     def setFields(self, wA, wB, p=None) -> None:
@@ -166,10 +171,26 @@ class DiblockSCFT:
         self.solveQ()
         return self.densities()
 
-    def slice(self, backward: bool, t: int) -> np.ndarray:
+    def slice(self, path, t: int) -> np.ndarray:
+        """Slice t of propagation path `path` (0 forward sweep, 1 backward
+        sweep of a chain; a bool picks the backward sweep)."""
         out = np.empty(self.N)
-        check(self._L.psx_scft_get_slice_host(self._h, int(bool(backward)), int(t), _ptr(out)))
+        check(self._L.psx_scft_get_slice_host(self._h, int(path), int(t), _ptr(out)))
         return out.reshape(self.shape)
+
+    def blockSlice(self, block: int, from_tail: bool, s: int) -> np.ndarray:
+        """Slice s of a block's propagator from its head or from its tail
+        (the reference's qs / qts of that block)."""
+        out = np.empty(self.N)
+        check(self._L.psx_scft_get_block_slice_host(self._h, int(block), int(bool(from_tail)), int(s),
+                                                    _ptr(out)))
+        return out.reshape(self.shape)
+
+    def graphInfo(self):
+        """(blocks, propagation paths, rounds of concurrent paths)."""
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(self._L.psx_scft_graph_info(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
 
     # -- effective Hamiltonian ----------------------------------------------
     def steepestDescent(self, lambda0: float, lambda1: float = 0.0) -> float:
@@ -244,10 +265,7 @@ class DiblockSCFT:
         self.cell = tuple(float(c) for c in cell)
         self.N = int(np.prod(self.shape))
         self.fA, self.ds, self.chiN = float(fA), float(ds), float(chiN)
-        sA, sB = ctypes.c_int(), ctypes.c_int()
-        check(self._L.psx_scft_block_steps(handle, ctypes.byref(sA), ctypes.byref(sB)))
-        self.sA, self.sB = sA.value, sB.value
-        self.Ns = self.sA + self.sB
+        self._steps()
         return self
 
     def close(self):
@@ -256,6 +274,60 @@ class DiblockSCFT:
         self._h = None
 
     __del__ = close
+
+
+@dataclass
+class Block:
+    """One ``Block`` of a ``blockCopolymer`` (pspolymer/PsBlockCopolymer.cpp,
+    psbase/PsBlockBase.cpp:110-131): species "A"/"B" (or 0/1; the block's
+    scfield), lengthfrac, bSegRatio, and the indices of the blocks named in
+    headjoined / tailjoined (empty = freeEnd)."""
+    species: object
+    lengthfrac: float
+    bSegRatio: float = 1.0
+    head: tuple = ()
+    tail: tuple = ()
+
+    def c(self) -> _abi.BlockDesc:
+        sp = {"A": 0, "B": 1, 0: 0, 1: 1}.get(self.species)
+        if sp is None:
+            raise ValueError(f"species must be A or B, not {self.species!r}")
+        if len(self.head) > _abi.PSX_MAX_JOIN or len(self.tail) > _abi.PSX_MAX_JOIN:
+            raise ValueError("too many joined blocks")
+        d = _abi.BlockDesc()
+        d.species, d.lengthfrac, d.bSegRatio = sp, float(self.lengthfrac), float(self.bSegRatio)
+        d.nhead, d.ntail = len(self.head), len(self.tail)
+        for i, j in enumerate(self.head):
+            d.head[i] = int(j)
+        for i, j in enumerate(self.tail):
+            d.tail[i] = int(j)
+        return d
+
+
+class BlockCopolymerSCFT(DiblockSCFT):
+    """Bulk melt of one AB block copolymer of any tree architecture
+    (``PsBlockCopolymer`` with ``flexPseudoSpec`` blocks sharing one ds):
+    linear multiblocks, stars, combs.  Same methods as ``DiblockSCFT``;
+    ``fA`` is the total A fraction."""
+
+    def __init__(self, shape, cell, blocks, ds, chiN, prec="fp64", stream=None,
+                 quadrature="reference"):
+        self._L = _abi.load()
+        self.shape = tuple(int(s) for s in shape)
+        self.cell = tuple(float(c) for c in cell)
+        self.N = int(np.prod(self.shape))
+        self.blocks = [b if isinstance(b, Block) else Block(*b) for b in blocks]
+        self.ds, self.chiN = float(ds), float(chiN)
+        self.fA = sum(b.lengthfrac for b in self.blocks if b.species in ("A", 0))
+        g = _abi.Grid(*self.shape, *self.cell)
+        arr = (_abi.BlockDesc * len(self.blocks))(*[b.c() for b in self.blocks])
+        h = ctypes.c_void_p()
+        check(self._L.psx_scft_create_blocks(ctypes.byref(g), len(self.blocks), arr, self.ds,
+                                             self.chiN, _prec(prec), stream, ctypes.byref(h)))
+        self._h = h
+        self._steps()
+        if quadrature != "reference":
+            self.setQuadrature(quadrature)
 
 
 class BatchSCFT:

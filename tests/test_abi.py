@@ -58,14 +58,19 @@ int main(void) {
   printf("%zu %zu %zu\n", sizeof(psx_diblock), offsetof(psx_diblock, bB), offsetof(psx_diblock, chiN));
   printf("%zu %zu %zu %zu\n", sizeof(psx_mixing), offsetof(psx_mixing, depth),
          offsetof(psx_mixing, beta), offsetof(psx_mixing, f_order));
+  printf("%zu %zu %zu %zu %zu %d %d\n", sizeof(psx_block), offsetof(psx_block, lengthfrac),
+         offsetof(psx_block, nhead), offsetof(psx_block, head), offsetof(psx_block, tail),
+         PSX_MAX_BLOCKS, PSX_MAX_JOIN);
   return 0;
 }
 '''
     out = _compile_and_run(src).split()
-    g, c, m = _abi.Grid, _abi.Diblock, _abi.Mixing
+    g, c, m, b = _abi.Grid, _abi.Diblock, _abi.Mixing, _abi.BlockDesc
     want = [ctypes.sizeof(g), g.dx.offset, g.dz.offset,
             ctypes.sizeof(c), c.bB.offset, c.chiN.offset,
-            ctypes.sizeof(m), m.depth.offset, m.beta.offset, m.f_order.offset]
+            ctypes.sizeof(m), m.depth.offset, m.beta.offset, m.f_order.offset,
+            ctypes.sizeof(b), b.lengthfrac.offset, b.nhead.offset, b.head.offset, b.tail.offset,
+            _abi.PSX_MAX_BLOCKS, _abi.PSX_MAX_JOIN]
     assert [int(x) for x in out] == want
 
 
