@@ -1,0 +1,136 @@
+"""Input-deck subset reader (txpp .pre files) on CPU: expressions,
+conditionals, macros, disabled blocks, the deck -> engine mapping, and the
+reference's own melt decks when the reference tree is present."""
+import os
+
+import pytest
+
+from polyswift_plusplus_amd import deck
+from polyswift_plusplus_amd.deck import DeckError, evaluate
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DECKS = os.path.join(HERE, "golden", "decks")
+MACROS = os.path.join(DECKS, "macros")
+REF = "/root/reference"
+
+
+def test_expressions():
+    env = {"chiNAB": 24.0, "NLEN": 100, "NZ": 1}
+    assert evaluate("chiNAB/float(NLEN)", env) == 0.24
+    assert evaluate("NZ == 1 and NLEN > 10", env) is True
+    assert evaluate("-(2 + 3) * 2 ** 3 % 7", env) == -(2 + 3) * 2 ** 3 % 7
+    assert evaluate("sqrt(16) + abs(-1) + min(3, 4)", env) == 8.0
+
+
+@pytest.mark.parametrize("bad", ["__import__('os')", "NLEN.real", "open('x')", "[x for x in y]",
+                                 "lambda: 1", "undefined_name + 1"])
+def test_expressions_execute_nothing_else(bad):
+    with pytest.raises(DeckError):
+        evaluate(bad, {"NLEN": 100})
+
+
+def test_diblock_fixture():
+    cfg = deck.configure(deck.load(os.path.join(DECKS, "melt_diblock.pre")))
+    assert cfg.shape == (16, 12, 10) and cfg.cell == (0.3, 0.25, 0.35)
+    assert [(b.species, b.head, b.tail) for b in cfg.blocks] == [("A", (), (1,)), ("B", (0,), ())]
+    assert cfg.blocks[0].lengthfrac == 0.30 and cfg.blocks[1].lengthfrac == pytest.approx(0.7)
+    assert cfg.ds == 0.05
+    assert cfg.chiN == pytest.approx(15.0, rel=1e-15)
+    assert cfg.lambdas == (0.1, 0.05)           # the $ else branch (NZ != 1)
+    assert cfg.nsteps == 12 and cfg.seed == 4242 and cfg.history_period == 2
+    assert cfg.species_fields == ("totA", "totB")
+
+
+def test_star_fixture_with_macro_file():
+    root = deck.load(os.path.join(DECKS, "melt_star.pre"), search_path=[MACROS])
+    cfg = deck.configure(root)
+    assert cfg.shape == (16, 12, 10) and cfg.cell == (0.3, 0.3, 0.3)
+    assert cfg.block_names == ["armA", "armB", "centreA"]
+    assert [(b.species, b.head, b.tail, b.bSegRatio) for b in cfg.blocks] == [
+        ("A", (), (1, 2), 1.0), ("B", (), (0, 2), 1.2), ("A", (0, 1), (), 1.0)]
+    assert cfg.chiN == pytest.approx(14.0) and cfg.noise == 0.005
+
+
+def test_disabled_block_is_dropped():
+    root = deck.load(os.path.join(DECKS, "melt_diblock.pre"))
+    inter = root.one("EffHamil").one("Interaction")
+    assert inter.children == []
+
+
+def test_missing_macro_file():
+    with pytest.raises(DeckError, match="not found"):
+        deck.parse("$ import nowhere\n")
+
+
+@pytest.mark.parametrize("text,msg", [
+    ("<Grid g>\n numCellsGlobal = [4 4 4]\n", "not closed"),
+    ("$ if 1 == 1\n", "endif"),
+    ("</Grid>\n", "unbalanced"),
+    ("what is this\n", "cannot parse"),
+])
+def test_malformed(text, msg):
+    with pytest.raises(DeckError, match=msg):
+        deck.parse(text)
+
+
+def test_unsupported_physics_raises():
+    base = open(os.path.join(DECKS, "melt_diblock.pre")).read()
+    with pytest.raises(DeckError, match="Solvent"):
+        deck.configure(deck.parse(base + "\n<Solvent s>\n kind = simpleSolvent\n</Solvent>\n"))
+    with pytest.raises(DeckError, match="one Polymer"):
+        extra = base[base.index("<Polymer"):base.index("</Polymer>") + len("</Polymer>")]
+        deck.configure(deck.parse(base + "\n" + extra.replace("diblock1", "diblock2")))
+
+
+# ---- the reference's own decks (read from the reference tree when present)
+REF_DECKS = os.path.join(REF, "pstests", "tests", "diblock")
+needs_ref = pytest.mark.skipif(not os.path.isdir(REF_DECKS), reason="reference tree not present")
+
+
+@needs_ref
+def test_reference_diblock_deck():
+    cfg = deck.configure(deck.load(os.path.join(REF_DECKS, "diblock.pre")))
+    assert cfg.shape == (64, 64, 1) and cfg.cell == (0.1, 0.1, 0.1)
+    assert [(b.species, b.lengthfrac, b.head, b.tail) for b in cfg.blocks] == [
+        ("A", 0.5, (), (1,)), ("B", 0.5, (0,), ())]
+    assert cfg.ds == 0.05 and cfg.chiN == pytest.approx(12.0)
+    assert cfg.lambdas == (0.4, 0.2) and cfg.noise == 0.002
+    assert cfg.nsteps == 200 and cfg.seed == 38383
+
+
+@needs_ref
+def test_reference_triblock_and_star_decks():
+    tri = deck.configure(deck.load(os.path.join(REF_DECKS, "triblock.pre")))
+    assert [(b.species, b.lengthfrac, b.head, b.tail) for b in tri.blocks] == [
+        ("A", 0.25, (), (1,)), ("B", 0.5, (0,), (2,)), ("A", 0.25, (1,), ())]
+    assert tri.chiN == pytest.approx(20.0) and tri.nsteps == 400
+    star = deck.configure(deck.load(os.path.join(REF_DECKS, "star3ab.pre")))
+    assert [(b.species, b.head, b.tail) for b in star.blocks] == [
+        ("A", (), (1, 2)), ("B", (), (0, 2)), ("A", (0, 1), ())]
+    asym = deck.configure(deck.load(os.path.join(REF_DECKS, "asymmetric.pre")))
+    assert asym.shape == (128, 128, 1) and asym.blocks[0].bSegRatio == 3.16227
+    assert asym.chiN == pytest.approx(12.5)
+
+
+@needs_ref
+def test_reference_example_with_expressions():
+    path = os.path.join(REF, "psexamples", "PSimBase", "linearTriblock", "linearTriblock.pre")
+    cfg = deck.configure(deck.load(path))
+    assert cfg.shape == (100, 100, 1)
+    assert [b.lengthfrac for b in cfg.blocks] == [0.2, 0.6, 0.2]
+    assert cfg.chiN == pytest.approx(24.0, rel=1e-14)
+
+
+@needs_ref
+@pytest.mark.parametrize("name", ["chargedAB.pre", "abSolventMix.pre", "abHomopMix.pre",
+                                  "polydBulk.pre", "multispecf.pre", "pspi.pre", "tri3abc.pre"])
+def test_reference_decks_outside_the_subset_raise(name):
+    """Decks with solvents, blends, charges, polydispersity, filters or three
+    species parse, then stop with a DeckError naming the feature."""
+    path = os.path.join(REF_DECKS, name)
+    try:
+        root = deck.load(path)
+    except DeckError:
+        return
+    with pytest.raises(DeckError):
+        deck.configure(root)
