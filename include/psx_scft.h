@@ -218,6 +218,20 @@ psx_status psx_scft_box_stress(psx_scft* e, double* h_dFdL);
  * reference. */
 psx_status psx_scft_set_quadrature(psx_scft* e, int mode);
 
+/* Contour checkpointing for long chains (no reference counterpart): with
+ * every = K > 0 the backward sweep keeps only its slices 0, K, 2K, ... and
+ * its final slice.  Densities, the iteration and the box stress then
+ * recompute K backward slices at a time from a checkpoint while the forward
+ * propagator advances over the points that pair with them (one extra
+ * backward sweep; memory per direction about Ns/K + 2K slices instead of
+ * Ns + 1).  every = 0 stores every slice again (the default).  Linear chains
+ * with every block oriented head to tail from block 0 only (PSX_ERR_INVALID
+ * otherwise); engines of a batch cannot switch (PSX_ERR_STATE).  While on,
+ * slice access, psx_scft_param_grad, psx_scft_time_step and
+ * psx_scft_autotune return PSX_ERR_STATE.  Fields are kept; propagate again
+ * after switching. */
+psx_status psx_scft_set_checkpointing(psx_scft* e, int every);
+
 /* Change the cell sizes (box lengths / grid counts) in place: rewrites the
  * exp(-ds b^2 k^2) tables (PsFlexPseudoSpec::build_k2 for the new
  * cellSizes); fields are kept.  Used by the box relaxation. */

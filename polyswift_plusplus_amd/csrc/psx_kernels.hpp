@@ -349,6 +349,13 @@ __global__ void k_fill(T* __restrict__ x, long long N, T v) {
 }
 
 template <typename T>
+__global__ void k_copy(const T* __restrict__ x, long long N, T* __restrict__ y) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long long)gridDim.x * blockDim.x)
+    y[i] = x[i];
+}
+
+template <typename T>
 __global__ void k_to_f64(const T* __restrict__ x, long long N, double* __restrict__ y) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
        i += (long long)gridDim.x * blockDim.x)
@@ -371,7 +378,8 @@ enum QuadMode {
   QM_SD = 1,        // phi + steepest-descent update + w-factors + partials
   QM_RESID = 2,     // phi + residual into the Anderson ring + err partial
   QM_SD_PHI = 3,    // steepest-descent update from stored phi
-  QM_FE = 4         // free-energy partials of stored phi and current w
+  QM_FE = 4,        // free-energy partials of stored phi and current w
+  QM_ACC = 5        // raw contour sums of one chunk into phiA/phiB (checkpointed sweeps)
 };
 constexpr int kNPart = 4;   // sum p_raw, sum(-phi w), sum(chiN phiA phiB), sum dH^2
 
@@ -399,6 +407,7 @@ struct QuadArgs {
   DevState* st;
   double* part;                  // [gridDim.x][kNPart]
   int mode, f_order;
+  int acc_in;                    // add the raw sums of earlier chunks held in phiA/phiB
 };
 
 template <typename T>
@@ -420,6 +429,7 @@ k_quad_update(QuadArgs a) {
       // per block: sum_s c_s q_head[s] q_tail[M-1-s] (pspolymer/PsBlock.cpp:256-380),
       // added into the block's species (pspolymer/PsBlock.cpp:154-196)
       double qa = 0.0, qb = 0.0;
+      if (a.acc_in) { qa = a.phiA[r]; qb = a.phiB[r]; }
       for (int b = 0; b < a.nblk; ++b) {
         const T* __restrict__ H = (const T*)a.blk[b].H + r;
         const T* __restrict__ Tq = (const T*)a.blk[b].T + r;
@@ -432,6 +442,7 @@ k_quad_update(QuadArgs a) {
         if (a.blk[b].species == 0) qa += acc;
         else qb += acc;
       }
+      if (a.mode == QM_ACC) { a.phiA[r] = qa; a.phiB[r] = qb; continue; }
       phA = qa * invQ * a.dens;
       phB = qb * invQ * a.dens;
       a.phiA[r] = phA;

@@ -463,6 +463,22 @@ struct SweepBufs {
   std::vector<std::array<void*, 3>> emul;    // [b class][axis idx] spectral factors (T)
   std::vector<PairIO*> rtab;       // per round: device table [launch][path slot]
   bool wfac_valid = false;
+  // contour checkpointing (linear chains; see CkptSeg)
+  void* ck = nullptr;              // backward slices 0, K, 2K, ... and the final one
+  void* seg = nullptr;             // K + 1 recomputed backward slices
+  void* fch[2] = {nullptr, nullptr};   // forward chunks, K + 1 slices each
+  void* scr = nullptr;             // 3 rotating slices of the checkpointing sweep
+  PairIO* btab = nullptr;          // backward sweep writing the checkpoints
+  std::vector<PairIO*> gtab, ftab; // per chunk: backward recompute, forward advance
+};
+
+// One chunk of a checkpointed contour pass: backward slices u0..u1 are
+// recomputed from checkpoint u0 = jK, and the forward propagator advances
+// over t0..t1 = Ns-u1 .. Ns-u0, the points that pair with them.  Chunks are
+// kept in processing order (t ascending).
+struct CkptSeg {
+  int u0, u1, t0, t1;
+  std::vector<int> g_axis, f_axis;   // launch axes of the two range sweeps
 };
 
 struct GraphKey {
@@ -497,6 +513,11 @@ struct psx_scft {
   std::vector<PathInfo> paths;
   std::vector<std::vector<int>> rounds; // path ids per round, slot order
   bool diblock = false;                 // two blocks A-B joined head to tail
+  int ckpt_K = 0;                       // contour checkpoint interval (0: every slice stored)
+  int nck = 0;                          // checkpoint slices
+  std::vector<CkptSeg> segs;
+  std::vector<int> btab_axis;
+  bool in_batch = false;                // owned by a psx_batch (its tables point at our slices)
   int prec;
   std::vector<Axis> axes;
   int variant[2][3] = {{-1, -1, -1}, {-1, -1, -1}};   // line-kernel choice per precision, pass
@@ -575,22 +596,21 @@ std::vector<std::pair<SweepOp, int>> sweep_launches(int na, int Ns) {
   return out;
 }
 
-// Field descriptors of every launch of path p (see sweep_launches): the
-// w-factor and spectral factors of each contour step are those of the
-// step's block.
-std::vector<PairIO> path_fio(const psx_scft* e, int prec, int p) {
+// Field descriptors of the launches (see sweep_launches) that advance path p
+// over its contour steps s0 .. s0+nsteps-1, with slice(k) giving the device
+// address of the k-th slice of that range (k = 0 .. nsteps).  The w-factor
+// and spectral factors of each step are those of the step's block.
+template <typename SliceFn>
+std::vector<PairIO> range_fio(const psx_scft* e, int prec, int p, int s0, int nsteps, SliceFn slice) {
   const SweepBufs& s = e->sw[prec];
   const PathInfo& pi = e->paths[p];
-  const size_t tb = prec_bytes(prec);
   const int na = (int)e->axes.size();
-  const auto launches = sweep_launches(na, pi.len);
+  const auto launches = sweep_launches(na, nsteps);
   std::vector<PairIO> tab(launches.size());
-  const char* base = (const char*)s.path[p].P;
   void* tmp = s.path[p].tmp;
-  auto blk = [&](int st) -> const BlockInfo& { return e->blocks[pi.step_block[st]]; };
+  auto blk = [&](int st) -> const BlockInfo& { return e->blocks[pi.step_block[s0 + st]]; };
   auto W = [&](int st) { return (const void*)s.wf[blk(st).species]; };
   auto E = [&](int a, int st) { return (const void*)s.emul[blk(st).bclass][a]; };
-  auto slice = [&](int st) { return (void*)(base + (size_t)st * e->N * tb); };
   for (size_t j = 0; j < launches.size(); ++j) {
     const SweepOp& o = launches[j].first;
     PairIO& f = tab[j];
@@ -613,6 +633,14 @@ std::vector<PairIO> path_fio(const psx_scft* e, int prec, int p) {
     }
   }
   return tab;
+}
+
+// All launches of path p into its own slice array.
+std::vector<PairIO> path_fio(const psx_scft* e, int prec, int p) {
+  const char* base = (const char*)e->sw[prec].path[p].P;
+  const size_t sb = (size_t)e->N * prec_bytes(prec);
+  return range_fio(e, prec, p, 0, e->paths[p].len,
+                   [&](int k) { return (void*)(base + (size_t)k * sb); });
 }
 
 // Paths of round r have identical launch lists (equal lengths).
@@ -641,13 +669,80 @@ psx_status upload_round_table(psx_scft* e, int prec, int r) {
   return upload((void**)&e->sw[prec].rtab[r], tab.data(), tab.size() * sizeof(PairIO));
 }
 
+// Checkpointed mode (linear chains, path 0 forward from block 0's free head,
+// path 1 backward): the backward sweep stores slices u = 0, K, 2K, ... and
+// its final slice; each chunk recomputes K backward slices from a checkpoint
+// and advances the forward propagator over the points that pair with them.
+void ckpt_plan(psx_scft* e) {
+  const int Ns = e->paths[1].len, K = e->ckpt_K;
+  e->nck = Ns / K + 1 + (Ns % K ? 1 : 0);
+  const int na = (int)e->axes.size();
+  e->btab_axis.clear();
+  for (const auto& l : sweep_launches(na, Ns)) e->btab_axis.push_back(l.first.axis);
+  e->segs.clear();
+  const int nseg = (Ns + K - 1) / K;
+  for (int j = nseg - 1; j >= 0; --j) {
+    CkptSeg c;
+    c.u0 = j * K;
+    c.u1 = std::min((j + 1) * K, Ns);
+    c.t0 = Ns - c.u1;
+    c.t1 = Ns - c.u0;
+    for (const auto& l : sweep_launches(na, c.u1 - c.u0)) {
+      c.g_axis.push_back(l.first.axis);
+      c.f_axis.push_back(l.first.axis);
+    }
+    e->segs.push_back(c);
+  }
+}
+
+psx_status ensure_ckpt(psx_scft* e, int prec) {
+  SweepBufs& s = e->sw[prec];
+  const size_t sb = (size_t)e->N * prec_bytes(prec);
+  const int K = e->ckpt_K, Ns = e->paths[1].len;
+  PSX_TRY(dalloc(&s.ck, (size_t)e->nck * sb));
+  PSX_TRY(dalloc(&s.seg, (size_t)(K + 1) * sb));
+  PSX_TRY(dalloc(&s.fch[0], (size_t)(K + 1) * sb));
+  PSX_TRY(dalloc(&s.fch[1], (size_t)(K + 1) * sb));
+  PSX_TRY(dalloc(&s.scr, 3 * sb));
+  char* ck = (char*)s.ck;
+  char* scr = (char*)s.scr;
+  auto bslice = [&](int k) -> void* {
+    if (k % K == 0) return ck + (size_t)(k / K) * sb;
+    if (k == Ns) return ck + (size_t)(e->nck - 1) * sb;
+    return scr + (size_t)(k % 3) * sb;
+  };
+  std::vector<PairIO> bt = range_fio(e, prec, 1, 0, Ns, bslice);
+  PSX_TRY(upload((void**)&s.btab, bt.data(), bt.size() * sizeof(PairIO)));
+  s.gtab.assign(e->segs.size(), nullptr);
+  s.ftab.assign(e->segs.size(), nullptr);
+  for (size_t ci = 0; ci < e->segs.size(); ++ci) {
+    const CkptSeg& c = e->segs[ci];
+    char* seg = (char*)s.seg;
+    char* fch = (char*)s.fch[ci % 2];
+    std::vector<PairIO> gt = range_fio(e, prec, 1, c.u0, c.u1 - c.u0,
+                                       [&](int k) { return (void*)(seg + (size_t)k * sb); });
+    std::vector<PairIO> ft = range_fio(e, prec, 0, c.t0, c.t1 - c.t0,
+                                       [&](int k) { return (void*)(fch + (size_t)k * sb); });
+    PSX_TRY(upload((void**)&s.gtab[ci], gt.data(), gt.size() * sizeof(PairIO)));
+    PSX_TRY(upload((void**)&s.ftab[ci], ft.data(), ft.size() * sizeof(PairIO)));
+  }
+  // the backward sweep starts at a free end: checkpoint 0 holds q = 1
+  const int g = grid_1d(e->N);
+  if (prec == PSX_FP64)
+    hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kThreads), 0, e->stream, (double*)s.ck, e->N, 1.0);
+  else
+    hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kThreads), 0, e->stream, (float*)s.ck, e->N, 1.0f);
+  PSX_HIP(hipGetLastError());
+  return PSX_OK;
+}
+
 psx_status ensure_sweep(psx_scft* e, int prec) {
   SweepBufs& s = e->sw[prec];
   if (s.ready) return PSX_OK;
   const size_t tb = prec_bytes(prec);
   s.path.assign(e->paths.size(), PathBufs());
   for (size_t p = 0; p < e->paths.size(); ++p) {
-    PSX_TRY(dalloc(&s.path[p].P, (size_t)(e->paths[p].len + 1) * e->N * tb));
+    if (e->ckpt_K == 0) PSX_TRY(dalloc(&s.path[p].P, (size_t)(e->paths[p].len + 1) * e->N * tb));
     PSX_TRY(dalloc(&s.path[p].tmp, e->N * tb));
   }
   PSX_TRY(dalloc(&s.wf[0], e->N * tb));
@@ -666,11 +761,17 @@ psx_status ensure_sweep(psx_scft* e, int prec) {
       }
     }
   }
+  const int g = grid_1d(e->N);
+  if (e->ckpt_K > 0) {
+    PSX_TRY(ensure_ckpt(e, prec));
+    s.ready = true;
+    s.wfac_valid = false;
+    return PSX_OK;
+  }
   // field tables of every launch of every path (static: graphs capture them)
   s.rtab.assign(e->rounds.size(), nullptr);
   for (size_t r = 0; r < e->rounds.size(); ++r) PSX_TRY(upload_round_table(e, prec, (int)r));
   // free ends: q = 1 (pspolymer/PsBlock.cpp:146-147)
-  const int g = grid_1d(e->N);
   for (size_t p = 0; p < e->paths.size(); ++p) {
     if (!e->paths[p].init.empty()) continue;
     if (prec == PSX_FP64)
@@ -687,6 +788,12 @@ psx_status ensure_sweep(psx_scft* e, int prec) {
 void free_sweep(SweepBufs& s) {
   for (auto& pb : s.path) { dfree(pb.P); dfree(pb.tmp); }
   s.path.clear();
+  dfree(s.ck); dfree(s.seg); dfree(s.fch[0]); dfree(s.fch[1]); dfree(s.scr);
+  dfree((void*&)s.btab);
+  for (auto& t : s.gtab) dfree((void*&)t);
+  for (auto& t : s.ftab) dfree((void*&)t);
+  s.gtab.clear();
+  s.ftab.clear();
   dfree(s.wf[0]); dfree(s.wf[1]);
   for (auto& t : s.rtab) dfree((void*&)t);
   s.rtab.clear();
@@ -784,17 +891,79 @@ psx_status sweep_range(psx_scft* e, int only_axis = -1) {
   return PSX_OK;
 }
 
+// Checkpointed mode: the backward path only, writing its checkpoints.
+template <typename T>
+psx_status sweep_ckpt(psx_scft* e) {
+  const PairIO* tab = e->sw[e->prec].btab;
+  for (size_t j = 0; j < e->btab_axis.size(); ++j) {
+    const int a = e->btab_axis[j];
+    PSX_TRY(launch_pair<T>(e->axes[a], e->prec, tab + j, 1, e->stream, e->variant[e->prec][a]));
+  }
+  return PSX_OK;
+}
+
 psx_status sweep(psx_scft* e) {
+  if (e->ckpt_K > 0) return e->prec == PSX_FP64 ? sweep_ckpt<double>(e) : sweep_ckpt<float>(e);
   if (e->prec == PSX_FP64) return sweep_range<double>(e);
   return sweep_range<float>(e);
+}
+
+// Checkpointed mode, chunk ci: recompute its backward slices (second stream)
+// while the forward propagator advances over the paired points.
+template <typename T>
+psx_status ckpt_chunk_sweeps(psx_scft* e, size_t ci) {
+  SweepBufs& s = e->sw[e->prec];
+  const CkptSeg& c = e->segs[ci];
+  const size_t sb = (size_t)e->N * sizeof(T);
+  const int g = grid_1d(e->N);
+  hipStream_t s2 = e->pstream.size() > 1 ? e->pstream[1] : e->stream;
+  if (s2 != e->stream) {
+    PSX_HIP(hipEventRecord(e->ev_fork, e->stream));
+    PSX_HIP(hipStreamWaitEvent(s2, e->ev_fork, 0));
+  }
+  const int j = c.u0 / e->ckpt_K;
+  hipLaunchKernelGGL(k_copy<T>, dim3(g), dim3(kThreads), 0, s2,
+                     (const T*)((const char*)s.ck + (size_t)j * sb), e->N, (T*)s.seg);
+  for (size_t k = 0; k < c.g_axis.size(); ++k) {
+    const int a = c.g_axis[k];
+    PSX_TRY(launch_pair<T>(e->axes[a], e->prec, s.gtab[ci] + k, 1, s2, e->variant[e->prec][a]));
+  }
+  T* cur = (T*)s.fch[ci % 2];
+  if (ci == 0) {
+    hipLaunchKernelGGL(k_fill<T>, dim3(g), dim3(kThreads), 0, e->stream, cur, e->N, (T)1);
+  } else {
+    const CkptSeg& pc = e->segs[ci - 1];
+    const T* prev = (const T*)((const char*)s.fch[(ci - 1) % 2] + (size_t)(pc.t1 - pc.t0) * sb);
+    hipLaunchKernelGGL(k_copy<T>, dim3(g), dim3(kThreads), 0, e->stream, prev, e->N, cur);
+  }
+  for (size_t k = 0; k < c.f_axis.size(); ++k) {
+    const int a = c.f_axis[k];
+    PSX_TRY(launch_pair<T>(e->axes[a], e->prec, s.ftab[ci] + k, 1, e->stream, e->variant[e->prec][a]));
+  }
+  if (s2 != e->stream) {
+    PSX_HIP(hipEventRecord(e->ev_join[1], s2));
+    PSX_HIP(hipStreamWaitEvent(e->stream, e->ev_join[1], 0));
+  }
+  PSX_HIP(hipGetLastError());
+  return PSX_OK;
+}
+
+psx_status ckpt_chunk(psx_scft* e, size_t ci) {
+  return e->prec == PSX_FP64 ? ckpt_chunk_sweeps<double>(e, ci) : ckpt_chunk_sweeps<float>(e, ci);
 }
 
 // Q = mean(qHeadInitial qHeadFinal) of block 0 (pspolymer/PsBlock.cpp:205-237);
 // a free head has qHeadInitial = 1.
 psx_status launch_Q(psx_scft* e) {
   const BlockInfo& b0 = e->blocks[0];
-  const void* fin = slice_ptr(e, e->prec, b0.path[1], b0.off[1] + b0.steps);
-  const void* ini = b0.cnt[0].empty() ? nullptr : slice_ptr(e, e->prec, b0.path[0], b0.off[0]);
+  const void* fin;
+  const void* ini = nullptr;
+  if (e->ckpt_K > 0) {   // block 0's head is the free start of the chain
+    fin = (const char*)e->sw[e->prec].ck + (size_t)(e->nck - 1) * e->N * prec_bytes(e->prec);
+  } else {
+    fin = slice_ptr(e, e->prec, b0.path[1], b0.off[1] + b0.steps);
+    if (!b0.cnt[0].empty()) ini = slice_ptr(e, e->prec, b0.path[0], b0.off[0]);
+  }
   const int g = e->nb;
   if (e->prec == PSX_FP64) {
     if (ini) hipLaunchKernelGGL(k_partial_dot<double>, dim3(g), dim3(kThreads), 0, e->stream, (const double*)ini, (const double*)fin, e->N, e->part);
@@ -813,7 +982,7 @@ QuadArgs quad_args(psx_scft* e, int mode, const psx_mixing* mix) {
   SweepBufs& s = e->sw[e->prec];
   QuadArgs a;
   std::memset(&a, 0, sizeof(a));
-  a.nblk = (int)e->blocks.size();
+  a.nblk = e->ckpt_K > 0 ? 0 : (int)e->blocks.size();   // checkpointed: per chunk
   int woff = 0;
   for (int b = 0; b < a.nblk; ++b) {
     const BlockInfo& bi = e->blocks[b];
@@ -843,7 +1012,43 @@ QuadArgs quad_args(psx_scft* e, int mode, const psx_mixing* mix) {
   return a;
 }
 
+// Checkpointed mode: the contour sums chunk by chunk (raw sums carried in
+// phiA/phiB), the last chunk finishing in `mode`.  Every (block, point) pair
+// belongs to one chunk: chunks own [t0, t1) and the last one also t1.
+template <typename T>
+psx_status launch_quad_ckpt(psx_scft* e, int mode, const psx_mixing* mix) {
+  SweepBufs& s = e->sw[e->prec];
+  const size_t sb = (size_t)e->N * sizeof(T);
+  const int Ns = e->paths[1].len;
+  std::vector<int> woff(e->blocks.size());
+  for (size_t b = 0, w = 0; b < e->blocks.size(); ++b) { woff[b] = (int)w; w += e->blocks[b].steps + 1; }
+  for (size_t ci = 0; ci < e->segs.size(); ++ci) {
+    PSX_TRY(ckpt_chunk(e, ci));
+    const CkptSeg& c = e->segs[ci];
+    const bool last = ci + 1 == e->segs.size();
+    QuadArgs a = quad_args(e, last ? mode : QM_ACC, mix);
+    a.acc_in = ci > 0;
+    const int tend = last ? c.t1 : c.t1 - 1;
+    for (size_t b = 0; b < e->blocks.size(); ++b) {
+      const BlockInfo& bi = e->blocks[b];
+      const int bs = bi.off[0], lo = std::max(bs, c.t0), hi = std::min(bs + bi.steps, tend);
+      if (lo > hi) continue;
+      QuadBlock& q = a.blk[a.nblk++];
+      q.H = (const char*)s.fch[ci % 2] + (size_t)(lo - c.t0) * sb;
+      q.T = (const char*)s.seg + (size_t)(Ns - hi - c.u0) * sb;
+      q.M = hi - lo + 1;
+      q.woff = woff[b] + (lo - bs);
+      q.species = bi.species;
+    }
+    hipLaunchKernelGGL(k_quad_update<T>, dim3(e->nb), dim3(kThreads), 0, e->stream, a);
+  }
+  PSX_HIP(hipGetLastError());
+  return PSX_OK;
+}
+
 psx_status launch_quad(psx_scft* e, int mode, const psx_mixing* mix) {
+  if (e->ckpt_K > 0 && mode != QM_SD_PHI && mode != QM_FE)
+    return e->prec == PSX_FP64 ? launch_quad_ckpt<double>(e, mode, mix) : launch_quad_ckpt<float>(e, mode, mix);
   QuadArgs a = quad_args(e, mode, mix);
   if (e->prec == PSX_FP64)
     hipLaunchKernelGGL(k_quad_update<double>, dim3(e->nb), dim3(kThreads), 0, e->stream, a);
@@ -1229,6 +1434,31 @@ extern "C" psx_status psx_scft_graph_info(const psx_scft* e, int* nblocks, int* 
   return PSX_OK;
 }
 
+extern "C" psx_status psx_scft_set_checkpointing(psx_scft* e, int every) {
+  if (!e) return fail(PSX_ERR_INVALID, "null engine");
+  if (every < 0) return fail(PSX_ERR_INVALID, "checkpoint interval must be >= 0");
+  if (every > 0) {
+    if (e->in_batch) return fail(PSX_ERR_STATE, "engine belongs to a batch");
+    bool chain = e->rounds.size() == 1 && e->paths.size() == 2 && e->paths[0].init.empty() &&
+                 e->blocks[0].cnt[0].empty();
+    for (const BlockInfo& b : e->blocks) chain = chain && b.path[0] == 0 && b.path[1] == 1;
+    if (!chain)
+      return fail(PSX_ERR_INVALID, "checkpointing needs a linear chain with every block oriented "
+                                   "head to tail from block 0");
+  }
+  if (every == e->ckpt_K) return PSX_OK;
+  PSX_HIP(hipStreamSynchronize(e->stream));
+  drop_graphs(e);
+  for (auto& sw : e->sw) free_sweep(sw);
+  e->ckpt_K = every;
+  if (every > 0) ckpt_plan(e);
+  else { e->segs.clear(); e->btab_axis.clear(); e->nck = 0; }
+  PSX_TRY(ensure_sweep(e, e->prec));
+  e->swept = false;
+  e->dens_valid = false;
+  return PSX_OK;
+}
+
 extern "C" psx_status psx_scft_set_quadrature(psx_scft* e, int mode) {
   if (!e) return fail(PSX_ERR_INVALID, "null engine");
   if (mode != 0 && mode != 1) return fail(PSX_ERR_INVALID, "quadrature mode must be 0 or 1");
@@ -1390,6 +1620,7 @@ static psx_status copy_slice_host(psx_scft* e, const void* src, double* out) {
 
 extern "C" psx_status psx_scft_get_slice_host(psx_scft* e, int path, int t, double* out) {
   if (!e || !out) return fail(PSX_ERR_INVALID, "null argument");
+  if (e->ckpt_K > 0) return fail(PSX_ERR_STATE, "slices are not stored in checkpointed mode");
   if (path < 0 || path >= (int)e->paths.size()) return fail(PSX_ERR_INVALID, "path index out of range");
   if (t < 0 || t > e->paths[path].len) return fail(PSX_ERR_INVALID, "slice index out of range");
   return copy_slice_host(e, slice_ptr(e, e->prec, path, t), out);
@@ -1400,6 +1631,7 @@ extern "C" psx_status psx_scft_get_block_slice_host(psx_scft* e, int block, int 
   if (!e || !out) return fail(PSX_ERR_INVALID, "null argument");
   if (block < 0 || block >= (int)e->blocks.size()) return fail(PSX_ERR_INVALID, "block index out of range");
   if (from_tail != 0 && from_tail != 1) return fail(PSX_ERR_INVALID, "from_tail must be 0 or 1");
+  if (e->ckpt_K > 0) return fail(PSX_ERR_STATE, "slices are not stored in checkpointed mode");
   const BlockInfo& b = e->blocks[block];
   if (t < 0 || t > b.steps) return fail(PSX_ERR_INVALID, "slice index out of range");
   return copy_slice_host(e, slice_ptr(e, e->prec, b.path[from_tail], b.off[from_tail] + t), out);
@@ -1608,36 +1840,62 @@ static psx_status ensure_stress(psx_scft* e) {
   return PSX_OK;
 }
 
-// For every contour step t of every block: q_t from the head and the
-// propagator from the tail after steps-1-t steps meet across the step
-// operator; sum their spectral product weighted by dK/dL.
+// One contour step's contribution to the stress: Z = (W f, W g) packed,
+// transformed, and its weighted spectral sum accumulated.
 template <typename T>
-static psx_status stress_sweep(psx_scft* e) {
+static psx_status stress_step(psx_scft* e, const T* f, const T* gg, const BlockInfo& blk) {
   SweepBufs& s = e->sw[e->prec];
   const int g = grid_1d(e->N);
+  const T* w = (const T*)s.wf[blk.species];
+  hipLaunchKernelGGL(k_stress_pack<T>, dim3(g), dim3(kThreads), 0, e->stream, f, gg, w, e->N,
+                     (double2*)e->zbuf);
+  for (auto& a : e->caxes) {
+    CplxIO io{};
+    io.in = e->zbuf; io.out = e->zbuf; io.kmul = nullptr;
+    io.in_real = 0; io.out_real = 0; io.do_fwd = 1; io.do_inv = 0;
+    PSX_TRY(launch_cplx<double>(a, PSX_FP64, io, e->stream));
+  }
+  StressArgs sa;
+  sa.z = (const double2*)e->zbuf;
+  sa.nx = e->dims[0]; sa.ny = e->dims[1]; sa.nz = e->dims[2];
+  for (int d = 0; d < 3; ++d) { sa.kf[d] = e->kf1[blk.bclass][d]; sa.kd2[d] = e->kd2[d]; }
+  sa.part = e->part;
+  hipLaunchKernelGGL(k_stress_reduce, dim3(e->nb), dim3(kThreads), 0, e->stream, sa);
+  hipLaunchKernelGGL(k_stress_accum, dim3(1), dim3(kThreads), 0, e->stream, (const double*)e->part,
+                     e->nb, e->ds * blk.b * blk.b, e->sacc);
+  PSX_HIP(hipGetLastError());
+  return PSX_OK;
+}
+
+// For every contour step t of every block: q_t from the head and the
+// propagator from the tail after steps-1-t steps meet across the step
+// operator; sum their spectral product weighted by dK/dL.  Checkpointed:
+// the same pairs, chunk by chunk (step t of the chain pairs forward slice t
+// with backward slice Ns-1-t).
+template <typename T>
+static psx_status stress_sweep(psx_scft* e) {
   PSX_HIP(hipMemsetAsync(e->sacc, 0, 4 * sizeof(double), e->stream));
+  if (e->ckpt_K > 0) {
+    SweepBufs& s = e->sw[e->prec];
+    const size_t sb = (size_t)e->N * sizeof(T);
+    const int Ns = e->paths[1].len;
+    const PathInfo& fwd = e->paths[0];
+    for (size_t ci = 0; ci < e->segs.size(); ++ci) {
+      PSX_TRY(ckpt_chunk(e, ci));
+      const CkptSeg& c = e->segs[ci];
+      for (int t = c.t0; t < c.t1; ++t) {
+        const T* f = (const T*)((const char*)s.fch[ci % 2] + (size_t)(t - c.t0) * sb);
+        const T* gg = (const T*)((const char*)s.seg + (size_t)(Ns - 1 - t - c.u0) * sb);
+        PSX_TRY(stress_step<T>(e, f, gg, e->blocks[fwd.step_block[t]]));
+      }
+    }
+    return PSX_OK;
+  }
   for (const BlockInfo& blk : e->blocks) {
     for (int t = 0; t < blk.steps; ++t) {
       const T* f = (const T*)slice_ptr(e, e->prec, blk.path[0], blk.off[0] + t);
       const T* gg = (const T*)slice_ptr(e, e->prec, blk.path[1], blk.off[1] + blk.steps - 1 - t);
-      const T* w = (const T*)s.wf[blk.species];
-      hipLaunchKernelGGL(k_stress_pack<T>, dim3(g), dim3(kThreads), 0, e->stream, f, gg, w, e->N,
-                         (double2*)e->zbuf);
-      for (auto& a : e->caxes) {
-        CplxIO io{};
-        io.in = e->zbuf; io.out = e->zbuf; io.kmul = nullptr;
-        io.in_real = 0; io.out_real = 0; io.do_fwd = 1; io.do_inv = 0;
-        PSX_TRY(launch_cplx<double>(a, PSX_FP64, io, e->stream));
-      }
-      StressArgs sa;
-      sa.z = (const double2*)e->zbuf;
-      sa.nx = e->dims[0]; sa.ny = e->dims[1]; sa.nz = e->dims[2];
-      for (int d = 0; d < 3; ++d) { sa.kf[d] = e->kf1[blk.bclass][d]; sa.kd2[d] = e->kd2[d]; }
-      sa.part = e->part;
-      hipLaunchKernelGGL(k_stress_reduce, dim3(e->nb), dim3(kThreads), 0, e->stream, sa);
-      hipLaunchKernelGGL(k_stress_accum, dim3(1), dim3(kThreads), 0, e->stream, (const double*)e->part,
-                         e->nb, e->ds * blk.b * blk.b, e->sacc);
-      PSX_HIP(hipGetLastError());
+      PSX_TRY(stress_step<T>(e, f, gg, blk));
     }
   }
   return PSX_OK;
@@ -1669,6 +1927,7 @@ extern "C" psx_status psx_scft_param_grad(psx_scft* e, double* dFdchiN, double* 
   if (!e->swept) return fail(PSX_ERR_STATE, "propagate before the parameter gradient");
   if (dFdfA && !e->diblock)
     return fail(PSX_ERR_INVALID, "dF/dfA is defined for the AB diblock only");
+  if (e->ckpt_K > 0) return fail(PSX_ERR_STATE, "the junction slices are not stored in checkpointed mode");
   double Q;
   PSX_TRY(psx_scft_Q(e, &Q));
   // junction of the diblock: q of B's head and q+ arriving there
@@ -1713,6 +1972,7 @@ static size_t representative_launch(const PathInfo& pi, int a) {
 
 extern "C" psx_status psx_scft_time_step(psx_scft* e, int reps, double* ms_step, double* ms_pass) {
   if (!e || reps < 1) return fail(PSX_ERR_INVALID, "bad argument");
+  if (e->ckpt_K > 0) return fail(PSX_ERR_STATE, "timing runs on the stored-slice sweep (checkpointing off)");
   if (!e->sw[e->prec].wfac_valid) PSX_TRY(compute_wfac(e));
   hipEvent_t a, b;
   PSX_HIP(hipEventCreate(&a));
@@ -1802,6 +2062,7 @@ struct BatchClass {
 
 extern "C" psx_status psx_scft_autotune(psx_scft* e, int reps) {
   if (!e || reps < 1) return fail(PSX_ERR_INVALID, "bad argument");
+  if (e->ckpt_K > 0) return fail(PSX_ERR_STATE, "autotune runs on the stored-slice sweep (checkpointing off)");
   if (!e->sw[e->prec].wfac_valid) PSX_TRY(compute_wfac(e));
   const int na = (int)e->axes.size();
   // tuned on round 0, all of its paths in one launch (each candidate sees
@@ -1877,6 +2138,7 @@ extern "C" psx_status psx_batch_create(int n, const psx_grid* grids, const psx_d
     psx_scft* e = nullptr;
     st = psx_scft_create(&grids[i], &chains[i], prec, b->stream, &e);
     if (st != PSX_OK) break;
+    e->in_batch = true;
     b->eng.push_back(e);
     if (i == 0) b->Ns = e->Ns;
     else if (e->Ns != b->Ns)

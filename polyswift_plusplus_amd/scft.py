@@ -231,6 +231,12 @@ class DiblockSCFT:
         m = {"reference": 0, "simpson": 0, "trapezoid": 1, "consistent": 1}.get(mode, mode)
         check(self._L.psx_scft_set_quadrature(self._h, int(m)))
 
+    def setCheckpointing(self, every: int) -> None:
+        """Keep only every `every`-th backward slice (0: store all slices);
+        densities, iterations and the stress recompute the rest chunk by
+        chunk.  Linear chains only."""
+        check(self._L.psx_scft_set_checkpointing(self._h, int(every)))
+
     def setCell(self, cell) -> None:
         """New cell sizes (box / grid); fields are kept."""
         self.cell = tuple(float(c) for c in cell)
