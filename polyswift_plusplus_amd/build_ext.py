@@ -49,6 +49,7 @@ def build(verbose: bool = False) -> str:
         print(errs, file=sys.stderr)
     os.replace(tmp, LIB)
     build_demo(verbose)
+    build_h5(verbose)
     return LIB
 
 
@@ -66,6 +67,36 @@ def build_demo(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"g++ failed ({r.returncode}):\n{r.stderr}")
     return DEMO
+
+
+H5LIB = os.path.join(HERE, "libpsx_h5.so")
+
+
+def hdf5_root():
+    """Install prefix of an HDF5 1.10+ C library (include/hdf5.h, lib/libhdf5.so)."""
+    for cand in (os.environ.get("PSX_HDF5_ROOT"), "/opt/conda", "/usr"):
+        if cand and os.path.exists(os.path.join(cand, "include", "hdf5.h")) and \
+                os.path.exists(os.path.join(cand, "lib", "libhdf5.so")):
+            return cand
+    return None
+
+
+def build_h5(verbose: bool = False):
+    """libpsx_h5.so: PhysField dump/restore files (host C++ over HDF5).
+    Skipped, with a message, when no HDF5 library is installed."""
+    root = hdf5_root()
+    if root is None:
+        print("libpsx_h5.so not built: no HDF5 library found (set PSX_HDF5_ROOT)", file=sys.stderr)
+        return None
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I", os.path.join(root, "include"),
+           os.path.join(CSRC, "psx_h5.cpp"), "-L", os.path.join(root, "lib"), "-lhdf5",
+           "-Wl,-rpath," + os.path.join(root, "lib"), "-o", H5LIB]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed ({r.returncode}):\n{r.stderr}")
+    return H5LIB
 
 
 if __name__ == "__main__":

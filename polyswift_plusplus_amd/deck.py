@@ -374,6 +374,8 @@ class MeltConfig:
     seed: int | None
     species_fields: tuple   # (A field name, B field name)
     history_period: int | None
+    constraint_field: str = "defaultPressure"
+    dump_period: int | None = None
 
 
 _SUPPORTED_TOP = {"Grid", "Decomp", "Comm", "FFT", "PhysField", "EffHamil", "Polymer", "History",
@@ -472,10 +474,13 @@ def configure(root: Node) -> MeltConfig:
     hist = [hh for hh in root.all("History") if hh.get("kind") == "freeEnergy"]
     period = int(hist[0].get("updatePeriodicity", 1)) if hist else None
     seed = root.get("randomSeed")
+    cons = [pf.name for pf in root.all("PhysField") if pf.get("kind") == "constraint"]
+    dump = root.get("dumpPeriodicity")
     return MeltConfig(shape=shape, cell=cell, blocks=blocks, block_names=names, ds=ds_vals.pop(),
                       chiN=chiN, lambdas=lam, noise=noise, nsteps=int(root.get("nsteps", 1)),
                       seed=None if seed is None else int(seed), species_fields=tuple(fields),
-                      history_period=period)
+                      history_period=period, constraint_field=cons[0] if cons else "defaultPressure",
+                      dump_period=None if dump is None else int(dump))
 
 
 def build(cfg: MeltConfig, prec="fp64", stream=None):
@@ -486,21 +491,52 @@ def build(cfg: MeltConfig, prec="fp64", stream=None):
 
 
 def run(deck, iters: int | None = None, prec="fp64", sigma: float = 0.1, fields_seed=None,
-        use_graph: bool = True, tol: float = 0.0):
+        use_graph: bool = True, tol: float = 0.0, prefix: str | None = None,
+        restore_step: int | None = None, search_path=()):
     """Run a deck: its steepest-descent relaxation for ``nsteps`` (or
-    ``iters``) iterations from seeded N(0, sigma) fields (Philox keyed by the
-    deck's randomSeed, or fields_seed).  The reference's noise term is not
-    applied (noise-free parity runs); the deck's value is returned beside the
-    histories."""
+    ``iters``) iterations, from seeded N(0, sigma) fields (Philox keyed by the
+    deck's randomSeed, or fields_seed) or, with ``restore_step``, from the
+    PhysField files ``<prefix>_<field>_<1000000+step>.h5`` of that step (the
+    reference's ``-r`` restart).  With ``prefix`` the PhysFields are dumped
+    every ``dumpPeriodicity`` steps, after the update, as the reference's main
+    loop does (polyswift/polyswift.cxx:544-588).  The reference's noise term
+    is not applied (noise-free parity runs); the deck's value is returned
+    beside the histories."""
     from . import fields as F
+    from . import physfield
     from .scft import Mixing
-    root = load(deck) if isinstance(deck, str) else deck
+    root = load(deck, search_path) if isinstance(deck, str) else deck
     cfg = configure(root)
     e = build(cfg, prec)
-    seed = fields_seed if fields_seed is not None else (cfg.seed or 0)
-    wA, wB = F.seeded_fields(cfg.shape, cfg.cell, sigma=sigma, seed=seed)
-    e.setFields(wA, wB)
-    h = e.iterate(int(iters or cfg.nsteps), Mixing("steepest", *cfg.lambdas), tol=tol,
-                  use_graph=use_graph)
-    h.update(config=cfg, engine=e, noise_ignored=cfg.noise)
-    return h
+    names = (*cfg.species_fields, cfg.constraint_field)
+    step0 = 0
+    if restore_step is not None:
+        if prefix is None:
+            raise DeckError("restore_step needs the dump prefix")
+        physfield.restore(e, prefix, restore_step, names)
+        step0 = int(restore_step)
+    else:
+        seed = fields_seed if fields_seed is not None else (cfg.seed or 0)
+        wA, wB = F.seeded_fields(cfg.shape, cfg.cell, sigma=sigma, seed=seed)
+        e.setFields(wA, wB)
+    n = int(iters or cfg.nsteps)
+    mix = Mixing("steepest", *cfg.lambdas)
+    period = cfg.dump_period if prefix else None
+    hist = {"F": [], "Q": [], "err": []}
+    t, done = step0, 0
+    while done < n:
+        chunk = n - done
+        if period:
+            chunk = min(chunk, period - t % period)
+        h = e.iterate(chunk, mix, tol=tol, use_graph=use_graph)
+        for k in hist:
+            hist[k].extend(h[k])
+        t += h["iters"]
+        done += h["iters"]
+        if period and t % period == 0:
+            physfield.dump(e, prefix, t, names)
+        if h["iters"] < chunk:     # converged
+            break
+    out = {k: np.asarray(v) for k, v in hist.items()}
+    out.update(iters=done, step=t, config=cfg, engine=e, noise_ignored=cfg.noise)
+    return out
