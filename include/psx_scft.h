@@ -258,6 +258,12 @@ psx_status psx_scft_time_step(psx_scft* e, int reps, double* h_ms_step,
  * Stockham) on this engine's data and keep the fastest.  No reference
  * counterpart. */
 psx_status psx_scft_autotune(psx_scft* e, int reps);
+/* Choose the line kernel of pass `pass` (grid axes with n > 1 from the
+ * last, z, to the first, x) for the current precision: -1 the default for
+ * the line length, 1 LDS Stockham, 2 and 3 the alternative register splits
+ * (n = 64: 16x4; n = 128: 32x4 and 16x8; n = 32: -, 4x8; other lengths use
+ * the default).  No reference counterpart. */
+psx_status psx_scft_set_variant(psx_scft* e, int pass, int variant);
 
 /* ===================================================================== */
 /* Batched parameter sweeps: many independent diblock instances (grid,     */

@@ -136,6 +136,97 @@ __device__ __forceinline__ void reg_roundtrip(C* v, C* L, const C* __restrict__ 
   dft_reg<N1, +1, T>(v, tw, N2);
 }
 
+// Exchange with the partner lane (lane ^ 1) through a DPP quad permutation
+// [1, 0, 3, 2] (VALU moves, no LDS).
+__device__ __forceinline__ float pair_swap(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double pair_swap(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), 0xB1, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0xB1, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Lane-pair round trip for n = 2 N1^2 on N2 = 2 N1 lanes, N1 points per lane
+// (lane t owns x[t + N2 r]): half the registers per lane and twice the lines
+// in flight of the N1 = N2^2 / n split.  Forward, with k = k1 + N1 k2g:
+//   in-register DFT_N1 over r, twiddle W_n^{t k1};
+//   LDS transpose: lane l = 2 k1 + h takes the points t = h + 2m of column k1
+//   (XOR swizzle 2 k1 on the column index keeps writes and reads on distinct banks);
+//   in-register DFT_N1 over m: E_h[k2] = sum_m W_N1^{m k2} z[h + 2m];
+//   radix 2 across the pair: lane h keeps X[k1 + N1 (k2 + N1 h)] =
+//   E_0[k2] + (-1)^h W_N2^{k2} E_1[k2], k2 < N1.
+// The spectral multiply follows, then the same steps inverted.
+template <typename T, int N1, typename C>
+__device__ __forceinline__ void pair_roundtrip(C* v, C* L, const C* __restrict__ tw,
+                                               const C* twl, const T* emul, int t) {
+  constexpr int N2 = 2 * N1;
+  dft_reg<N1, -1, T>(v, tw, N2);
+#pragma unroll
+  for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], twiddle<-1>(twl, t * k1));
+  __syncthreads();
+#pragma unroll
+  for (int k1 = 0; k1 < N1; ++k1) L[k1 * N2 + (t ^ ((2 * k1) % N2))] = v[k1];
+  __syncthreads();
+  const int k1 = t >> 1, h = t & 1;
+  const int sw = (2 * k1) % N2;
+#pragma unroll
+  for (int m = 0; m < N1; ++m) v[m] = L[k1 * N2 + ((h + 2 * m) ^ sw)];
+  dft_reg<N1, -1, T>(v, tw, N2);
+  // radix 2 across the lane pair
+#pragma unroll
+  for (int k2 = 0; k2 < N1; ++k2) {
+    C o;
+    o.x = pair_swap(v[k2].x);
+    o.y = pair_swap(v[k2].y);
+    const C e0 = h ? o : v[k2];
+    const C e1 = h ? v[k2] : o;
+    const C w1 = cmul(e1, twiddle<-1>(tw, k2 * N1));
+    C x;
+    if (h) { x.x = e0.x - w1.x; x.y = e0.y - w1.y; }
+    else { x.x = e0.x + w1.x; x.y = e0.y + w1.y; }
+    const T m = emul[k1 + N1 * (k2 + N1 * h)];
+    x.x *= m; x.y *= m;
+    v[k2] = x;
+  }
+  // inverse radix 2: E_0 = Y[k2] + Y[k2 + N1], E_1 = (Y[k2] - Y[k2 + N1]) W_N2^{-k2}
+#pragma unroll
+  for (int k2 = 0; k2 < N1; ++k2) {
+    C o;
+    o.x = pair_swap(v[k2].x);
+    o.y = pair_swap(v[k2].y);
+    const C lo = h ? o : v[k2];
+    const C hi = h ? v[k2] : o;
+    C e;
+    if (h) {
+      C d; d.x = lo.x - hi.x; d.y = lo.y - hi.y;
+      e = cmul(d, twiddle<+1>(tw, k2 * N1));
+    } else {
+      e.x = lo.x + hi.x; e.y = lo.y + hi.y;
+    }
+    v[k2] = e;
+  }
+  dft_reg<N1, +1, T>(v, tw, N2);
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < N1; ++m) L[k1 * N2 + ((h + 2 * m) ^ sw)] = v[m];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < N1; ++j) v[j] = L[j * N2 + (t ^ ((2 * j) % N2))];
+#pragma unroll
+  for (int j = 1; j < N1; ++j) v[j] = cmul(v[j], twiddle<+1>(twl, t * j));
+  dft_reg<N1, +1, T>(v, tw, N2);
+}
+
+// Round trip on the lanes of the split (N1, N2): lane-pair form when
+// N2 = 2 N1, else the N1 % N2 == 0 form.
+template <typename T, int N1, int N2, typename C>
+__device__ __forceinline__ void line_roundtrip(C* v, C* L, const C* __restrict__ tw,
+                                               const C* twl, const T* emul, int t) {
+  if constexpr (N2 == 2 * N1) pair_roundtrip<T, N1>(v, L, tw, twl, emul, t);
+  else reg_roundtrip<T, N1, N2>(v, L, tw, twl, emul, t);
+}
+
 // Round trip(s) along one axis for 2*64/N2 real lines per wave:
 //   first:  u = post * IFFT(emul * FFT(pre * in))
 //   fused:  mid = u;  out = post2 * IFFT(emul2 * FFT(pre2 * u))   (emul2 != null)
@@ -153,7 +244,7 @@ k_line_reg(LineGeom g, const PairIO* __restrict__ fio) {
   constexpr int n = N1 * N2;
   constexpr int CL = 64 / N2;     // complex lines per wave
   constexpr int RL = 2 * CL;      // real lines per wave
-  static_assert(N1 % N2 == 0 && 64 % N2 == 0, "bad split");
+  static_assert((N1 % N2 == 0 || N2 == 2 * N1) && 64 % N2 == 0, "bad split");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C* lds = (C*)smem_raw;
   int tile, field;
@@ -258,7 +349,7 @@ k_line_reg(LineGeom g, const PairIO* __restrict__ fio) {
     }
   }
 
-  reg_roundtrip<T, N1, N2>(v, L, tw, twl, e1, t);
+  line_roundtrip<T, N1, N2>(v, L, tw, twl, e1, t);
 
   // ---- store: the step result (post) to mid and on through the second
   // round trip, or straight to out
@@ -330,7 +421,7 @@ k_line_reg(LineGeom g, const PairIO* __restrict__ fio) {
       }
     }
     if (last) break;
-    reg_roundtrip<T, N1, N2>(v, L, tw, twl, e2, t);
+    line_roundtrip<T, N1, N2>(v, L, tw, twl, e2, t);
   }
 }
 

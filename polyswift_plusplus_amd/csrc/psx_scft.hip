@@ -242,6 +242,12 @@ void launch_stockham(const Axis& a, int prec, const PairIO* fio, int nfields,
 // PSX_FORCE_STOCKHAM=1 in the environment selects the Stockham kernel for
 // every length (used to cross-check the two kernel families).
 int g_force_stockham = -1;
+// PSX_LINE_VARIANT=<v> makes variant v the default for every pass (kernel
+// cross-checks and timing; an engine's tuned choice still wins).
+int g_default_variant = [] {
+  const char* v = getenv("PSX_LINE_VARIANT");
+  return v ? atoi(v) : 0;
+}();
 
 template <typename T>
 psx_status launch_pair(const Axis& a, int prec, const PairIO* fio,
@@ -250,12 +256,17 @@ psx_status launch_pair(const Axis& a, int prec, const PairIO* fio,
     const char* v = getenv("PSX_FORCE_STOCKHAM");
     g_force_stockham = (v && v[0] == '1') ? 1 : 0;
   }
+  if (variant < 0 && g_default_variant != 0) variant = g_default_variant;
   if (g_force_stockham == 1 || variant == 1) {
     launch_stockham<T>(a, prec, fio, nfields, s);
   } else if (variant == 2 && a.n == 64) {
     launch_reg<T, 16, 4>(a, prec, fio, nfields, s);
   } else if (variant == 2 && a.n == 128) {
     launch_reg<T, 32, 4>(a, prec, fio, nfields, s);
+  } else if (variant == 3 && a.n == 128) {
+    launch_reg<T, 16, 8>(a, prec, fio, nfields, s);
+  } else if (variant == 3 && a.n == 32) {
+    launch_reg<T, 4, 8>(a, prec, fio, nfields, s);
   } else {
     switch (a.n) {
       case 1: launch_lane<T, 1>(a, prec, fio, nfields, s); break;
@@ -272,7 +283,9 @@ psx_status launch_pair(const Axis& a, int prec, const PairIO* fio,
       case 48: launch_reg<T, 12, 4>(a, prec, fio, nfields, s); break;
       case 64: launch_reg<T, 8, 8>(a, prec, fio, nfields, s); break;
       case 96: launch_reg<T, 24, 4>(a, prec, fio, nfields, s); break;
-      case 128: launch_reg<T, 16, 8>(a, prec, fio, nfields, s); break;
+      // lane pairs at 128: 8 points per lane, 3 (FP64) / 5 (FP32) waves per
+      // SIMD against 2 for 16 points per lane
+      case 128: launch_reg<T, 8, 16>(a, prec, fio, nfields, s); break;
       case 256: launch_reg<T, 16, 16>(a, prec, fio, nfields, s); break;
       default: launch_stockham<T>(a, prec, fio, nfields, s); break;
     }
@@ -2018,11 +2031,14 @@ extern "C" psx_status psx_scft_time_step(psx_scft* e, int reps, double* ms_step,
 // ===========================================================================
 namespace {
 
-// Candidate line-kernel variants for length n: -1 default split, 1 LDS
-// Stockham, 2 alternative register split (n = 64: 16x4, n = 128: 32x4).
+// Candidate line-kernel variants for length n: -1 default split (n = 128:
+// lane pairs 8x16), 1 LDS Stockham, 2 alternative register split (n = 64:
+// 16x4, n = 128: 32x4), 3 second alternative (n = 32: lane pairs 4x8,
+// n = 128: 16x8).
 std::vector<int> variant_candidates(int n) {
   std::vector<int> v{-1, 1};
   if (n == 64 || n == 128) v.push_back(2);
+  if (n == 32 || n == 128) v.push_back(3);
   return v;
 }
 
@@ -2059,6 +2075,17 @@ struct BatchClass {
 };
 
 }  // namespace
+
+extern "C" psx_status psx_scft_set_variant(psx_scft* e, int pass, int variant) {
+  if (!e) return fail(PSX_ERR_INVALID, "null engine");
+  if (pass < 0 || pass >= (int)e->axes.size()) return fail(PSX_ERR_INVALID, "pass index out of range");
+  if (variant != -1 && variant != 1 && variant != 2 && variant != 3)
+    return fail(PSX_ERR_INVALID, "line-kernel variant must be -1, 1, 2 or 3");
+  e->variant[e->prec][pass] = variant;
+  drop_graphs(e);
+  e->swept = false;
+  return PSX_OK;
+}
 
 extern "C" psx_status psx_scft_autotune(psx_scft* e, int reps) {
   if (!e || reps < 1) return fail(PSX_ERR_INVALID, "bad argument");

@@ -256,6 +256,13 @@ class DiblockSCFT:
                                          ctypes.cast(passes, _dp)))
         return step.value, list(passes)
 
+    def setVariant(self, variant: int, passes=None) -> None:
+        """Line kernel of the given passes (all by default): -1 default,
+        1 LDS Stockham, 2 / 3 alternative register splits."""
+        na = sum(1 for n in self.shape if n > 1) or 1
+        for p in (range(na) if passes is None else passes):
+            check(self._L.psx_scft_set_variant(self._h, int(p), int(variant)))
+
     def autotune(self, reps: int = 10) -> None:
         """Pick the fastest line kernel per pass on this engine's data."""
         check(self._L.psx_scft_autotune(self._h, int(reps)))

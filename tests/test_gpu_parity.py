@@ -85,6 +85,28 @@ def test_propagator_slices(shape, fA, ds):
     assert rel(gA, phiA) < RTOL64 and rel(gB, phiB) < RTOL64
 
 
+@pytest.mark.parametrize("variant", [-1, 1, 2, 3])
+@pytest.mark.parametrize("shape", [(8, 12, 128), (8, 128, 12), (128, 6, 10), (128, 3, 5),
+                                   (6, 128, 3), (32, 16, 64), (64, 32, 12)])
+def test_line_kernel_variants(shape, variant):
+    """Every line-kernel variant on every axis role (contiguous, strided with
+    paired loads, strided without) at n = 32, 64, 128, against the oracle."""
+    print(f"variant {variant} {shape}", flush=True)
+    cell = (0.3, 0.25, 0.35)
+    P, wA, wB = _case(shape, cell, 0.3, 0.05)
+    phiA, phiB, Q, (qA, qB, qtB, qtA) = O.diblock_densities(P, wA, wB, want_slices=True)
+    e = DiblockSCFT(shape, cell, 0.3, 0.05, P.chiN)
+    e.setVariant(variant)
+    e.setFields(wA, wB)
+    e.solveQ()
+    for t in (1, e.sA, e.Ns):
+        ref = qA[t] if t <= e.sA else qB[t - e.sA]
+        assert rel(e.slice(0, t), ref) < 1e-12, t
+    assert abs(e.calcBigQ() / Q - 1) < 1e-12
+    gA, gB = e.densities()
+    assert rel(gA, phiA) < RTOL64 and rel(gB, phiB) < RTOL64
+
+
 def test_steepest_descent_trajectory():
     P, wA, wB = _case()
     P.lambda0, P.lambda1 = 0.1, 0.05
