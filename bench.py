@@ -168,6 +168,8 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default line kernels (no per-pass autotuning)")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3 counters: the dual contour step of "
                          "both precisions (5 reps each) and one FP64 copy of known size")
@@ -203,13 +205,15 @@ def main():
     legs = {}
     for prec in ("fp64", "fp32"):
         e = make_engine(prec, w)
-        print(f"[bench] {prec}: warmup {args.warmup}, timing {args.steps} iterations", file=sys.stderr, flush=True)
+        choice = None if args.no_tune else e.autotune(5)
+        print(f"[bench] {prec}: line kernels {choice}, warmup {args.warmup}, timing {args.steps} "
+              "iterations", file=sys.stderr, flush=True)
         dt, h = time_iters(e, mix, args.warmup, args.steps)
         step_ms, pass_ms = e.timeStep(10)
         Ns = e.Ns
         legs[prec] = dict(it_s=args.steps / dt, ms_per_iter=1e3 * dt / args.steps,
                           step_ms=step_ms, pass_ms=pass_ms, F_last=float(h["F"][-1]),
-                          err_last=float(h["err"][-1]))
+                          err_last=float(h["err"][-1]), line_kernels=choice)
         e.close()
         print(f"[bench] {prec}: {legs[prec]['it_s']:.2f} it/s, step {step_ms * 1e3:.1f} us "
               f"(passes {[round(p * 1e3, 1) for p in pass_ms]} us)", file=sys.stderr, flush=True)
@@ -267,9 +271,11 @@ def main():
                      "traffic": (traffic or {}).get("fp32") if args.grid == 128 else None,
                      "traffic_source": "profiles/k_line_reg_128cubed_hbm_counters.md (bytes per dual step)",
                      "step_us": m["step_ms"] * 1e3, "pass_us": [p * 1e3 for p in m["pass_ms"]],
+                     "line_kernels": m["line_kernels"],
                      "fp64": {"achieved": ach_d, "frac": ach_d / HBM_PEAK_GBS,
                               "traffic": (traffic or {}).get("fp64") if args.grid == 128 else None,
-                              "step_us": d["step_ms"] * 1e3, "pass_us": [p * 1e3 for p in d["pass_ms"]]}},
+                              "step_us": d["step_ms"] * 1e3, "pass_us": [p * 1e3 for p in d["pass_ms"]],
+                              "line_kernels": d["line_kernels"]}},
         "accuracy": acc,
         "batch": batch,
         "cpu_baseline": cpu,

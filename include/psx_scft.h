@@ -255,9 +255,10 @@ psx_status psx_scft_time_step(psx_scft* e, int reps, double* h_ms_step,
                               double* h_ms_pass3);
 
 /* Time the candidate line kernels of each pass (register splits, LDS
- * Stockham) on this engine's data and keep the fastest.  No reference
- * counterpart. */
-psx_status psx_scft_autotune(psx_scft* e, int reps);
+ * Stockham) on this engine's data and keep the fastest; h_choice (3 ints,
+ * may be NULL) receives the chosen variant per pass (see
+ * psx_scft_set_variant).  No reference counterpart. */
+psx_status psx_scft_autotune(psx_scft* e, int reps, int* h_choice);
 /* Choose the line kernel of pass `pass` (grid axes with n > 1 from the
  * last, z, to the first, x) for the current precision: -1 the default for
  * the line length, 1 LDS Stockham, 2 and 3 the alternative register splits

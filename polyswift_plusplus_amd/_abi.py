@@ -89,7 +89,7 @@ EXPORTS = {
     "psx_scft_set_checkpointing": (ctypes.c_int, [_vp, ctypes.c_int]),
     "psx_scft_set_cell": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     "psx_scft_param_grad": (ctypes.c_int, [_vp, _dp, _dp]),
-    "psx_scft_autotune": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "psx_scft_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
     "psx_scft_set_variant": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "psx_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Grid), ctypes.POINTER(Diblock),
                                         ctypes.c_int, _vp, ctypes.POINTER(_vp)]),

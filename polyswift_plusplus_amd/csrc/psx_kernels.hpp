@@ -410,72 +410,115 @@ struct QuadArgs {
   int acc_in;                    // add the raw sums of earlier chunks held in phiA/phiB
 };
 
+// V consecutive cells of T in one aligned load (16 bytes for V = 16 / sizeof(T)).
+template <typename T, int V>
+struct alignas(sizeof(T) * V) CellVec { T v[V]; };
+
+// Everything after the densities of one cell: the Flory dH, then the
+// steepest-descent update (w, p_raw, w-factors) or the Anderson ring entry,
+// and the partial sums.
 template <typename T>
+__device__ __forceinline__ void quad_tail(const QuadArgs& a, long long r, double phA, double phB,
+                                          double* acc, long long slot, double pbar) {
+  const long long N = a.N;
+  const double wA = a.wA[r], wB = a.wB[r];
+  if (a.mode == QM_FE) {
+    acc[1] += -(phA * wA) - (phB * wB);
+    acc[2] += a.chiN * phA * phB;
+    return;
+  }
+  const double p = a.praw[r] - pbar;
+  const double dHA = p - wA + a.chiN * (phB - a.fB);
+  const double dHB = p - wB + a.chiN * (phA - a.fA);
+  acc[3] += dHA * dHA + dHB * dHB;
+  if (a.mode == QM_RESID) {
+    double* fx = a.ringx + slot * 2 * N;
+    double* ff = a.ringf + slot * 2 * N;
+    fx[r] = wA; fx[N + r] = wB;
+    ff[r] = dHA; ff[N + r] = dHB;
+    return;
+  }
+  const double nA = wA + (dHA + (-a.lam1) * dHB) * a.lam0;
+  const double nB = wB + (dHB + (-a.lam1) * dHA) * a.lam0;
+  const double pr = (nA + nB) * 0.5;
+  a.wA[r] = nA; a.wB[r] = nB; a.praw[r] = pr;
+  ((T*)a.wfA)[r] = (T)exp(-a.halfds * nA);
+  ((T*)a.wfB)[r] = (T)exp(-a.halfds * nB);
+  acc[0] += pr;
+  if (a.f_order == 0) acc[1] += -(phA * nA) - (phB * nB);
+  else acc[1] += -(phA * wA) - (phB * wB);
+  acc[2] += a.chiN * phA * phB;
+}
+
+// Contour quadrature, densities and the field update in one pass over the
+// cells, V consecutive cells per thread (slices read as aligned V-vectors;
+// N % V == 0).  Every cell's contour sum runs in the same order for any V.
+template <typename T, int V>
 __global__ void __launch_bounds__(kThreads)
 k_quad_update(QuadArgs a) {
-  const long long N = a.N;
+  typedef CellVec<T, V> VT;
+  const long long N = a.N, NV = N / V;
   const double* __restrict__ cw = a.cw;
   const double invQ = 1.0 / a.st->Q;
   const double pbar = a.st->pbar;
   const long long slot = (a.mode == QM_RESID) ? (a.st->iter % a.m1) : 0;
+  const bool slices = !(a.mode == QM_SD_PHI || a.mode == QM_FE);
   double acc[kNPart] = {0.0, 0.0, 0.0, 0.0};
-  for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < N;
-       r += (long long)gridDim.x * blockDim.x) {
-    double phA, phB;
-    if (a.mode == QM_SD_PHI || a.mode == QM_FE) {
-      phA = a.phiA[r];
-      phB = a.phiB[r];
+  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < NV;
+       g += (long long)gridDim.x * blockDim.x) {
+    const long long r0 = g * V;
+    double phA[V], phB[V];
+    if (!slices) {
+#pragma unroll
+      for (int v = 0; v < V; ++v) { phA[v] = a.phiA[r0 + v]; phB[v] = a.phiB[r0 + v]; }
     } else {
       // per block: sum_s c_s q_head[s] q_tail[M-1-s] (pspolymer/PsBlock.cpp:256-380),
       // added into the block's species (pspolymer/PsBlock.cpp:154-196)
-      double qa = 0.0, qb = 0.0;
-      if (a.acc_in) { qa = a.phiA[r]; qb = a.phiB[r]; }
+      double qa[V], qb[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        qa[v] = a.acc_in ? a.phiA[r0 + v] : 0.0;
+        qb[v] = a.acc_in ? a.phiB[r0 + v] : 0.0;
+      }
       for (int b = 0; b < a.nblk; ++b) {
-        const T* __restrict__ H = (const T*)a.blk[b].H + r;
-        const T* __restrict__ Tq = (const T*)a.blk[b].T + r;
+        const VT* __restrict__ H = (const VT*)a.blk[b].H + g;
+        const VT* __restrict__ Tq = (const VT*)a.blk[b].T + g;
         const double* __restrict__ c = cw + a.blk[b].woff;
         const int M = a.blk[b].M;
-        double acc = 0.0;
-#pragma unroll 8
-        for (int t = 0; t < M; ++t)
-          acc += c[t] * ((double)H[(long long)t * N] * (double)Tq[(long long)(M - 1 - t) * N]);
-        if (a.blk[b].species == 0) qa += acc;
-        else qb += acc;
+        double s[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) s[v] = 0.0;
+        constexpr int UNR = V == 1 ? 8 : 4;
+#pragma unroll UNR
+        for (int t = 0; t < M; ++t) {
+          const VT h = H[(long long)t * NV];
+          const VT q = Tq[(long long)(M - 1 - t) * NV];
+          const double ct = c[t];
+#pragma unroll
+          for (int v = 0; v < V; ++v) s[v] += ct * ((double)h.v[v] * (double)q.v[v]);
+        }
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          if (a.blk[b].species == 0) qa[v] += s[v];
+          else qb[v] += s[v];
+        }
       }
-      if (a.mode == QM_ACC) { a.phiA[r] = qa; a.phiB[r] = qb; continue; }
-      phA = qa * invQ * a.dens;
-      phB = qb * invQ * a.dens;
-      a.phiA[r] = phA;
-      a.phiB[r] = phB;
+      if (a.mode == QM_ACC) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) { a.phiA[r0 + v] = qa[v]; a.phiB[r0 + v] = qb[v]; }
+        continue;
+      }
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        phA[v] = qa[v] * invQ * a.dens;
+        phB[v] = qb[v] * invQ * a.dens;
+        a.phiA[r0 + v] = phA[v];
+        a.phiB[r0 + v] = phB[v];
+      }
     }
     if (a.mode == QM_DENS) continue;
-    const double wA = a.wA[r], wB = a.wB[r];
-    if (a.mode == QM_FE) {
-      acc[1] += -(phA * wA) - (phB * wB);
-      acc[2] += a.chiN * phA * phB;
-      continue;
-    }
-    const double p = a.praw[r] - pbar;
-    const double dHA = p - wA + a.chiN * (phB - a.fB);
-    const double dHB = p - wB + a.chiN * (phA - a.fA);
-    acc[3] += dHA * dHA + dHB * dHB;
-    if (a.mode == QM_RESID) {
-      double* fx = a.ringx + slot * 2 * N;
-      double* ff = a.ringf + slot * 2 * N;
-      fx[r] = wA; fx[N + r] = wB;
-      ff[r] = dHA; ff[N + r] = dHB;
-      continue;
-    }
-    const double nA = wA + (dHA + (-a.lam1) * dHB) * a.lam0;
-    const double nB = wB + (dHB + (-a.lam1) * dHA) * a.lam0;
-    const double pr = (nA + nB) * 0.5;
-    a.wA[r] = nA; a.wB[r] = nB; a.praw[r] = pr;
-    ((T*)a.wfA)[r] = (T)exp(-a.halfds * nA);
-    ((T*)a.wfB)[r] = (T)exp(-a.halfds * nB);
-    acc[0] += pr;
-    if (a.f_order == 0) acc[1] += -(phA * nA) - (phB * nB);
-    else acc[1] += -(phA * wA) - (phB * wB);
-    acc[2] += a.chiN * phA * phB;
+#pragma unroll
+    for (int v = 0; v < V; ++v) quad_tail<T>(a, r0 + v, phA[v], phB[v], acc, slot, pbar);
   }
   block_sum<kNPart>(acc);
   if (threadIdx.x == 0) {

@@ -263,9 +263,13 @@ class DiblockSCFT:
         for p in (range(na) if passes is None else passes):
             check(self._L.psx_scft_set_variant(self._h, int(p), int(variant)))
 
-    def autotune(self, reps: int = 10) -> None:
-        """Pick the fastest line kernel per pass on this engine's data."""
-        check(self._L.psx_scft_autotune(self._h, int(reps)))
+    def autotune(self, reps: int = 10) -> list:
+        """Pick the fastest line kernel per pass on this engine's data;
+        returns the choice per pass (-1 default, 1 Stockham, 2 / 3 the
+        alternative register splits)."""
+        ch = (ctypes.c_int * 3)()
+        check(self._L.psx_scft_autotune(self._h, int(reps), ch))
+        return list(ch)
 
     @classmethod
     def _borrow(cls, handle, shape, cell, fA, ds, chiN):
