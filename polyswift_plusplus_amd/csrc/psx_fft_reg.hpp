@@ -425,6 +425,122 @@ k_line_reg(LineGeom g, const PairIO* __restrict__ fio) {
   }
 }
 
+// Strided axes through 128-byte rows: a block of W waves takes LT = W * RL
+// lines that sit next to each other in memory (one 128-byte segment per
+// point), loads the tile [n points][LT lines] with 16-byte chunks into LDS,
+// and every wave reads its RL lines in the lane layout of line_roundtrip.
+// The staging area doubles as the waves' transpose buffers (barriers keep the
+// two uses apart).  Stores go back through the same tile.  Same stages as
+// k_line_reg (pre, fused mid / pre2 / second round trip, post).
+template <typename T, int N1, int N2, int W>
+__global__ void __launch_bounds__(64 * W)
+k_line_tile(LineGeom g, const PairIO* __restrict__ fio) {
+  typedef typename Cx<T>::type C;
+  typedef typename Vec16<T>::type V;
+  constexpr int n = N1 * N2;
+  constexpr int RL = 2 * 64 / N2;                 // real lines per wave
+  constexpr int LT = W * RL;                      // real lines per tile
+  constexpr int LS = LT + 2;                      // LDS row stride (elements)
+  constexpr int VW = 16 / (int)sizeof(T);
+  constexpr int CPR = LT / VW;                    // 16-byte chunks per row
+  constexpr int NCH = n * CPR;
+  constexpr int SB = n * LS * (int)sizeof(T);
+  constexpr int TB = W * 64 * N1 * 2 * (int)sizeof(T);
+  constexpr int BUF = ((SB > TB ? SB : TB) + 15) / 16 * 16;
+  static_assert(LT % VW == 0, "tile rows must be whole 16-byte chunks");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* S = (T*)smem_raw;
+  int tile, field;
+  xcd_tile(g.ntiles, g.nfields, &tile, &field);
+  const PairIO fd = fio[field];
+  const C* __restrict__ tw = (const C*)g.tw;
+  const bool fused = fd.emul2 != nullptr;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int p = lane / N2, t = lane % N2;
+  const int inner = g.inner;
+  const int l0 = tile * LT;
+  const long long base = (long long)(l0 / inner) * n * inner + (l0 - (l0 / inner) * inner);
+  C* L = (C*)smem_raw + wave * (64 * N1) + p * n;
+  const int j0 = wave * RL + 2 * p;               // this lane's real-line pair in the tile
+
+  constexpr bool TLDS = sizeof(T) == 4;
+  const C* twl = tw;
+  const T* e1 = (const T*)fd.emul;
+  const T* e2 = (const T*)fd.emul2;
+  if (TLDS) {
+    C* tws = (C*)(smem_raw + BUF);
+    T* e1s = (T*)(tws + n);
+    T* e2s = e1s + n;
+    for (int i = tid; i < n; i += 64 * W) {
+      tws[i] = tw[i];
+      e1s[i] = e1[i];
+      if (fused) e2s[i] = e2[i];
+    }
+    twl = tws; e1 = e1s; e2 = e2s;
+  }
+
+  // ---- load the tile (times pre), then the lane layout
+  {
+    const T* src = (const T*)fd.in;
+    const T* pre = (const T*)fd.pre;
+#pragma unroll 4
+    for (int c = tid; c < NCH; c += 64 * W) {
+      const int row = c / CPR, col = (c - row * CPR) * VW;
+      const long long ga = base + (long long)row * inner + col;
+      V x = *(const V*)(src + ga);
+      if (pre) { const V w = *(const V*)(pre + ga); vmul(x, w); }
+      vstore_lds(S + row * LS + col, x);
+    }
+  }
+  __syncthreads();
+  C v[N1];
+#pragma unroll
+  for (int r = 0; r < N1; ++r) {
+    const int k = t + N2 * r;
+    v[r].x = S[k * LS + j0];
+    v[r].y = S[k * LS + j0 + 1];
+  }
+  __syncthreads();
+  line_roundtrip<T, N1, N2>(v, L, tw, twl, e1, t);
+
+  for (int stage = 0; stage < 2; ++stage) {
+    const bool last = stage == 1 || !fused;
+    T* __restrict__ dst = (T*)(last ? fd.out : fd.mid);
+    const T* __restrict__ post = (const T*)(stage == 0 ? fd.post : fd.post2);
+    const T* __restrict__ pre2 = last ? nullptr : (const T*)fd.pre2;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < N1; ++r) {
+      const int k = t + N2 * r;
+      S[k * LS + j0] = v[r].x;
+      S[k * LS + j0 + 1] = v[r].y;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int c = tid; c < NCH; c += 64 * W) {
+      const int row = c / CPR, col = (c - row * CPR) * VW;
+      const long long ga = base + (long long)row * inner + col;
+      V x = vload_lds<T>(S + row * LS + col);
+      if (post) { const V w = *(const V*)(post + ga); vmul(x, w); }
+      *(V*)(dst + ga) = x;
+      if (!last) {
+        if (pre2) { const V w = (pre2 == post) ? *(const V*)(post + ga) : *(const V*)(pre2 + ga); vmul(x, w); }
+        vstore_lds(S + row * LS + col, x);
+      }
+    }
+    if (last) break;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < N1; ++r) {
+      const int k = t + N2 * r;
+      v[r].x = S[k * LS + j0];
+      v[r].y = S[k * LS + j0 + 1];
+    }
+    __syncthreads();
+    line_roundtrip<T, N1, N2>(v, L, tw, twl, e2, t);
+  }
+}
+
 // Whole line in one lane (n <= 8): no transposes.  Same stages as k_line_reg.
 template <typename T, int NN, bool CONTIG>
 __global__ void __launch_bounds__(64)

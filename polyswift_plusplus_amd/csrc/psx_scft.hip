@@ -207,6 +207,28 @@ void launch_reg(const Axis& a, int prec, const PairIO* fio, int nfields,
     hipLaunchKernelGGL((k_line_reg<T, N1, N2, false, false>), grid, dim3(64), smem, s, g, fio);
 }
 
+// The tiled strided kernel applies at n = 64, 128 when whole 128-byte tile
+// rows (32 FP32 / 16 FP64 lines) fit the line stride.
+template <typename T>
+bool tile_fits(const Axis& a) {
+  const int lt = sizeof(T) == 4 ? 32 : 16;
+  return (a.n == 128 || a.n == 64) && !a.contig && a.inner % lt == 0;
+}
+
+// Strided axis through 128-byte tile rows (k_line_tile): W waves of RL lines.
+template <typename T, int N1, int N2, int W>
+void launch_tile(const Axis& a, int prec, const PairIO* fio, int nfields, hipStream_t s) {
+  constexpr int n = N1 * N2, RL = 2 * 64 / N2, LT = W * RL;
+  LineGeom g = geom(a, prec, LT);
+  g.ntiles = a.nlines / LT;
+  g.nfields = nfields;
+  const size_t sb = (size_t)n * (LT + 2) * sizeof(T);
+  const size_t tb = (size_t)W * 64 * N1 * 2 * sizeof(T);
+  const size_t buf = ((sb > tb ? sb : tb) + 15) / 16 * 16;
+  const size_t smem = buf + (sizeof(T) == 4 ? (size_t)n * 4 * sizeof(T) : 0);
+  hipLaunchKernelGGL((k_line_tile<T, N1, N2, W>), dim3(g.ntiles * nfields), dim3(64 * W), smem, s, g, fio);
+}
+
 template <typename T, int NN>
 void launch_lane(const Axis& a, int prec, const PairIO* fio, int nfields,
                  hipStream_t s) {
@@ -263,6 +285,18 @@ psx_status launch_pair(const Axis& a, int prec, const PairIO* fio,
     launch_reg<T, 16, 4>(a, prec, fio, nfields, s);
   } else if (variant == 2 && a.n == 128) {
     launch_reg<T, 32, 4>(a, prec, fio, nfields, s);
+  } else if ((variant == 4 || (variant < 0 && sizeof(T) == 4)) && tile_fits<T>(a)) {
+    // 128-byte tile rows: FP32 default on strided passes (measured at 128^3:
+    // 10.7 -> 7.5 us per y pass of both sweeps); for FP64 a tuning candidate
+    if (a.n == 128) {
+      if (sizeof(T) == 4) launch_tile<T, 8, 16, 4>(a, prec, fio, nfields, s);
+      else launch_tile<T, 8, 16, 2>(a, prec, fio, nfields, s);
+    } else {
+      if (sizeof(T) == 4) launch_tile<T, 8, 8, 2>(a, prec, fio, nfields, s);
+      else launch_tile<T, 8, 8, 1>(a, prec, fio, nfields, s);
+    }
+  } else if (variant == 5 && a.n == 128) {
+    launch_reg<T, 8, 16>(a, prec, fio, nfields, s);
   } else if (variant == 3 && a.n == 128) {
     launch_reg<T, 16, 8>(a, prec, fio, nfields, s);
   } else if (variant == 3 && a.n == 32) {
@@ -2044,14 +2078,17 @@ extern "C" psx_status psx_scft_time_step(psx_scft* e, int reps, double* ms_step,
 // ===========================================================================
 namespace {
 
-// Candidate line-kernel variants for length n: -1 default split (n = 128:
-// lane pairs 8x16), 1 LDS Stockham, 2 alternative register split (n = 64:
-// 16x4, n = 128: 32x4), 3 second alternative (n = 32: lane pairs 4x8,
-// n = 128: 16x8).
+// Candidate line-kernel variants for length n: -1 default (n = 128: lane
+// pairs 8x16; FP32 strided passes at n = 64, 128: 128-byte tiles), 1 LDS
+// Stockham, 2 alternative register split (n = 64: 16x4, n = 128: 32x4),
+// 3 second alternative (n = 32: lane pairs 4x8, n = 128: 16x8), 4 128-byte
+// tiles on strided passes (n = 64, 128), 5 lane pairs 8x16 (n = 128).
 std::vector<int> variant_candidates(int n) {
   std::vector<int> v{-1, 1};
   if (n == 64 || n == 128) v.push_back(2);
   if (n == 32 || n == 128) v.push_back(3);
+  if (n == 64 || n == 128) v.push_back(4);
+  if (n == 128) v.push_back(5);
   return v;
 }
 
@@ -2092,8 +2129,8 @@ struct BatchClass {
 extern "C" psx_status psx_scft_set_variant(psx_scft* e, int pass, int variant) {
   if (!e) return fail(PSX_ERR_INVALID, "null engine");
   if (pass < 0 || pass >= (int)e->axes.size()) return fail(PSX_ERR_INVALID, "pass index out of range");
-  if (variant != -1 && variant != 1 && variant != 2 && variant != 3)
-    return fail(PSX_ERR_INVALID, "line-kernel variant must be -1, 1, 2 or 3");
+  if (variant != -1 && (variant < 1 || variant > 5))
+    return fail(PSX_ERR_INVALID, "line-kernel variant must be -1 or 1..5");
   e->variant[e->prec][pass] = variant;
   drop_graphs(e);
   e->swept = false;
