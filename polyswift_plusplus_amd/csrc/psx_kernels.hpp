@@ -520,6 +520,7 @@ k_quad_update(QuadArgs a) {
 #pragma unroll
     for (int v = 0; v < V; ++v) quad_tail<T>(a, r0 + v, phA[v], phB[v], acc, slot, pbar);
   }
+  if (a.mode == QM_ACC) return;   // no partials (the launch may overlap other work)
   block_sum<kNPart>(acc);
   if (threadIdx.x == 0) {
 #pragma unroll

@@ -550,6 +550,23 @@ struct psx_scft {
   hipEvent_t ev_fork = nullptr;
   std::vector<hipEvent_t> ev_join;      // per slot
   bool split_sweeps = true;
+  // Quadrature under the sweeps (linear chains, PSX_OVERLAP_QUAD=1): point t
+  // of the contour needs forward slice t and backward slice Ns-t, both
+  // written once the sweeps pass step max(t, Ns-t).  Chunks of points run on
+  // qstream as their slices appear (raw sums into phiA/phiB); the last chunk
+  // finishes after the join.  Off by default: at 128^3 the sweeps already
+  // fill the wave slots and the concurrent chunks slowed the iteration
+  // (FP64 173.3 -> 171.0 it/s, FP32 292.1 -> 286.2).
+  bool overlap_quad = false;
+  hipStream_t qstream = nullptr;
+  hipEvent_t ev_qdone = nullptr;
+  struct QChunk {
+    std::vector<std::pair<int, int>> ranges;   // forward point ranges [lo, hi]
+    int launch = 0;                            // after this launch of both paths its slices exist
+    hipEvent_t ev[2] = {nullptr, nullptr};
+  };
+  std::vector<QChunk> qchunks;                 // empty: quadrature after the sweeps
+  bool quad_partial = false;                   // enqueueing an overlapped iteration
   int dims[3];
   long long N;
   double cell[3];
@@ -894,7 +911,12 @@ psx_status launch_junctions(psx_scft* e, int r) {
 // split_sweeps off and equal path lengths, all paths share each launch.
 // only_axis >= 0 keeps the launches along that axis only (timing).
 template <typename T>
-psx_status sweep_range(psx_scft* e, int only_axis = -1) {
+psx_status launch_quad_points(psx_scft* e, int mode, const psx_mixing* mix,
+                              const std::vector<std::pair<int, int>>& ranges, int acc_in,
+                              hipStream_t stream);
+
+template <typename T>
+psx_status sweep_range(psx_scft* e, int only_axis = -1, bool quad_chunks = false) {
   SweepBufs& s = e->sw[e->prec];
   for (size_t r = 0; r < e->rounds.size(); ++r) {
     const std::vector<int>& rp = e->rounds[r];
@@ -927,11 +949,27 @@ psx_status sweep_range(psx_scft* e, int only_axis = -1) {
         if (only_axis >= 0 && a != only_axis) continue;
         hipStream_t st = (fork && k > 0) ? e->pstream[k] : e->stream;
         PSX_TRY(launch_pair<T>(e->axes[a], e->prec, tab + j * np + k, 1, st, e->variant[e->prec][a]));
+        if (quad_chunks)
+          for (size_t c = 0; c + 1 < e->qchunks.size(); ++c)
+            if (e->qchunks[c].launch == (int)j) PSX_HIP(hipEventRecord(e->qchunks[c].ev[k], st));
       }
+    // contour chunks whose slices now exist: raw sums on the quadrature stream
+    if (quad_chunks && fork)
+      for (size_t j = 0; j < rows; ++j)
+        for (size_t c = 0; c + 1 < e->qchunks.size(); ++c) {
+          if (e->qchunks[c].launch != (int)j) continue;
+          PSX_HIP(hipStreamWaitEvent(e->qstream, e->qchunks[c].ev[0], 0));
+          PSX_HIP(hipStreamWaitEvent(e->qstream, e->qchunks[c].ev[1], 0));
+          PSX_TRY(launch_quad_points<T>(e, QM_ACC, nullptr, e->qchunks[c].ranges, c > 0 ? 1 : 0, e->qstream));
+        }
     if (fork) {
       for (int k = 1; k < np; ++k) {
         PSX_HIP(hipEventRecord(e->ev_join[k], e->pstream[k]));
         PSX_HIP(hipStreamWaitEvent(e->stream, e->ev_join[k], 0));
+      }
+      if (quad_chunks) {
+        PSX_HIP(hipEventRecord(e->ev_qdone, e->qstream));
+        PSX_HIP(hipStreamWaitEvent(e->stream, e->ev_qdone, 0));
       }
     }
   }
@@ -1064,14 +1102,46 @@ QuadArgs quad_args(psx_scft* e, int mode, const psx_mixing* mix) {
 // 128^3), FP64 cell by cell (double2 measured slower, 780 -> 815 us).
 // Always e->nb blocks: k_finish reduces that many partials.
 template <typename T>
-psx_status launch_quad_kernel(psx_scft* e, const QuadArgs& a) {
+psx_status launch_quad_kernel(psx_scft* e, const QuadArgs& a, hipStream_t stream = nullptr) {
   constexpr int V = sizeof(T) == 4 ? 4 : 1;
+  if (!stream) stream = e->stream;
   if (V > 1 && e->N % V == 0) {
-    hipLaunchKernelGGL((k_quad_update<T, V>), dim3(e->nb), dim3(kThreads), 0, e->stream, a);
+    hipLaunchKernelGGL((k_quad_update<T, V>), dim3(e->nb), dim3(kThreads), 0, stream, a);
   } else {
-    hipLaunchKernelGGL((k_quad_update<T, 1>), dim3(e->nb), dim3(kThreads), 0, e->stream, a);
+    hipLaunchKernelGGL((k_quad_update<T, 1>), dim3(e->nb), dim3(kThreads), 0, stream, a);
   }
+  PSX_HIP(hipGetLastError());
   return PSX_OK;
+}
+
+// The quadrature over forward point ranges [lo, hi] of an oriented linear
+// chain (block b covers points off[0] .. off[0] + steps): point t pairs
+// forward slice t (path 0) with backward slice Ns - t (path 1).
+template <typename T>
+psx_status launch_quad_points(psx_scft* e, int mode, const psx_mixing* mix,
+                              const std::vector<std::pair<int, int>>& ranges, int acc_in,
+                              hipStream_t stream) {
+  QuadArgs a = quad_args(e, mode, mix);
+  a.nblk = 0;
+  a.acc_in = acc_in;
+  const int Ns = e->paths[0].len;
+  int woff = 0;
+  for (const BlockInfo& bi : e->blocks) {
+    const int bs = bi.off[0];
+    for (const auto& r : ranges) {
+      const int lo = std::max(bs, r.first), hi = std::min(bs + bi.steps, r.second);
+      if (lo > hi) continue;
+      if (a.nblk == kMaxBlocks) return fail(PSX_ERR_INVALID, "quadrature chunk spans too many blocks");
+      QuadBlock& q = a.blk[a.nblk++];
+      q.H = slice_ptr(e, e->prec, 0, lo);
+      q.T = slice_ptr(e, e->prec, 1, Ns - hi);
+      q.M = hi - lo + 1;
+      q.woff = woff + (lo - bs);
+      q.species = bi.species;
+    }
+    woff += bi.steps + 1;
+  }
+  return launch_quad_kernel<T>(e, a, stream);
 }
 
 // Checkpointed mode: the contour sums chunk by chunk (raw sums carried in
@@ -1111,6 +1181,13 @@ psx_status launch_quad_ckpt(psx_scft* e, int mode, const psx_mixing* mix) {
 psx_status launch_quad(psx_scft* e, int mode, const psx_mixing* mix) {
   if (e->ckpt_K > 0 && mode != QM_SD_PHI && mode != QM_FE)
     return e->prec == PSX_FP64 ? launch_quad_ckpt<double>(e, mode, mix) : launch_quad_ckpt<float>(e, mode, mix);
+  if (e->quad_partial && mode != QM_SD_PHI && mode != QM_FE) {
+    // the earlier chunks ran under the sweeps; the last one finishes here
+    const int acc = e->qchunks.size() > 1 ? 1 : 0;
+    return e->prec == PSX_FP64
+               ? launch_quad_points<double>(e, mode, mix, e->qchunks.back().ranges, acc, e->stream)
+               : launch_quad_points<float>(e, mode, mix, e->qchunks.back().ranges, acc, e->stream);
+  }
   QuadArgs a = quad_args(e, mode, mix);
   if (e->prec == PSX_FP64) PSX_TRY(launch_quad_kernel<double>(e, a));
   else PSX_TRY(launch_quad_kernel<float>(e, a));
@@ -1152,8 +1229,16 @@ psx_status enqueue_post_sweep(psx_scft* e, const psx_mixing* mix);
 
 // Enqueue one full SCFT iteration on e->stream (capturable).
 psx_status enqueue_iteration(psx_scft* e, const psx_mixing* mix) {
-  PSX_TRY(sweep(e));
-  return enqueue_post_sweep(e, mix);
+  const bool ov = !e->qchunks.empty() && e->split_sweeps && e->ckpt_K == 0;
+  if (!ov) {
+    PSX_TRY(sweep(e));
+    return enqueue_post_sweep(e, mix);
+  }
+  PSX_TRY(e->prec == PSX_FP64 ? sweep_range<double>(e, -1, true) : sweep_range<float>(e, -1, true));
+  e->quad_partial = true;
+  const psx_status st = enqueue_post_sweep(e, mix);
+  e->quad_partial = false;
+  return st;
 }
 
 // Q, quadrature and the field update after both sweeps (capturable).
@@ -1220,6 +1305,9 @@ extern "C" psx_status psx_scft_destroy(psx_scft* e) {
     if (e->pstream[k]) { (void)hipStreamSynchronize(e->pstream[k]); (void)hipStreamDestroy(e->pstream[k]); }
   if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
   for (auto& ev : e->ev_join) if (ev) (void)hipEventDestroy(ev);
+  if (e->qstream) { (void)hipStreamSynchronize(e->qstream); (void)hipStreamDestroy(e->qstream); }
+  if (e->ev_qdone) (void)hipEventDestroy(e->ev_qdone);
+  for (auto& c : e->qchunks) for (auto& ev : c.ev) if (ev) (void)hipEventDestroy(ev);
   if (e->own_stream && e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
   return PSX_OK;
@@ -1395,6 +1483,47 @@ static psx_status build_graph(psx_scft* e, int nb, const psx_block* d) {
   return PSX_OK;
 }
 
+// Chunks of contour points for the quadrature under the sweeps (see
+// psx_scft::qchunks): oriented linear chains, two paths in one round.
+static void plan_quad_overlap(psx_scft* e) {
+  e->qchunks.clear();
+  if (!e->overlap_quad || e->rounds.size() != 1 || e->paths.size() != 2) return;
+  if (!e->blocks[0].cnt[0].empty() || 2 * e->blocks.size() > (size_t)kMaxBlocks) return;
+  for (const BlockInfo& b : e->blocks)
+    if (b.path[0] != 0 || b.path[1] != 1) return;
+  const int Ns = e->paths[0].len;
+  if (Ns < 8) return;
+  // highest slice each launch of a path has written
+  const int na = (int)e->axes.size();
+  std::vector<int> written;
+  int w = 0;
+  for (const auto& l : sweep_launches(na, Ns)) {
+    const SweepOp& o = l.first;
+    if (l.second == 2) w = std::max(w, o.step + 1 + (na == 1 ? 1 : 0));
+    else if (o.last) w = std::max(w, o.step + 1);
+    written.push_back(w);
+  }
+  // point t is ready at step r(t) = max(t, Ns - t) in [rmin, Ns]
+  const int nch = 4, rmin = (Ns + 1) / 2;
+  std::vector<int> S(nch);
+  for (int c = 0; c < nch; ++c) S[c] = rmin + ((Ns - rmin) * (c + 1) + nch - 1) / nch;
+  S[nch - 1] = Ns;
+  e->qchunks.assign(nch, psx_scft::QChunk());
+  for (int t = 0; t <= Ns; ++t) {
+    const int r = std::max(t, Ns - t);
+    int c = 0;
+    while (r > S[c]) ++c;
+    auto& rg = e->qchunks[c].ranges;
+    if (!rg.empty() && rg.back().second == t - 1) rg.back().second = t;
+    else rg.push_back({t, t});
+  }
+  for (int c = 0; c < nch; ++c) {
+    int j = 0;
+    while (j + 1 < (int)written.size() && written[j] < S[c]) ++j;
+    e->qchunks[c].launch = j;
+  }
+}
+
 static psx_status create_engine(const psx_grid* g, int nb, const psx_block* blocks, double ds,
                                 double chiN, psx_prec prec, void* stream, psx_scft** out) {
   if (!out || !g || !blocks) return fail(PSX_ERR_INVALID, "null argument");
@@ -1429,6 +1558,19 @@ static psx_status create_engine(const psx_grid* g, int nb, const psx_block* bloc
   E_TRY(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming) == hipSuccess
             ? PSX_OK : fail(PSX_ERR_HIP, "event creation failed"));
   if (const char* v = getenv("PSX_SPLIT_SWEEPS")) e->split_sweeps = v[0] != '0';
+  if (const char* v = getenv("PSX_OVERLAP_QUAD")) e->overlap_quad = v[0] == '1';
+  if (st == PSX_OK) {
+    plan_quad_overlap(e);
+    if (!e->qchunks.empty()) {
+      E_TRY(hipStreamCreateWithFlags(&e->qstream, hipStreamNonBlocking) == hipSuccess &&
+                    hipEventCreateWithFlags(&e->ev_qdone, hipEventDisableTiming) == hipSuccess
+                ? PSX_OK : fail(PSX_ERR_HIP, "stream creation failed"));
+      for (auto& c : e->qchunks)
+        for (auto& ev : c.ev)
+          E_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess
+                    ? PSX_OK : fail(PSX_ERR_HIP, "event creation failed"));
+    }
+  }
   const size_t fb = e->N * sizeof(double);
   E_TRY(dalloc((void**)&e->wA, fb));
   E_TRY(dalloc((void**)&e->wB, fb));

@@ -26,8 +26,11 @@ def _single(sp, i, prec, n):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_batch_matches_single_instances(prec):
+def test_batch_matches_single_instances(prec, monkeypatch):
+    """Like for like: the batch runs its quadrature after the sweeps, so the
+    single-instance references do too (PSX_OVERLAP_QUAD=0, the default)."""
     print(f"batch {prec}", flush=True)
+    monkeypatch.setenv("PSX_OVERLAP_QUAD", "0")
     b = BatchSCFT(SPECS, prec=prec)
     assert b.nclasses == 3
     for i, sp in enumerate(SPECS):
