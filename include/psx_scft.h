@@ -262,8 +262,9 @@ psx_status psx_scft_autotune(psx_scft* e, int reps, int* h_choice);
 /* Choose the line kernel of pass `pass` (grid axes with n > 1 from the
  * last, z, to the first, x) for the current precision: -1 the default for
  * the line length, 1 LDS Stockham, 2 and 3 the alternative register splits
- * (n = 64: 16x4; n = 128: 32x4 and 16x8; n = 32: -, 4x8), 4 the 128-byte
- * tiles for strided passes at n = 64, 128 (the FP32 default there), 5 the
+ * (n = 64: 16x4; n = 96: 24x4, -; n = 128: 32x4 and 16x8; n = 32: -,
+ * 4x8), 4 the 128-byte LDS row tiles for strided passes at n = 64, 96, 128
+ * (the FP32 default there), 5 the
  * lane-pair register split 8x16 at n = 128 (the FP64 default); a variant
  * that does not apply to a pass uses the default.  No reference
  * counterpart. */

@@ -218,13 +218,92 @@ __device__ __forceinline__ void pair_roundtrip(C* v, C* L, const C* __restrict__
   dft_reg<N1, +1, T>(v, tw, N2);
 }
 
+// Lane-pair round trip for the general split 2 N1 % N2 == 0 (N2 a power of
+// two): the pair k = t >> 1 takes the P = 2 N1 / N2 columns k1 = k + H u
+// (H = N2 / 2, u < P), lane h of the pair the points t' = h + 2 m (m < H) of
+// each; per column an in-register DFT_H over m and a radix 2 across the
+// pair, as in pair_roundtrip (its P = 1 case).  Registers: column u holds
+// v[u H .. u H + H - 1].  With N1 = 12, N2 = 8 a line of 96 takes 8 lanes
+// of 12 points: twice the lines per wave of the 24 x 4 split.
+template <typename T, int N1, int N2, typename C>
+__device__ __forceinline__ void pairs_roundtrip(C* v, C* L, const C* __restrict__ tw,
+                                                const C* twl, const T* emul, int t) {
+  constexpr int n = N1 * N2;
+  constexpr int H = N2 / 2;
+  constexpr int P = N1 / H;
+  static_assert(P * H == N1 && (N2 & (N2 - 1)) == 0, "bad lane-pair split");
+  dft_reg<N1, -1, T>(v, tw, N2);
+#pragma unroll
+  for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], twiddle<-1>(twl, t * k1));
+  __syncthreads();
+#pragma unroll
+  for (int k1 = 0; k1 < N1; ++k1) L[k1 * N2 + (t ^ ((2 * k1) % N2))] = v[k1];
+  __syncthreads();
+  const int k = t >> 1, h = t & 1;
+#pragma unroll
+  for (int u = 0; u < P; ++u) {
+    const int k1 = k + H * u, sw = (2 * k1) % N2;
+#pragma unroll
+    for (int m = 0; m < H; ++m) v[u * H + m] = L[k1 * N2 + ((h + 2 * m) ^ sw)];
+  }
+#pragma unroll
+  for (int u = 0; u < P; ++u) dft_reg<H, -1, T>(v + u * H, tw, n / H);
+  // radix 2 across the pair, the spectral multiply at k1 + N1 (j + H h),
+  // and the inverse radix 2
+#pragma unroll
+  for (int u = 0; u < P; ++u) {
+    const int k1 = k + H * u;
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      C& y = v[u * H + j];
+      C o;
+      o.x = pair_swap(y.x);
+      o.y = pair_swap(y.y);
+      const C e0 = h ? o : y;
+      const C e1 = h ? y : o;
+      const C w1 = cmul(e1, twiddle<-1>(tw, j * N1));
+      C x;
+      if (h) { x.x = e0.x - w1.x; x.y = e0.y - w1.y; }
+      else { x.x = e0.x + w1.x; x.y = e0.y + w1.y; }
+      const T m = emul[k1 + N1 * (j + H * h)];
+      x.x *= m; x.y *= m;
+      o.x = pair_swap(x.x);
+      o.y = pair_swap(x.y);
+      const C lo = h ? o : x;
+      const C hi = h ? x : o;
+      if (h) {
+        C d; d.x = lo.x - hi.x; d.y = lo.y - hi.y;
+        y = cmul(d, twiddle<+1>(tw, j * N1));
+      } else {
+        y.x = lo.x + hi.x; y.y = lo.y + hi.y;
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < P; ++u) dft_reg<H, +1, T>(v + u * H, tw, n / H);
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < P; ++u) {
+    const int k1 = k + H * u, sw = (2 * k1) % N2;
+#pragma unroll
+    for (int m = 0; m < H; ++m) L[k1 * N2 + ((h + 2 * m) ^ sw)] = v[u * H + m];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < N1; ++j) v[j] = L[j * N2 + (t ^ ((2 * j) % N2))];
+#pragma unroll
+  for (int j = 1; j < N1; ++j) v[j] = cmul(v[j], twiddle<+1>(twl, t * j));
+  dft_reg<N1, +1, T>(v, tw, N2);
+}
+
 // Round trip on the lanes of the split (N1, N2): lane-pair form when
-// N2 = 2 N1, else the N1 % N2 == 0 form.
+// N2 = 2 N1, the N1 % N2 == 0 form, else the general lane-pair form.
 template <typename T, int N1, int N2, typename C>
 __device__ __forceinline__ void line_roundtrip(C* v, C* L, const C* __restrict__ tw,
                                                const C* twl, const T* emul, int t) {
   if constexpr (N2 == 2 * N1) pair_roundtrip<T, N1>(v, L, tw, twl, emul, t);
-  else reg_roundtrip<T, N1, N2>(v, L, tw, twl, emul, t);
+  else if constexpr (N1 % N2 == 0) reg_roundtrip<T, N1, N2>(v, L, tw, twl, emul, t);
+  else pairs_roundtrip<T, N1, N2>(v, L, tw, twl, emul, t);
 }
 
 // Round trip(s) along one axis for 2*64/N2 real lines per wave:
@@ -244,7 +323,7 @@ k_line_reg(LineGeom g, const PairIO* __restrict__ fio) {
   constexpr int n = N1 * N2;
   constexpr int CL = 64 / N2;     // complex lines per wave
   constexpr int RL = 2 * CL;      // real lines per wave
-  static_assert((N1 % N2 == 0 || N2 == 2 * N1) && 64 % N2 == 0, "bad split");
+  static_assert((N1 % N2 == 0 || (2 * N1) % N2 == 0) && 64 % N2 == 0, "bad split");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C* lds = (C*)smem_raw;
   int tile, field;

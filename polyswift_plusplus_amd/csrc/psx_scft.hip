@@ -207,12 +207,11 @@ void launch_reg(const Axis& a, int prec, const PairIO* fio, int nfields,
     hipLaunchKernelGGL((k_line_reg<T, N1, N2, false, false>), grid, dim3(64), smem, s, g, fio);
 }
 
-// The tiled strided kernel applies at n = 64, 128 when whole 128-byte tile
-// rows (32 FP32 / 16 FP64 lines) fit the line stride.
+// The tiled strided kernel applies at n = 64, 96, 128 when whole tile rows
+// of lt lines fit the line stride (128-byte rows: 32 FP32 / 16 FP64 lines).
 template <typename T>
-bool tile_fits(const Axis& a) {
-  const int lt = sizeof(T) == 4 ? 32 : 16;
-  return (a.n == 128 || a.n == 64) && !a.contig && a.inner % lt == 0;
+bool tile_fits(const Axis& a, int lt = sizeof(T) == 4 ? 32 : 16) {
+  return (a.n == 128 || a.n == 96 || a.n == 64) && !a.contig && a.inner % lt == 0;
 }
 
 // Strided axis through 128-byte tile rows (k_line_tile): W waves of RL lines.
@@ -285,10 +284,15 @@ psx_status launch_pair(const Axis& a, int prec, const PairIO* fio,
     launch_reg<T, 16, 4>(a, prec, fio, nfields, s);
   } else if (variant == 2 && a.n == 128) {
     launch_reg<T, 32, 4>(a, prec, fio, nfields, s);
+  } else if (variant == 2 && a.n == 96) {
+    launch_reg<T, 24, 4>(a, prec, fio, nfields, s);
   } else if ((variant == 4 || (variant < 0 && sizeof(T) == 4)) && tile_fits<T>(a)) {
     // 128-byte tile rows: FP32 default on strided passes (measured at 128^3:
     // 10.7 -> 7.5 us per y pass of both sweeps); for FP64 a tuning candidate
-    if (a.n == 128) {
+    if (a.n == 96) {
+      if (sizeof(T) == 4) launch_tile<T, 12, 8, 2>(a, prec, fio, nfields, s);
+      else launch_tile<T, 12, 8, 1>(a, prec, fio, nfields, s);
+    } else if (a.n == 128) {
       if (sizeof(T) == 4) launch_tile<T, 8, 16, 4>(a, prec, fio, nfields, s);
       else launch_tile<T, 8, 16, 2>(a, prec, fio, nfields, s);
     } else {
@@ -316,7 +320,10 @@ psx_status launch_pair(const Axis& a, int prec, const PairIO* fio,
       case 32: launch_reg<T, 8, 4>(a, prec, fio, nfields, s); break;
       case 48: launch_reg<T, 12, 4>(a, prec, fio, nfields, s); break;
       case 64: launch_reg<T, 8, 8>(a, prec, fio, nfields, s); break;
-      case 96: launch_reg<T, 24, 4>(a, prec, fio, nfields, s); break;
+      // general lane pairs at 96: 8 lanes of 12 points, 16 real lines per
+      // wave (measured at 96^3: FP32 step 35.3 -> 26.0 us, FP64 51.5 ->
+      // 34.7 us against 4 lanes of 24 points)
+      case 96: launch_reg<T, 12, 8>(a, prec, fio, nfields, s); break;
       // lane pairs at 128: 8 points per lane, 3 (FP64) / 5 (FP32) waves per
       // SIMD against 2 for 16 points per lane
       case 128: launch_reg<T, 8, 16>(a, prec, fio, nfields, s); break;
@@ -2221,15 +2228,16 @@ extern "C" psx_status psx_scft_time_step(psx_scft* e, int reps, double* ms_step,
 namespace {
 
 // Candidate line-kernel variants for length n: -1 default (n = 128: lane
-// pairs 8x16; FP32 strided passes at n = 64, 128: 128-byte tiles), 1 LDS
-// Stockham, 2 alternative register split (n = 64: 16x4, n = 128: 32x4),
-// 3 second alternative (n = 32: lane pairs 4x8, n = 128: 16x8), 4 128-byte
-// tiles on strided passes (n = 64, 128), 5 lane pairs 8x16 (n = 128).
+// pairs 8x16; n = 96: lane pairs 12x8; FP32 strided passes at n = 64, 96,
+// 128: 128-byte tiles), 1 LDS Stockham, 2 alternative register split
+// (n = 64: 16x4, n = 96: 24x4, n = 128: 32x4),
+// 3 second alternative (n = 32: lane pairs 4x8, n = 128: 16x8), 4 tiles on
+// strided passes (n = 64, 96, 128), 5 lane pairs 8x16 (n = 128).
 std::vector<int> variant_candidates(int n) {
   std::vector<int> v{-1, 1};
-  if (n == 64 || n == 128) v.push_back(2);
+  if (n == 64 || n == 96 || n == 128) v.push_back(2);
   if (n == 32 || n == 128) v.push_back(3);
-  if (n == 64 || n == 128) v.push_back(4);
+  if (n == 64 || n == 96 || n == 128) v.push_back(4);
   if (n == 128) v.push_back(5);
   return v;
 }
@@ -2318,6 +2326,14 @@ extern "C" psx_status psx_scft_autotune(psx_scft* e, int reps, int* h_choice) {
   return PSX_OK;
 }
 
+// A batch runs on kBatchStreams streams (the caller's and three of its
+// own): each size class sweeps on one of them and each instance's
+// post-sweep work (Q, quadrature, Anderson, mixing) on one of them after its
+// class's sweep, so the small classes' latency-bound launches overlap the
+// large classes' HBM-bound ones.  The placement is a list schedule over
+// estimated costs, fixed at creation; results do not depend on it.
+constexpr int kBatchStreams = 4;
+
 struct psx_batch {
   std::vector<psx_scft*> eng;
   std::vector<BatchClass> cls;
@@ -2326,7 +2342,65 @@ struct psx_batch {
   int prec = PSX_FP64;
   int Ns = 0;
   std::map<GraphKey, hipGraphExec_t> graphs;
+  hipStream_t ss[kBatchStreams] = {};   // ss[0] = stream
+  hipEvent_t ev_fork = nullptr;
+  hipEvent_t ev_join[kBatchStreams] = {};
+  std::vector<hipEvent_t> ev_cls;       // per class: its sweep is done
+  std::vector<int> cls_stream;          // per class
+  std::vector<int> post_order;          // instances in post-sweep enqueue order
+  std::vector<int> inst_stream;         // per instance
 };
+
+// Estimated device time (us) of a class's sweep and of one instance's
+// post-sweep work: HBM bytes at 4 TB/s plus 4 us per launch.
+static double batch_sweep_cost(const psx_batch* b, const BatchClass& c) {
+  const psx_scft* e0 = b->eng[c.members[0]];
+  const double cells = (double)e0->N * c.members.size();
+  const double pb = prec_bytes(b->prec);
+  return b->Ns * (12.0 * pb * cells / 4.0e6) + 4.0 * e0->paths[0].launch_axis.size();
+}
+
+static double batch_post_cost(const psx_batch* b, const psx_scft* e) {
+  const double pb = prec_bytes(b->prec);
+  return (double)e->N * (2.0 * (b->Ns + 1) * pb + 200.0) / 4.0e6 + 9 * 4.0;
+}
+
+static void batch_schedule(psx_batch* b) {
+  const int nc = (int)b->cls.size();
+  double ready[kBatchStreams] = {};
+  std::vector<double> done(nc);
+  std::vector<int> order(nc);
+  for (int c = 0; c < nc; ++c) order[c] = c;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    return batch_sweep_cost(b, b->cls[x]) > batch_sweep_cost(b, b->cls[y]);
+  });
+  b->cls_stream.assign(nc, 0);
+  for (int c : order) {
+    int k = 0;
+    for (int j = 1; j < kBatchStreams; ++j) if (ready[j] < ready[k]) k = j;
+    b->cls_stream[c] = k;
+    ready[k] += batch_sweep_cost(b, b->cls[c]);
+    done[c] = ready[k];
+  }
+  std::vector<int> cls_of(b->eng.size());
+  for (int c = 0; c < nc; ++c) for (int m : b->cls[c].members) cls_of[m] = c;
+  b->post_order.resize(b->eng.size());
+  for (size_t i = 0; i < b->eng.size(); ++i) b->post_order[i] = (int)i;
+  std::stable_sort(b->post_order.begin(), b->post_order.end(), [&](int x, int y) {
+    return done[cls_of[x]] < done[cls_of[y]];
+  });
+  b->inst_stream.assign(b->eng.size(), 0);
+  for (int i : b->post_order) {
+    int k = 0;
+    double best = 1e300;
+    for (int j = 0; j < kBatchStreams; ++j) {
+      const double start = std::max(ready[j], done[cls_of[i]]);
+      if (start < best) { best = start; k = j; }
+    }
+    b->inst_stream[i] = k;
+    ready[k] = best + batch_post_cost(b, b->eng[i]);
+  }
+}
 
 static void batch_drop_graphs(psx_batch* b) {
   for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
@@ -2356,8 +2430,12 @@ static psx_status batch_tables(psx_batch* b, int prec) {
 
 extern "C" psx_status psx_batch_destroy(psx_batch* b) {
   if (!b) return PSX_OK;
-  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (auto s : b->ss) if (s) (void)hipStreamSynchronize(s);
   batch_drop_graphs(b);
+  for (int k = 1; k < kBatchStreams; ++k) if (b->ss[k]) (void)hipStreamDestroy(b->ss[k]);
+  if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+  for (auto ev : b->ev_join) if (ev) (void)hipEventDestroy(ev);
+  for (auto ev : b->ev_cls) if (ev) (void)hipEventDestroy(ev);
   for (auto& c : b->cls) { dfree((void*&)c.fio[0]); dfree((void*&)c.fio[1]); }
   for (auto* e : b->eng) psx_scft_destroy(e);
   if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
@@ -2397,6 +2475,21 @@ extern "C" psx_status psx_batch_create(int n, const psx_grid* grids, const psx_d
     }
     st = batch_tables(b, prec);
   }
+  if (st == PSX_OK) {
+    b->ss[0] = b->stream;
+    for (int k = 1; k < kBatchStreams && st == PSX_OK; ++k)
+      if (hipStreamCreateWithFlags(&b->ss[k], hipStreamNonBlocking) != hipSuccess)
+        st = fail(PSX_ERR_HIP, "hipStreamCreate");
+    auto mk = [&](hipEvent_t* ev) {
+      if (st == PSX_OK && hipEventCreateWithFlags(ev, hipEventDisableTiming) != hipSuccess)
+        st = fail(PSX_ERR_HIP, "hipEventCreate");
+    };
+    mk(&b->ev_fork);
+    for (auto& ev : b->ev_join) mk(&ev);
+    b->ev_cls.assign(b->cls.size(), nullptr);
+    for (auto& ev : b->ev_cls) mk(&ev);
+    if (st == PSX_OK) batch_schedule(b);
+  }
   if (st != PSX_OK) { psx_batch_destroy(b); return st; }
   *out = b;
   return PSX_OK;
@@ -2420,6 +2513,7 @@ extern "C" psx_status psx_batch_set_precision(psx_batch* b, psx_prec prec) {
   for (auto* e : b->eng) PSX_TRY(psx_scft_set_precision(e, prec));
   PSX_TRY(batch_tables(b, prec));
   b->prec = prec;
+  batch_schedule(b);
   return PSX_OK;
 }
 
@@ -2446,26 +2540,47 @@ extern "C" psx_status psx_batch_autotune(psx_batch* b, int reps, int* h_choice) 
 }
 
 template <typename T>
-static psx_status batch_sweep(psx_batch* b) {
-  size_t nmax = 0;
-  for (auto& c : b->cls) nmax = std::max(nmax, b->eng[c.members[0]]->paths[0].launch_axis.size());
-  for (size_t j = 0; j < nmax; ++j)
-    for (auto& c : b->cls) {
-      psx_scft* e0 = b->eng[c.members[0]];
-      const PathInfo& pi = e0->paths[0];
-      if (j >= pi.launch_axis.size()) continue;
-      const int a = pi.launch_axis[j];
-      const int K = (int)c.members.size();
-      PSX_TRY(launch_pair<T>(e0->axes[a], b->prec, c.fio[b->prec] + j * 2 * K, 2 * K, b->stream,
-                             c.variant[b->prec][a]));
-    }
+static psx_status batch_sweep(psx_batch* b, size_t ci) {
+  BatchClass& c = b->cls[ci];
+  psx_scft* e0 = b->eng[c.members[0]];
+  const PathInfo& pi = e0->paths[0];
+  const int K = (int)c.members.size();
+  hipStream_t s = b->ss[b->cls_stream[ci]];
+  for (size_t j = 0; j < pi.launch_axis.size(); ++j) {
+    const int a = pi.launch_axis[j];
+    PSX_TRY(launch_pair<T>(e0->axes[a], b->prec, c.fio[b->prec] + j * 2 * K, 2 * K, s, c.variant[b->prec][a]));
+  }
+  PSX_HIP(hipEventRecord(b->ev_cls[ci], s));
   return PSX_OK;
 }
 
+// One batch iteration on the batch streams, forked from and joined back to
+// b->stream (capturable).  An instance's post-sweep work is enqueued on its
+// batch stream by lending that stream to the engine for the call.
 static psx_status batch_enqueue(psx_batch* b, const psx_mixing* mix) {
-  if (b->prec == PSX_FP64) PSX_TRY(batch_sweep<double>(b));
-  else PSX_TRY(batch_sweep<float>(b));
-  for (auto* e : b->eng) PSX_TRY(enqueue_post_sweep(e, mix));
+  PSX_HIP(hipEventRecord(b->ev_fork, b->stream));
+  for (int k = 1; k < kBatchStreams; ++k) PSX_HIP(hipStreamWaitEvent(b->ss[k], b->ev_fork, 0));
+  for (size_t ci = 0; ci < b->cls.size(); ++ci) {
+    if (b->prec == PSX_FP64) PSX_TRY(batch_sweep<double>(b, ci));
+    else PSX_TRY(batch_sweep<float>(b, ci));
+  }
+  std::vector<int> cls_of(b->eng.size());
+  for (size_t c = 0; c < b->cls.size(); ++c) for (int m : b->cls[c].members) cls_of[m] = (int)c;
+  for (int i : b->post_order) {
+    psx_scft* e = b->eng[i];
+    hipStream_t s = b->ss[b->inst_stream[i]];
+    const int c = cls_of[i];
+    if (b->cls_stream[c] != b->inst_stream[i]) PSX_HIP(hipStreamWaitEvent(s, b->ev_cls[c], 0));
+    hipStream_t own = e->stream;
+    e->stream = s;
+    const psx_status st = enqueue_post_sweep(e, mix);
+    e->stream = own;
+    PSX_TRY(st);
+  }
+  for (int k = 1; k < kBatchStreams; ++k) {
+    PSX_HIP(hipEventRecord(b->ev_join[k], b->ss[k]));
+    PSX_HIP(hipStreamWaitEvent(b->stream, b->ev_join[k], 0));
+  }
   return PSX_OK;
 }
 

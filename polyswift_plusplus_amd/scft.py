@@ -258,15 +258,15 @@ class DiblockSCFT:
 
     def setVariant(self, variant: int, passes=None) -> None:
         """Line kernel of the given passes (all by default): -1 default,
-        1 LDS Stockham, 2 / 3 alternative register splits."""
+        1 LDS Stockham, 2 / 3 alternative register splits, 4 128-byte LDS
+        row tiles (strided passes), 5 lane pairs 8x16 (n = 128)."""
         na = sum(1 for n in self.shape if n > 1) or 1
         for p in (range(na) if passes is None else passes):
             check(self._L.psx_scft_set_variant(self._h, int(p), int(variant)))
 
     def autotune(self, reps: int = 10) -> list:
         """Pick the fastest line kernel per pass on this engine's data;
-        returns the choice per pass (-1 default, 1 Stockham, 2 / 3 the
-        alternative register splits)."""
+        returns the choice per pass (the variants of setVariant)."""
         ch = (ctypes.c_int * 3)()
         check(self._L.psx_scft_autotune(self._h, int(reps), ch))
         return list(ch)

@@ -88,11 +88,12 @@ def test_propagator_slices(shape, fA, ds):
 @pytest.mark.parametrize("variant", [-1, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("shape", [(8, 12, 128), (8, 128, 12), (128, 6, 10), (128, 3, 5),
                                    (6, 128, 3), (32, 16, 64), (64, 32, 12), (4, 128, 32),
-                                   (128, 2, 16), (64, 64, 32), (8, 64, 32)])
+                                   (128, 2, 16), (64, 64, 32), (8, 64, 32), (96, 4, 32),
+                                   (4, 96, 32), (6, 8, 96)])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_line_kernel_variants(shape, variant, prec):
     """Every line-kernel variant on every axis role (contiguous, strided with
-    paired loads, strided without, 128-byte tiles) at n = 32, 64, 128,
+    paired loads, strided without, LDS row tiles) at n = 32, 64, 96, 128,
     against the oracle (FP64 to 1e-12 in slices, FP32 to 2e-5)."""
     print(f"variant {variant} {shape} {prec}", flush=True)
     cell = (0.3, 0.25, 0.35)
