@@ -66,6 +66,55 @@ psx_status psx_fft_scaled_pair(psx_fft* plan, const void* d_data,
 psx_status psx_fft_scaled_pair_host(psx_fft* plan, const void* h_data,
                                     const void* h_kdata, void* h_res);
 
+/* The rest of PsFFTBase (psbase/PsFFTBase.h:83-139, psfft/PsFFTW.cpp:
+ * 121-194,226-296): device pointers of N elements of the plan's type,
+ * asynchronous on the plan's stream, unnormalised transforms.
+ *   forward_abs:    res = |FFT(data)|^2 (the squared modulus, as
+ *                   PsFFTW::forwardFFTAbs computes it)
+ *   convolve_re:    res = Re(IFFT(FFT(data1) .* FFT(data2)))
+ *   scaled_pair_im: res = Re(IFFT(kdata .* FFT(0 + i data)))
+ *   forward_re:     res = Re(FFT(data))        (calcForwardFFT)
+ *   backward_re:    res = Re(IFFT(data + 0i))  (calcBackwardFFT) */
+psx_status psx_fft_forward_abs(psx_fft* plan, const void* d_data, void* d_res);
+psx_status psx_fft_convolve_re(psx_fft* plan, const void* d_data1,
+                               const void* d_data2, void* d_res);
+psx_status psx_fft_scaled_pair_im(psx_fft* plan, const void* d_data,
+                                  const void* d_kdata, void* d_res);
+psx_status psx_fft_forward_re(psx_fft* plan, const void* d_data, void* d_res);
+psx_status psx_fft_backward_re(psx_fft* plan, const void* d_data, void* d_res);
+
+/* Spectral filter of one field, in place (device pointer, plan's type):
+ * PsSpecFilterUpdater::subtractAverage (w -= sum(w) / (N - wall_volume)),
+ * then the mask filterStrength where |FFT(w)|^2 < cutoffFactor * max, 1
+ * elsewhere, applied by the FFT pair, w = pair / N.  spec_cells = NULL:
+ * PsSimpleSpecFilter (one maximum, pseffhamil/PsSimpleSpecFilter.cpp:
+ * 74-144); spec_cells = {sx, sy, sz}: PsMultiSpecFilter with one maximum
+ * per spectral cell (pseffhamil/PsMultiSpecFilter.cpp:123-267, the region
+ * map as a serial run lays it out); each s must divide its grid size.
+ * cutoff_factor <= 0 leaves w untouched, as the reference returns early. */
+psx_status psx_fft_spec_filter(psx_fft* plan, void* d_w, double cutoff_factor,
+                               double filter_strength, const int* spec_cells,
+                               double wall_volume);
+/* PsPoissonUpdater::update for one field (pseffhamil/PsPoissonUpdater.cpp:
+ * 214-289), in place: the charge density loses its mean; psi becomes
+ * pair(charge, 4 * 3.1415 * bjerrum_len / k^2) / N (k = 0 mode 0, k^2 from
+ * the cell sizes as PsGridField::calck2 forms it), except where that changes
+ * psi by more than 20%: there psi becomes 1.2 psi_old / N, the reference's
+ * limiter as written. */
+psx_status psx_fft_poisson_update(psx_fft* plan, void* d_charge, void* d_psi,
+                                  double bjerrum_len, double dx, double dy,
+                                  double dz);
+/* Wait for the plan's stream. */
+psx_status psx_fft_synchronize(psx_fft* plan);
+
+/* Device buffers for callers without a HIP runtime of their own (FFI
+ * hosts): allocate, free, and synchronous copies.  No reference
+ * counterpart (the reference passes host arrays). */
+psx_status psx_buffer_create(size_t bytes, void** d_out);
+psx_status psx_buffer_destroy(void* d);
+psx_status psx_buffer_upload(void* d_dst, const void* h_src, size_t bytes);
+psx_status psx_buffer_download(void* h_dst, const void* d_src, size_t bytes);
+
 /* ===================================================================== */
 /* SCFT engine for a bulk AB diblock melt: the block propagators          */
 /* (PsBlockBase / PsFlexPseudoSpec), Q, densities, Flory interaction and  */
@@ -151,6 +200,13 @@ psx_status psx_scft_set_fields_host(psx_scft* e, const double* h_wA,
                                     const double* h_wB, const double* h_p);
 psx_status psx_scft_get_fields_host(psx_scft* e, double* h_wA, double* h_wB,
                                     double* h_p);
+/* Spectral filter updater (PsSimpleSpecFilter / PsMultiSpecFilter::update,
+ * pseffhamil/PsSimpleSpecFilter.cpp:74-103) on the engine's conjugate
+ * fields in place: fields is a mask of 1 (w_A) and 2 (w_B); the other
+ * arguments as psx_fft_spec_filter (bulk: no wall volume).  The next
+ * iteration propagates in the filtered fields. */
+psx_status psx_scft_spec_filter(psx_scft* e, int fields, double cutoff_factor,
+                                double filter_strength, const int* spec_cells);
 
 /* w-factors and all block propagators (PsFlexPseudoSpec::reset + solveQ,
  * pspolymer/PsFlexPseudoSpec.cpp:159-181,244-293; block-graph order of

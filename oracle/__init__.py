@@ -148,6 +148,15 @@ def lib():
         L.orc_box_stress.argtypes = [ctypes.POINTER(OrcParams), _dp, _dp, _dp]
         L.orc_time_steps.argtypes = [ctypes.c_int] * 4
         L.orc_time_steps.restype = ctypes.c_double
+        L.orc_k2.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [_dp]
+        L.orc_k2.restype = None
+        for nm in ("orc_fft_forward_abs", "orc_fft_forward_re", "orc_fft_backward_re"):
+            getattr(L, nm).argtypes = [ctypes.c_int] * 3 + [_dp] * 2
+        for nm in ("orc_fft_convolve_re", "orc_fft_scaled_pair_im"):
+            getattr(L, nm).argtypes = [ctypes.c_int] * 3 + [_dp] * 3
+        L.orc_spec_filter.argtypes = [ctypes.c_int] * 3 + [_dp, ctypes.c_double, ctypes.c_double,
+                                                           ctypes.POINTER(ctypes.c_int), ctypes.c_double]
+        L.orc_poisson_update.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 4 + [_dp] * 2
         _lib = L
     return _lib
 
@@ -322,3 +331,70 @@ def box_stress(P: Params, wA, wB) -> np.ndarray:
 
 def time_steps(shape, steps: int) -> float:
     return lib().orc_time_steps(*shape, steps)
+
+
+# -- the rest of PsFFTBase and its consumers ----------------------------------
+def k2(shape, cell) -> np.ndarray:
+    """k^2 of PsGridField::calck2 (psbase/PsGridField.cpp:66-104)."""
+    out = np.empty(shape, dtype=np.float64)
+    lib().orc_k2(*shape, *cell, _p(out))
+    return out
+
+
+def _unary(name, data):
+    d = _f64(data)
+    out = np.empty_like(d)
+    if getattr(lib(), name)(*d.shape, _p(d), _p(out)):
+        raise ValueError("bad shape")
+    return out
+
+
+def forward_fft_abs(data) -> np.ndarray:
+    """|FFT(data)|^2 (psfft/PsFFTW.cpp:121-146)."""
+    return _unary("orc_fft_forward_abs", data)
+
+
+def forward_fft_re(data) -> np.ndarray:
+    """Re(FFT(data)) (psfft/PsFFTW.cpp:258-276)."""
+    return _unary("orc_fft_forward_re", data)
+
+
+def backward_fft_re(data) -> np.ndarray:
+    """Re(IFFT(data + 0i)), unnormalised (psfft/PsFFTW.cpp:278-296)."""
+    return _unary("orc_fft_backward_re", data)
+
+
+def convolve_re(d1, d2) -> np.ndarray:
+    """Re(IFFT(FFT(d1) FFT(d2))) (psfft/PsFFTW.cpp:148-188)."""
+    a, b = _f64(d1), _f64(d2)
+    out = np.empty_like(a)
+    lib().orc_fft_convolve_re(*a.shape, _p(a), _p(b), _p(out))
+    return out
+
+
+def scaled_fft_pair_im(data, kdata) -> np.ndarray:
+    """Re(IFFT(kdata FFT(0 + i data))) (psfft/PsFFTW.cpp:226-256)."""
+    a, k = _f64(data), _f64(kdata)
+    out = np.empty_like(a)
+    lib().orc_fft_scaled_pair_im(*a.shape, _p(a), _p(k), _p(out))
+    return out
+
+
+def spec_filter(w, cutoff: float, strength: float, cells=None, vwall: float = 0.0) -> np.ndarray:
+    """Filtered copy of w: PsSimpleSpecFilter (cells None) or PsMultiSpecFilter
+    (pseffhamil/PsSimpleSpecFilter.cpp:74-144, PsMultiSpecFilter.cpp:123-267,
+    PsSpecFilterUpdater.cpp:222-239)."""
+    out = _f64(w).copy()
+    c = None if cells is None else (ctypes.c_int * 3)(*[int(x) for x in cells])
+    r = lib().orc_spec_filter(*out.shape, _p(out), float(cutoff), float(strength), c, float(vwall))
+    if r:
+        raise ValueError("bad shape or spectral cell sizes")
+    return out
+
+
+def poisson_update(charge, psi, lB: float, cell):
+    """(shifted charge, new psi) of PsPoissonUpdater::update
+    (pseffhamil/PsPoissonUpdater.cpp:171-289)."""
+    c, p = _f64(charge).copy(), _f64(psi).copy()
+    lib().orc_poisson_update(*c.shape, *[float(x) for x in cell], float(lB), _p(c), _p(p))
+    return c, p

@@ -190,6 +190,28 @@ extern "C" void orc_kfac(int nx, int ny, int nz, double dx, double dy,
       }
 }
 
+extern "C" void orc_k2(int nx, int ny, int nz, double dx, double dy, double dz, double* k2) {
+  // psbase/PsGridField.cpp:66-104 (PsGridField::calck2), C order
+  const double twopi = 2.0 * std::acos(-1.0);
+  const int g[3] = {nx, ny, nz};
+  const double h[3] = {dx, dy, dz};
+  double n2[3];
+  for (int d = 0; d < 3; ++d) n2[d] = (double)g[d] / 2.0;
+  long n = 0;
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j)
+      for (int k = 0; k < nz; ++k) {
+        const int idx[3] = {i, j, k};
+        double kv[3];
+        for (int d = 2; d >= 0; --d) {
+          const double nk = n2[d] - std::fabs((double)idx[d] - n2[d]);
+          kv[d] = twopi * nk / (double)g[d];
+          kv[d] = kv[d] / h[d];
+        }
+        k2[n++] = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+      }
+}
+
 extern "C" int orc_fft3(int nx, int ny, int nz, double* re, double* im,
                         int sign) {
   if (nx < 1 || ny < 1 || nz < 1) return -1;
@@ -822,4 +844,154 @@ extern "C" double orc_time_steps(int nx, int ny, int nz, int steps) {
   orc_solve_q(nx, ny, nz, k.data(), w.data(), q0.data(), steps, sl.data());
   auto t1 = std::chrono::steady_clock::now();
   return std::chrono::duration<double>(t1 - t0).count();
+}
+
+/* ------------------------------------------------------------------------ */
+/* The rest of PsFFTBase and its consumers (spectral filters, Poisson).      */
+/* ------------------------------------------------------------------------ */
+namespace {
+std::vector<cd> forward_of_real(int nx, int ny, int nz, const double* data, bool imag) {
+  const long N = (long)nx * ny * nz;
+  std::vector<cd> a(N);
+  for (long n = 0; n < N; ++n) a[n] = imag ? cd(0.0, data[n]) : cd(data[n], 0.0);
+  fft3(a.data(), nx, ny, nz, false);
+  return a;
+}
+}  // namespace
+
+extern "C" int orc_fft_forward_abs(int nx, int ny, int nz, const double* data, double* res) {
+  // psfft/PsFFTW.cpp:121-146: forward, then re^2 + im^2 (the squared modulus)
+  if (nx < 1 || ny < 1 || nz < 1) return -1;
+  std::vector<cd> a = forward_of_real(nx, ny, nz, data, false);
+  for (size_t n = 0; n < a.size(); ++n)
+    res[n] = (a[n].real() * a[n].real()) + (a[n].imag() * a[n].imag());
+  return 0;
+}
+
+extern "C" int orc_fft_convolve_re(int nx, int ny, int nz, const double* d1, const double* d2,
+                                   double* res) {
+  // psfft/PsFFTW.cpp:148-188
+  if (nx < 1 || ny < 1 || nz < 1) return -1;
+  std::vector<cd> a = forward_of_real(nx, ny, nz, d1, false);
+  std::vector<cd> b = forward_of_real(nx, ny, nz, d2, false);
+  for (size_t n = 0; n < a.size(); ++n) {
+    const double re = (a[n].real() * b[n].real()) - (a[n].imag() * b[n].imag());
+    const double im = (a[n].imag() * b[n].real()) + (a[n].real() * b[n].imag());
+    a[n] = cd(re, im);
+  }
+  fft3(a.data(), nx, ny, nz, true);
+  for (size_t n = 0; n < a.size(); ++n) res[n] = a[n].real();
+  return 0;
+}
+
+extern "C" int orc_fft_scaled_pair_im(int nx, int ny, int nz, const double* data,
+                                      const double* kdata, double* res) {
+  // psfft/PsFFTW.cpp:226-256: the input is 0 + i data
+  if (nx < 1 || ny < 1 || nz < 1) return -1;
+  std::vector<cd> a = forward_of_real(nx, ny, nz, data, true);
+  for (size_t n = 0; n < a.size(); ++n) a[n] = cd(a[n].real() * kdata[n], a[n].imag() * kdata[n]);
+  fft3(a.data(), nx, ny, nz, true);
+  for (size_t n = 0; n < a.size(); ++n) res[n] = a[n].real();
+  return 0;
+}
+
+extern "C" int orc_fft_forward_re(int nx, int ny, int nz, const double* data, double* res) {
+  // psfft/PsFFTW.cpp:258-276 (calcForwardFFT)
+  if (nx < 1 || ny < 1 || nz < 1) return -1;
+  std::vector<cd> a = forward_of_real(nx, ny, nz, data, false);
+  for (size_t n = 0; n < a.size(); ++n) res[n] = a[n].real();
+  return 0;
+}
+
+extern "C" int orc_fft_backward_re(int nx, int ny, int nz, const double* data, double* res) {
+  // psfft/PsFFTW.cpp:278-296 (calcBackwardFFT)
+  if (nx < 1 || ny < 1 || nz < 1) return -1;
+  const long N = (long)nx * ny * nz;
+  std::vector<cd> a(N);
+  for (long n = 0; n < N; ++n) a[n] = cd(data[n], 0.0);
+  fft3(a.data(), nx, ny, nz, true);
+  for (long n = 0; n < N; ++n) res[n] = a[n].real();
+  return 0;
+}
+
+extern "C" int orc_spec_filter(int nx, int ny, int nz, double* w, double cutoff, double strength,
+                               const int* cells, double vwall) {
+  // pseffhamil/PsSimpleSpecFilter.cpp:74-144, PsMultiSpecFilter.cpp:123-267,
+  // PsSpecFilterUpdater.cpp:222-239 (one rank, FLOATTYPE double)
+  if (nx < 1 || ny < 1 || nz < 1) return -1;
+  if (cutoff <= 0.0) return 0;
+  const long N = (long)nx * ny * nz;
+  // subtractAverage: sequential sum (getSumAll), then w += -(sum / vol)
+  double tot = 0.0;
+  for (long n = 0; n < N; ++n) tot = tot + w[n];
+  const double localVol = (double)N - vwall;
+  const double wtot = -1.0 * (tot / (double)(1 * localVol));
+  for (long n = 0; n < N; ++n) w[n] += wtot;
+  std::vector<double> res(N), mask(N);
+  orc_fft_forward_abs(nx, ny, nz, w, res.data());
+  if (!cells) {
+    double maxVal = -1000000.00;
+    for (long n = 0; n < N; ++n) if (res[n] > maxVal) maxVal = res[n];
+    const double maxCutoff = cutoff * maxVal;
+    for (long n = 0; n < N; ++n) mask[n] = res[n] < maxCutoff ? strength : 1.0;
+  } else {
+    const int g[3] = {nx, ny, nz};
+    long nsc[3];
+    for (int d = 0; d < 3; ++d) {
+      if (cells[d] < 1 || g[d] % cells[d] != 0) return -2;
+      nsc[d] = g[d] / cells[d];
+    }
+    const long tot_cells = nsc[0] * nsc[1] * nsc[2];
+    // build_specCells_transpose on one rank: flat index in (y, x, z) order
+    std::vector<long> kcellMap(N);
+    for (long jloc = 0; jloc < ny; ++jloc)
+      for (long i = 0; i < nx; ++i)
+        for (long k = 0; k < nz; ++k) {
+          const long j = jloc;
+          const long ic0 = i / cells[0], ic1 = j / cells[1], ic2 = k / cells[2];
+          const long nn = ((jloc * nx) + i) * nz + k;
+          kcellMap[nn] = ic2 * (nsc[0] * nsc[1]) + ic1 * nsc[0] + ic0;
+        }
+    std::vector<double> cutTmp(tot_cells, 0.0);
+    for (long n = 0; n < N; ++n) {
+      const long r = kcellMap[n];
+      const double kCut = cutoff * res[n];
+      if (kCut > cutTmp[r]) cutTmp[r] = kCut;
+    }
+    for (long n = 0; n < N; ++n) mask[n] = res[n] < cutTmp[kcellMap[n]] ? strength : 1.0;
+  }
+  std::vector<double> out(N);
+  orc_scaled_fft_pair(nx, ny, nz, w, mask.data(), out.data());
+  const double scaleFFT = 1.0 / (double)N;
+  for (long n = 0; n < N; ++n) w[n] = scaleFFT * out[n];
+  return 0;
+}
+
+extern "C" int orc_poisson_update(int nx, int ny, int nz, double dx, double dy, double dz,
+                                  double lB, double* charge, double* psi) {
+  // pseffhamil/PsPoissonUpdater.cpp:171-289 (one rank, FLOATTYPE double)
+  if (nx < 1 || ny < 1 || nz < 1) return -1;
+  const long N = (long)nx * ny * nz;
+  std::vector<double> k2(N), kf(N);
+  orc_k2(nx, ny, nz, dx, dy, dz, k2.data());
+  for (long n = 0; n < N; ++n) kf[n] = k2[n] < 0.000000001 ? 0.0 : (4.0 * 3.1415 * lB) / k2[n];
+  double localC = 0.0;
+  for (long n = 0; n < N; ++n) localC = localC + charge[n];
+  localC = localC / ((double)1 * (double)N);
+  const double totalC = -1.0 * localC;
+  for (long n = 0; n < N; ++n) charge[n] += totalC;
+  std::vector<double> res(N);
+  orc_scaled_fft_pair(nx, ny, nz, charge, kf.data(), res.data());
+  const double scaleFFT = 1.0 / (double)N;
+  for (long n = 0; n < N; ++n) {
+    const double psi0 = psi[n];
+    const double psi1 = res[n] * scaleFFT;
+    const double frac = std::fabs((psi1 - psi0) / psi0);
+    if (frac > 0.2) res[n] = psi0 * 1.2;
+  }
+  for (long n = 0; n < N; ++n) {
+    const double tmp = (1.0 * scaleFFT) * res[n];
+    psi[n] = tmp;
+  }
+  return 0;
 }

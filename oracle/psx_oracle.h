@@ -153,6 +153,26 @@ int orc_box_stress(const orc_params* P, const double* wA, const double* wB,
 /* Time `steps` faithful contour steps on an nx*ny*nz grid (seconds). */
 double orc_time_steps(int nx, int ny, int nz, int steps);
 
+/* k^2 of PsGridField::calck2 (psbase/PsGridField.cpp:66-104), C order. */
+void orc_k2(int nx, int ny, int nz, double dx, double dy, double dz, double* k2);
+/* The rest of PsFFTBase (psfft/PsFFTW.cpp:121-296), unnormalised:
+ * |FFT(data)|^2; Re(IFFT(FFT(d1) FFT(d2))); Re(IFFT(kdata FFT(i data)));
+ * Re(FFT(data)); Re(IFFT(data)). */
+int orc_fft_forward_abs(int nx, int ny, int nz, const double* data, double* res);
+int orc_fft_convolve_re(int nx, int ny, int nz, const double* d1, const double* d2, double* res);
+int orc_fft_scaled_pair_im(int nx, int ny, int nz, const double* data, const double* kdata,
+                           double* res);
+int orc_fft_forward_re(int nx, int ny, int nz, const double* data, double* res);
+int orc_fft_backward_re(int nx, int ny, int nz, const double* data, double* res);
+/* Spectral filter in place (PsSimpleSpecFilter for cells == NULL, else
+ * PsMultiSpecFilter with spectral cells cells[3]); -2 if a cell size does
+ * not divide its grid size. */
+int orc_spec_filter(int nx, int ny, int nz, double* w, double cutoff, double strength,
+                    const int* cells, double vwall);
+/* PsPoissonUpdater::update in place on charge and psi. */
+int orc_poisson_update(int nx, int ny, int nz, double dx, double dy, double dz, double lB,
+                       double* charge, double* psi);
+
 #ifdef __cplusplus
 }
 #endif

@@ -60,6 +60,19 @@ EXPORTS = {
     "psx_fft_size": (ctypes.c_size_t, [_vp]),
     "psx_fft_scaled_pair": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "psx_fft_scaled_pair_host": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "psx_fft_forward_abs": (ctypes.c_int, [_vp, _vp, _vp]),
+    "psx_fft_convolve_re": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "psx_fft_scaled_pair_im": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "psx_fft_forward_re": (ctypes.c_int, [_vp, _vp, _vp]),
+    "psx_fft_backward_re": (ctypes.c_int, [_vp, _vp, _vp]),
+    "psx_fft_spec_filter": (ctypes.c_int, [_vp, _vp, ctypes.c_double, ctypes.c_double, _ip,
+                                           ctypes.c_double]),
+    "psx_fft_poisson_update": (ctypes.c_int, [_vp, _vp, _vp] + [ctypes.c_double] * 4),
+    "psx_fft_synchronize": (ctypes.c_int, [_vp]),
+    "psx_buffer_create": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(_vp)]),
+    "psx_buffer_destroy": (ctypes.c_int, [_vp]),
+    "psx_buffer_upload": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
+    "psx_buffer_download": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
     "psx_scft_create": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Diblock), ctypes.c_int,
                                        _vp, ctypes.POINTER(_vp)]),
     "psx_scft_create_blocks": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.c_int, ctypes.POINTER(BlockDesc),
@@ -74,6 +87,7 @@ EXPORTS = {
     "psx_scft_get_fields": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "psx_scft_set_fields_host": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
     "psx_scft_get_fields_host": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+    "psx_scft_spec_filter": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _ip]),
     "psx_scft_propagate": (ctypes.c_int, [_vp]),
     "psx_scft_Q": (ctypes.c_int, [_vp, _dp]),
     "psx_scft_densities": (ctypes.c_int, [_vp, _vp, _vp]),

@@ -157,10 +157,12 @@ k_pair_roundtrip(LineGeom g, const PairIO* __restrict__ fio) {
 
 // Complex line pass of the general FFT pair (any real kdata, full grid).
 struct CplxIO {
-  const void* in;     // complex (interleaved) T[2N], or real T[N] if in_real
-  void* out;          // complex T[2N], or real part only T[N] if out_real
+  const void* in;     // in_mode 0: complex (interleaved) T[2N]; 1: real part
+                      // T[N] (imaginary 0); 2: imaginary part T[N] (real 0)
+  void* out;          // out_mode 0: complex T[2N]; 1: real part T[N];
+                      // 2: squared modulus re^2 + im^2, T[N]
   const void* kmul;   // T[N] full-grid multiplier between fwd and inv, or null
-  int in_real, out_real, do_fwd, do_inv;
+  int in_mode, out_mode, do_fwd, do_inv;
 };
 
 template <typename T, bool CONTIG>
@@ -183,7 +185,8 @@ k_cplx_lines(LineGeom g, CplxIO f) {
     C v; v.x = (T)0; v.y = (T)0;
     if (l < g.nlines) {
       const int addr = CONTIG ? l * n + k : line_addr(l, k, n, g.inner);
-      if (f.in_real) v.x = ((const T*)f.in)[addr];
+      if (f.in_mode == 1) v.x = ((const T*)f.in)[addr];
+      else if (f.in_mode == 2) v.y = ((const T*)f.in)[addr];
       else v = ((const C*)f.in)[addr];
     }
     A[li * ls + k] = v;
@@ -221,7 +224,8 @@ k_cplx_lines(LineGeom g, CplxIO f) {
     if (l < g.nlines) {
       const int addr = CONTIG ? l * n + k : line_addr(l, k, n, g.inner);
       const C v = res[li * ls + k];
-      if (f.out_real) ((T*)f.out)[addr] = v.x;
+      if (f.out_mode == 1) ((T*)f.out)[addr] = v.x;
+      else if (f.out_mode == 2) ((T*)f.out)[addr] = v.x * v.x + v.y * v.y;
       else ((C*)f.out)[addr] = v;
     }
   }

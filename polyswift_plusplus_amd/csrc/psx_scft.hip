@@ -23,6 +23,7 @@
 #include "../../include/psx_scft.h"
 #include "psx_kernels.hpp"
 #include "psx_fft_reg.hpp"
+#include "psx_filter.hpp"
 
 using namespace psx;
 
@@ -377,6 +378,18 @@ struct psx_fft {
   std::vector<Axis> axes;
   void* work = nullptr;   // N complex of T
   void* hbuf[3] = {nullptr, nullptr, nullptr};  // device staging for the host API
+  // the spectral consumers (allocated on first use)
+  void* work2 = nullptr;           // second spectrum (convolveRe), N complex
+  void* rbuf = nullptr;            // N of T: |F|^2, then the filtered field
+  void* mbuf = nullptr;            // N of T: filter mask
+  void* kbuf = nullptr;            // N of T: Poisson multiplier 4 pi lB / k^2
+  double k2key[4] = {0, 0, 0, 0};  // (lB, dx, dy, dz) of kbuf
+  bool k2valid = false;
+  double* dsum = nullptr;          // reduction partials, then the sum
+  unsigned long long* rmax = nullptr;  // per-region maxima (bit patterns)
+  int* region = nullptr;           // spectral region of every cell
+  int region_cells[3] = {0, 0, 0};
+  int nregions = 0;
 };
 
 extern "C" void psx_last_error(char* buf, size_t len) {
@@ -413,8 +426,11 @@ extern "C" psx_status psx_fft_plan_create(int nx, int ny, int nz, psx_prec prec,
 
 extern "C" psx_status psx_fft_plan_destroy(psx_fft* p) {
   if (!p) return PSX_OK;
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
   for (auto& a : p->axes) { dfree(a.tw[0]); dfree(a.tw[1]); }
   dfree(p->work);
+  dfree(p->work2); dfree(p->rbuf); dfree(p->mbuf); dfree(p->kbuf);
+  dfree((void*&)p->dsum); dfree((void*&)p->rmax); dfree((void*&)p->region);
   for (auto& h : p->hbuf) dfree(h);
   if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
@@ -423,32 +439,35 @@ extern "C" psx_status psx_fft_plan_destroy(psx_fft* p) {
 
 extern "C" size_t psx_fft_size(const psx_fft* p) { return p ? (size_t)p->N : 0; }
 
+// in_mode 1: data is the real part of the input, 2: its imaginary part
+// (scaledFFTPairIm).
 template <typename T>
-static psx_status fft_pair_impl(psx_fft* p, const void* data, const void* kdata, void* res) {
+static psx_status fft_pair_impl(psx_fft* p, const void* data, const void* kdata, void* res,
+                                int in_mode = 1) {
   const int na = (int)p->axes.size();
   // forward along z, y (in order), then x with the k multiply and its inverse,
   // then the inverses back out; the last axis keeps only the real part.
   for (int i = 0; i < na; ++i) {
     CplxIO f{};
     f.in = (i == 0) ? data : p->work;
-    f.in_real = (i == 0);
+    f.in_mode = (i == 0) ? in_mode : 0;
     f.do_fwd = 1;
     const bool last = (i == na - 1);
     f.kmul = last ? kdata : nullptr;
     f.do_inv = last ? 1 : 0;
     f.out = (last && na == 1) ? res : p->work;
-    f.out_real = (last && na == 1);
+    f.out_mode = (last && na == 1) ? 1 : 0;
     PSX_TRY(launch_cplx<T>(p->axes[i], p->prec, f, p->stream));
   }
   for (int i = na - 2; i >= 0; --i) {
     CplxIO f{};
     f.in = p->work;
-    f.in_real = 0;
+    f.in_mode = 0;
     f.do_fwd = 0;
     f.do_inv = 1;
     f.kmul = nullptr;
     f.out = (i == 0) ? res : p->work;
-    f.out_real = (i == 0);
+    f.out_mode = (i == 0) ? 1 : 0;
     PSX_TRY(launch_cplx<T>(p->axes[i], p->prec, f, p->stream));
   }
   return PSX_OK;
@@ -472,6 +491,274 @@ extern "C" psx_status psx_fft_scaled_pair_host(psx_fft* p, const void* h_data,
   PSX_TRY(psx_fft_scaled_pair(p, p->hbuf[0], p->hbuf[1], p->hbuf[2]));
   PSX_HIP(hipMemcpyAsync(h_res, p->hbuf[2], bytes, hipMemcpyDeviceToHost, p->stream));
   PSX_HIP(hipStreamSynchronize(p->stream));
+  return PSX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The rest of PsFFTBase (psbase/PsFFTBase.h:83-139) and its consumers
+// ---------------------------------------------------------------------------
+
+// Unnormalised forward transform along every axis: the first pass reads `in`
+// (in_mode), the last writes `out` (out_mode), the passes between work in
+// place in p->work.
+template <typename T>
+static psx_status fft_passes(psx_fft* p, bool inverse, const void* in, int in_mode, void* out,
+                             int out_mode) {
+  const int na = (int)p->axes.size();
+  for (int j = 0; j < na; ++j) {
+    const int i = inverse ? na - 1 - j : j;
+    CplxIO f{};
+    f.in = j == 0 ? in : p->work;
+    f.in_mode = j == 0 ? in_mode : 0;
+    f.do_fwd = inverse ? 0 : 1;
+    f.do_inv = inverse ? 1 : 0;
+    f.kmul = nullptr;
+    const bool last = j == na - 1;
+    f.out = last ? out : p->work;
+    f.out_mode = last ? out_mode : 0;
+    PSX_TRY(launch_cplx<T>(p->axes[i], p->prec, f, p->stream));
+  }
+  return PSX_OK;
+}
+
+static psx_status plan_buf(psx_fft* p, void** b, size_t elems) {
+  if (!*b) PSX_TRY(dalloc(b, elems * prec_bytes(p->prec)));
+  return PSX_OK;
+}
+
+static int fft_red_blocks(const psx_fft* p) { return grid_1d(p->N); }
+
+static psx_status plan_sum_buf(psx_fft* p) {
+  if (!p->dsum) PSX_TRY(dalloc((void**)&p->dsum, (size_t)(fft_red_blocks(p) + 1) * sizeof(double)));
+  return PSX_OK;
+}
+
+// x <- x - sum(x) / vol on the plan's stream.
+template <typename T>
+static psx_status shift_by_mean(psx_fft* p, T* x, double vol) {
+  PSX_TRY(plan_sum_buf(p));
+  const int g = fft_red_blocks(p);
+  hipLaunchKernelGGL(k_partial_sum<T>, dim3(g), dim3(kThreads), 0, p->stream, (const T*)x, p->N, p->dsum);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kThreads), 0, p->stream, (const double*)p->dsum, g,
+                     p->dsum + g);
+  hipLaunchKernelGGL(k_shift_mean<T>, dim3(g), dim3(kThreads), 0, p->stream, x, p->N,
+                     (const double*)(p->dsum + g), vol);
+  PSX_HIP(hipGetLastError());
+  return PSX_OK;
+}
+
+// The spectral regions of PsMultiSpecFilter::build_specCells_transpose
+// (pseffhamil/PsMultiSpecFilter.cpp:165-215) as a serial run builds them:
+// the map is laid out in the transposed (y, x, z) order of the transpose
+// plan's grid, while the spectrum it is applied to comes from the plain plan
+// in (x, y, z) order, so cell n = (i ny + j) nz + k gets the region of the
+// transposed position with the same flat index.  On a square grid with
+// equal x and y cell sizes this only relabels the regions.
+static psx_status ensure_regions(psx_fft* p, const int* cells) {
+  if (p->region && cells[0] == p->region_cells[0] && cells[1] == p->region_cells[1] &&
+      cells[2] == p->region_cells[2])
+    return PSX_OK;
+  const int nx = p->dims[0], ny = p->dims[1], nz = p->dims[2];
+  const int dims[3] = {nx, ny, nz};
+  int nsc[3];
+  for (int d = 0; d < 3; ++d) {
+    if (cells[d] < 1 || dims[d] % cells[d] != 0)
+      return fail(PSX_ERR_INVALID, "specCellSizes must divide the grid (" + std::to_string(dims[d]) +
+                                       " cells, spectral cell " + std::to_string(cells[d]) + ")");
+    nsc[d] = dims[d] / cells[d];
+  }
+  std::vector<int> map((size_t)p->N);
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i)
+      for (int k = 0; k < nz; ++k) {
+        const size_t nn = ((size_t)j * nx + i) * nz + k;
+        map[nn] = (k / cells[2]) * (nsc[0] * nsc[1]) + (j / cells[1]) * nsc[0] + i / cells[0];
+      }
+  dfree((void*&)p->region);
+  dfree((void*&)p->rmax);
+  p->nregions = nsc[0] * nsc[1] * nsc[2];
+  PSX_TRY(upload((void**)&p->region, map.data(), map.size() * sizeof(int)));
+  PSX_TRY(dalloc((void**)&p->rmax, (size_t)p->nregions * sizeof(unsigned long long)));
+  for (int d = 0; d < 3; ++d) p->region_cells[d] = cells[d];
+  return PSX_OK;
+}
+
+// PsSpecFilterUpdater::subtractAverage then PsSimpleSpecFilter::applyFilter
+// (cells == nullptr) or PsMultiSpecFilter::applyFilter, in place on w.
+template <typename T>
+static psx_status spec_filter_impl(psx_fft* p, T* w, double cutoff, double strength,
+                                   const int* cells, double vwall) {
+  if (cutoff <= 0.0) return PSX_OK;   // PsSimpleSpecFilter.cpp:82-86
+  const long long N = p->N;
+  PSX_TRY(plan_buf(p, &p->rbuf, N));
+  PSX_TRY(plan_buf(p, &p->mbuf, N));
+  const int* reg = nullptr;
+  int nreg = 1;
+  if (cells) {
+    PSX_TRY(ensure_regions(p, cells));
+    reg = p->region;
+    nreg = p->nregions;
+  } else if (!p->rmax || p->nregions < 1) {
+    dfree((void*&)p->rmax);
+    PSX_TRY(dalloc((void**)&p->rmax, sizeof(unsigned long long)));
+    p->nregions = 1;
+    p->region_cells[0] = p->region_cells[1] = p->region_cells[2] = 0;
+  }
+  PSX_TRY(shift_by_mean<T>(p, w, (double)N - vwall));
+  PSX_TRY(fft_passes<T>(p, false, w, 1, p->rbuf, 2));   // forwardFFTAbs
+  const int g = fft_red_blocks(p);
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_1d(nreg)), dim3(kThreads), 0, p->stream, p->rmax,
+                     (long long)nreg, 0ull);
+  hipLaunchKernelGGL(k_region_max<T>, dim3(g), dim3(kThreads), 0, p->stream, (const T*)p->rbuf, reg, N,
+                     p->rmax);
+  hipLaunchKernelGGL(k_filter_mask<T>, dim3(g), dim3(kThreads), 0, p->stream, (const T*)p->rbuf, reg,
+                     (const unsigned long long*)p->rmax, (T)cutoff, (T)strength, (T*)p->mbuf, N);
+  PSX_HIP(hipGetLastError());
+  PSX_TRY(fft_pair_impl<T>(p, w, p->mbuf, p->rbuf));
+  const T scale = (T)(1.0 / (double)(T)N);
+  hipLaunchKernelGGL(k_scale_store<T>, dim3(g), dim3(kThreads), 0, p->stream, (const T*)p->rbuf, scale, w, N);
+  PSX_HIP(hipGetLastError());
+  return PSX_OK;
+}
+
+// 4 * 3.1415 * lB / k^2 (0 where k^2 < 1e-9) in C order, k^2 as
+// PsGridField::calck2 forms it (psbase/PsGridField.cpp:66-104);
+// PsPoissonUpdater::build_k2 (pseffhamil/PsPoissonUpdater.cpp:171-203).
+template <typename T>
+static psx_status ensure_k2inv(psx_fft* p, double lB, const double* h) {
+  const double key[4] = {lB, h[0], h[1], h[2]};
+  if (p->k2valid && std::memcmp(key, p->k2key, sizeof(key)) == 0) return PSX_OK;
+  const double twopi = 2.0 * std::acos(-1.0);
+  const int* n = p->dims;
+  double n2[3];
+  for (int d = 0; d < 3; ++d) n2[d] = (double)n[d] / 2.0;
+  std::vector<T> tab((size_t)p->N);
+  size_t idx = 0;
+  for (int i = 0; i < n[0]; ++i)
+    for (int j = 0; j < n[1]; ++j)
+      for (int k = 0; k < n[2]; ++k) {
+        const int pos[3] = {i, j, k};
+        double kv[3];
+        for (int d = 2; d >= 0; --d) {
+          const double nk = n2[d] - std::fabs((double)pos[d] - n2[d]);
+          kv[d] = twopi * nk / (double)n[d];
+          kv[d] = kv[d] / h[d];
+        }
+        const double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+        tab[idx++] = k2 < 0.000000001 ? (T)0 : (T)((4.0 * 3.1415 * lB) / k2);
+      }
+  dfree(p->kbuf);
+  p->k2valid = false;
+  PSX_TRY(upload(&p->kbuf, tab.data(), tab.size() * sizeof(T)));
+  std::memcpy(p->k2key, key, sizeof(key));
+  p->k2valid = true;
+  return PSX_OK;
+}
+
+template <typename T>
+static psx_status poisson_impl(psx_fft* p, T* charge, T* psi, double lB, const double* h) {
+  PSX_TRY(ensure_k2inv<T>(p, lB, h));
+  PSX_TRY(plan_buf(p, &p->rbuf, p->N));
+  PSX_TRY(shift_by_mean<T>(p, charge, (double)p->N));
+  PSX_TRY(fft_pair_impl<T>(p, charge, p->kbuf, p->rbuf));
+  const T scale = (T)(1.0 / (double)(T)p->N);
+  hipLaunchKernelGGL(k_poisson_psi<T>, dim3(fft_red_blocks(p)), dim3(kThreads), 0, p->stream,
+                     (const T*)p->rbuf, psi, scale, p->N);
+  PSX_HIP(hipGetLastError());
+  return PSX_OK;
+}
+
+#define PSX_FFT_DISPATCH(call_d, call_f) \
+  ((p->prec == PSX_FP64) ? (call_d) : (call_f))
+
+extern "C" psx_status psx_fft_forward_abs(psx_fft* p, const void* d_data, void* d_res) {
+  if (!p || !d_data || !d_res) return fail(PSX_ERR_INVALID, "null argument");
+  return PSX_FFT_DISPATCH(fft_passes<double>(p, false, d_data, 1, d_res, 2),
+                          fft_passes<float>(p, false, d_data, 1, d_res, 2));
+}
+
+extern "C" psx_status psx_fft_forward_re(psx_fft* p, const void* d_data, void* d_res) {
+  if (!p || !d_data || !d_res) return fail(PSX_ERR_INVALID, "null argument");
+  return PSX_FFT_DISPATCH(fft_passes<double>(p, false, d_data, 1, d_res, 1),
+                          fft_passes<float>(p, false, d_data, 1, d_res, 1));
+}
+
+extern "C" psx_status psx_fft_backward_re(psx_fft* p, const void* d_data, void* d_res) {
+  if (!p || !d_data || !d_res) return fail(PSX_ERR_INVALID, "null argument");
+  return PSX_FFT_DISPATCH(fft_passes<double>(p, true, d_data, 1, d_res, 1),
+                          fft_passes<float>(p, true, d_data, 1, d_res, 1));
+}
+
+extern "C" psx_status psx_fft_scaled_pair_im(psx_fft* p, const void* d_data, const void* d_kdata,
+                                             void* d_res) {
+  if (!p || !d_data || !d_kdata || !d_res) return fail(PSX_ERR_INVALID, "null argument");
+  return PSX_FFT_DISPATCH(fft_pair_impl<double>(p, d_data, d_kdata, d_res, 2),
+                          fft_pair_impl<float>(p, d_data, d_kdata, d_res, 2));
+}
+
+template <typename T>
+static psx_status convolve_impl(psx_fft* p, const void* d1, const void* d2, void* res) {
+  PSX_TRY(plan_buf(p, &p->work2, 2 * p->N));
+  PSX_TRY(fft_passes<T>(p, false, d2, 1, p->work2, 0));
+  PSX_TRY(fft_passes<T>(p, false, d1, 1, p->work, 0));
+  hipLaunchKernelGGL(k_spec_cmul<T>, dim3(fft_red_blocks(p)), dim3(kThreads), 0, p->stream, (T*)p->work,
+                     (const T*)p->work2, p->N);
+  PSX_HIP(hipGetLastError());
+  return fft_passes<T>(p, true, p->work, 0, res, 1);
+}
+
+extern "C" psx_status psx_fft_convolve_re(psx_fft* p, const void* d_data1, const void* d_data2,
+                                          void* d_res) {
+  if (!p || !d_data1 || !d_data2 || !d_res) return fail(PSX_ERR_INVALID, "null argument");
+  return PSX_FFT_DISPATCH(convolve_impl<double>(p, d_data1, d_data2, d_res),
+                          convolve_impl<float>(p, d_data1, d_data2, d_res));
+}
+
+extern "C" psx_status psx_fft_spec_filter(psx_fft* p, void* d_w, double cutoff_factor,
+                                          double filter_strength, const int* spec_cells,
+                                          double wall_volume) {
+  if (!p || !d_w) return fail(PSX_ERR_INVALID, "null argument");
+  if (!(wall_volume >= 0.0 && wall_volume < (double)p->N))
+    return fail(PSX_ERR_INVALID, "wall volume must lie in [0, N)");
+  return PSX_FFT_DISPATCH(
+      spec_filter_impl<double>(p, (double*)d_w, cutoff_factor, filter_strength, spec_cells, wall_volume),
+      spec_filter_impl<float>(p, (float*)d_w, cutoff_factor, filter_strength, spec_cells, wall_volume));
+}
+
+extern "C" psx_status psx_fft_poisson_update(psx_fft* p, void* d_charge, void* d_psi, double bjerrum_len,
+                                             double dx, double dy, double dz) {
+  if (!p || !d_charge || !d_psi) return fail(PSX_ERR_INVALID, "null argument");
+  if (!(dx > 0.0 && dy > 0.0 && dz > 0.0)) return fail(PSX_ERR_INVALID, "cell sizes must be > 0");
+  const double h[3] = {dx, dy, dz};
+  return PSX_FFT_DISPATCH(poisson_impl<double>(p, (double*)d_charge, (double*)d_psi, bjerrum_len, h),
+                          poisson_impl<float>(p, (float*)d_charge, (float*)d_psi, bjerrum_len, h));
+}
+
+extern "C" psx_status psx_fft_synchronize(psx_fft* p) {
+  if (!p) return fail(PSX_ERR_INVALID, "null plan");
+  PSX_HIP(hipStreamSynchronize(p->stream));
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_buffer_create(size_t bytes, void** d_out) {
+  if (!d_out) return fail(PSX_ERR_INVALID, "null argument");
+  return dalloc(d_out, bytes);
+}
+
+extern "C" psx_status psx_buffer_destroy(void* d) {
+  dfree(d);
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_buffer_upload(void* d_dst, const void* h_src, size_t bytes) {
+  if (!d_dst || !h_src) return fail(PSX_ERR_INVALID, "null argument");
+  PSX_HIP(hipMemcpy(d_dst, h_src, bytes, hipMemcpyHostToDevice));
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_buffer_download(void* h_dst, const void* d_src, size_t bytes) {
+  if (!h_dst || !d_src) return fail(PSX_ERR_INVALID, "null argument");
+  PSX_HIP(hipMemcpy(h_dst, d_src, bytes, hipMemcpyDeviceToHost));
   return PSX_OK;
 }
 
@@ -589,6 +876,7 @@ struct psx_scft {
   std::vector<CkptSeg> segs;
   std::vector<int> btab_axis;
   bool in_batch = false;                // owned by a psx_batch (its tables point at our slices)
+  psx_fft* fplan = nullptr;             // FP64 plan of the spectral filters (first use)
   int prec;
   std::vector<Axis> axes;
   int variant[2][3] = {{-1, -1, -1}, {-1, -1, -1}};   // line-kernel choice per precision, pass
@@ -1298,6 +1586,7 @@ extern "C" psx_status psx_scft_destroy(psx_scft* e) {
   if (!e) return PSX_OK;
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   drop_graphs(e);
+  psx_fft_plan_destroy(e->fplan);
   for (auto& s : e->sw) free_sweep(s);
   for (auto& a : e->axes) { dfree(a.tw[0]); dfree(a.tw[1]); }
   dfree((void*&)e->wA); dfree((void*&)e->wB); dfree((void*&)e->praw);
@@ -1739,6 +2028,25 @@ extern "C" psx_status psx_scft_set_fields_host(psx_scft* e, const double* wA, co
   return PSX_OK;
 }
 
+// Spectral filter updaters on the engine's conjugate fields, in place:
+// PsSimpleSpecFilter / PsMultiSpecFilter::update for updatefields naming the
+// A (bit 0) and B (bit 1) monomer densities (pseffhamil/PsSimpleSpecFilter.cpp:
+// 74-103).  The next iteration sees the filtered fields; densities and F of
+// the last iteration stay as they were.
+extern "C" psx_status psx_scft_spec_filter(psx_scft* e, int fields, double cutoff_factor,
+                                           double filter_strength, const int* spec_cells) {
+  if (!e) return fail(PSX_ERR_INVALID, "null engine");
+  if (fields < 0 || fields > 3) return fail(PSX_ERR_INVALID, "fields is a mask of 1 (w_A) and 2 (w_B)");
+  if (e->in_batch) return fail(PSX_ERR_STATE, "engine belongs to a batch");
+  if (!e->fplan) PSX_TRY(psx_fft_plan_create(e->dims[0], e->dims[1], e->dims[2], PSX_FP64, e->stream, &e->fplan));
+  if (fields & 1) PSX_TRY(psx_fft_spec_filter(e->fplan, e->wA, cutoff_factor, filter_strength, spec_cells, 0.0));
+  if (fields & 2) PSX_TRY(psx_fft_spec_filter(e->fplan, e->wB, cutoff_factor, filter_strength, spec_cells, 0.0));
+  e->sw[0].wfac_valid = false;
+  e->sw[1].wfac_valid = false;
+  e->swept = false;
+  return PSX_OK;
+}
+
 static psx_status get_fields_impl(psx_scft* e, double* wA, double* wB, double* p, hipMemcpyKind k) {
   const size_t fb = e->N * sizeof(double);
   if (wA) PSX_HIP(hipMemcpyAsync(wA, e->wA, fb, k, e->stream));
@@ -2061,7 +2369,7 @@ static psx_status stress_step(psx_scft* e, const T* f, const T* gg, const BlockI
   for (auto& a : e->caxes) {
     CplxIO io{};
     io.in = e->zbuf; io.out = e->zbuf; io.kmul = nullptr;
-    io.in_real = 0; io.out_real = 0; io.do_fwd = 1; io.do_inv = 0;
+    io.in_mode = 0; io.out_mode = 0; io.do_fwd = 1; io.do_inv = 0;
     PSX_TRY(launch_cplx<double>(a, PSX_FP64, io, e->stream));
   }
   StressArgs sa;

@@ -32,7 +32,13 @@ Mapping to the engine (``configure``):
 * chiN = chi * length of the first Polymer (its scale length,
   pseffhamil/PsFloryInteraction.cpp:160-161);
 * the ``steepestDescent`` Updater's ``relaxlambdas`` and ``noise``
-  (pseffhamil/PsSteepDUpdater.cpp:56-63); ``nsteps``, ``randomSeed``.
+  (pseffhamil/PsSteepDUpdater.cpp:56-63); ``nsteps``, ``randomSeed``;
+* ``simpleSpecFilter`` / ``multiSpecFilter`` Updaters after it in the
+  EffHamil's ``updaterSequence`` (pseffhamil/PsEffHamil.cpp:185-203): their
+  ``updatefields``, ``cutoffFactor``, ``filterStrength``, ``specCellSizes``
+  and the ``applyStart`` / ``applyFrequency`` / ``applyEnd`` gate
+  (psbase/PsUpdater.cpp:60-118), applied to the engine's conjugate fields
+  after the field update of each step they are due at.
 
 Anything else that changes the physics (solvents, several polymers,
 charged or polydisperse blocks, walls, chi functions) raises ``DeckError``.
@@ -40,12 +46,15 @@ charged or polydisperse blocks, walls, chi functions) raises ``DeckError``.
 from __future__ import annotations
 
 import ast
+import dataclasses
 import math
 import os
 import re
 from dataclasses import dataclass, field
 
 import numpy as np
+
+from .updaters import SpecFilterUpdater, UpdaterSchedule
 
 
 class DeckError(ValueError):
@@ -376,6 +385,7 @@ class MeltConfig:
     history_period: int | None
     constraint_field: str = "defaultPressure"
     dump_period: int | None = None
+    filters: list = field(default_factory=list)   # SpecFilterUpdater, in sequence order
 
 
 _SUPPORTED_TOP = {"Grid", "Decomp", "Comm", "FFT", "PhysField", "EffHamil", "Polymer", "History",
@@ -429,13 +439,52 @@ def configure(root: Node) -> MeltConfig:
         raise DeckError("flory scfields must name two fields")
     chiN = float(inter.get("chi")) * length
     ups = ham.all("Updater")
-    if len(ups) != 1 or ups[0].get("kind") != "steepestDescent":
+    sds = [u for u in ups if u.get("kind") == "steepestDescent"]
+    if len(sds) != 1:
         raise DeckError("one steepestDescent Updater expected")
-    lam = ups[0].get("relaxlambdas", [0.1, 0.0])
+    sd = sds[0]
+    lam = sd.get("relaxlambdas", [0.1, 0.0])
     if not isinstance(lam, list):
         lam = [lam]
     lam = (float(lam[0]), float(lam[1]) if len(lam) > 1 else 0.0)
-    noise = float(ups[0].get("noise", 0.0))
+    noise = float(sd.get("noise", 0.0))
+    seq = ham.get("updaterSequence", [u.name for u in ups])
+    seq = seq if isinstance(seq, list) else [seq]
+    by_name = {u.name: u for u in ups}
+    filters = []
+    for pos, nm in enumerate(seq):
+        u = by_name.get(nm)
+        if u is None:
+            raise DeckError(f"updaterSequence names an undefined Updater {nm!r}")
+        if u is sd:
+            continue
+        kind = u.get("kind")
+        if kind == "poisson" or kind == "poissonUpdater":
+            raise DeckError("the Poisson updater needs charged species, outside the melt subset")
+        if kind not in ("simpleSpecFilter", "multiSpecFilter"):
+            raise DeckError(f"Updater kind {kind!r} unsupported")
+        if pos < seq.index(sd.name):
+            raise DeckError(f"spectral filter {nm!r} before the field update in updaterSequence "
+                            "is unsupported")
+        if u.get("fftKind") is None:
+            raise DeckError(f"Updater {nm!r}: fftKind not set")
+        ufields = u.get("updatefields")
+        ufields = ufields if isinstance(ufields, list) else [ufields]
+        which = ""
+        for fname in ufields:
+            if fname not in fields:
+                raise DeckError(f"Updater {nm!r}: updatefield {fname!r} is not a flory scfield")
+            which += "A" if fname == fields[0] else "B"
+        cells = None
+        if kind == "multiSpecFilter":
+            cells = u.get("specCellSizes")
+            if not isinstance(cells, list):
+                raise DeckError(f"Updater {nm!r}: specCellSizes not set")
+            cells = tuple(int(c) for c in cells) + (1,) * (3 - len(cells))
+        sched = UpdaterSchedule(int(u.get("applyStart", 0)), int(u.get("applyFrequency", 1)),
+                                None if u.get("applyEnd") is None else int(u.get("applyEnd")))
+        filters.append(SpecFilterUpdater(which, float(u.get("cutoffFactor", 0.0)),
+                                         float(u.get("filterStrength", 0.0)), cells, sched, nm))
 
     bnodes = poly.all("Block")
     if not bnodes:
@@ -480,7 +529,7 @@ def configure(root: Node) -> MeltConfig:
                       chiN=chiN, lambdas=lam, noise=noise, nsteps=int(root.get("nsteps", 1)),
                       seed=None if seed is None else int(seed), species_fields=tuple(fields),
                       history_period=period, constraint_field=cons[0] if cons else "defaultPressure",
-                      dump_period=None if dump is None else int(dump))
+                      dump_period=None if dump is None else int(dump), filters=filters)
 
 
 def build(cfg: MeltConfig, prec="fp64", stream=None):
@@ -522,17 +571,26 @@ def run(deck, iters: int | None = None, prec="fp64", sigma: float = 0.1, fields_
     n = int(iters or cfg.nsteps)
     mix = Mixing("steepest", *cfg.lambdas)
     period = cfg.dump_period if prefix else None
+    filters = [dataclasses.replace(f, schedule=f.schedule.resolve(cfg.nsteps, step0))
+               for f in cfg.filters]
     hist = {"F": [], "Q": [], "err": []}
     t, done = step0, 0
     while done < n:
         chunk = n - done
         if period:
             chunk = min(chunk, period - t % period)
+        for f in filters:
+            nd = f.schedule.next_due(t)
+            if nd is not None:
+                chunk = min(chunk, nd - t)
         h = e.iterate(chunk, mix, tol=tol, use_graph=use_graph)
         for k in hist:
             hist[k].extend(h[k])
         t += h["iters"]
         done += h["iters"]
+        if h["iters"] == chunk:
+            for f in filters:          # after the field update, in updaterSequence order
+                f.update(e, t)
         if period and t % period == 0:
             physfield.dump(e, prefix, t, names)
         if h["iters"] < chunk:     # converged

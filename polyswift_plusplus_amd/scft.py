@@ -75,6 +75,76 @@ class PsFFT:
         """Device pointers (e.g. torch ``tensor.data_ptr()``); asynchronous."""
         check(self._L.psx_fft_scaled_pair(self._h, d_data, d_kdata, d_res))
 
+    # -- the rest of PsFFTBase, host arrays in and out through device buffers --
+    def _call(self, fn, arrays, n_out=1, extra=()):
+        """Upload `arrays`, run fn(plan, *inputs, *outputs, *extra) on the
+        device, and return the outputs (and the inputs, which some calls
+        change in place) as host arrays of the grid's shape."""
+        n = self.getFFTSize()
+        ins = [np.ascontiguousarray(a, dtype=self.dtype).reshape(-1) for a in arrays]
+        if any(a.size != n for a in ins):
+            raise PsxError(2, "array size differs from the FFT size")
+        nbytes = n * np.dtype(self.dtype).itemsize
+        bufs = []
+        try:
+            for _ in range(len(ins) + n_out):
+                d = ctypes.c_void_p()
+                check(self._L.psx_buffer_create(nbytes, ctypes.byref(d)))
+                bufs.append(d)
+            for d, a in zip(bufs, ins):
+                check(self._L.psx_buffer_upload(d, a.ctypes.data, nbytes))
+            check(fn(self._h, *bufs, *extra))
+            check(self._L.psx_fft_synchronize(self._h))
+            outs = []
+            for d in bufs:
+                o = np.empty(n, dtype=self.dtype)
+                check(self._L.psx_buffer_download(o.ctypes.data, d, nbytes))
+                outs.append(o.reshape(self.shape))
+            return outs
+        finally:
+            for d in bufs:
+                self._L.psx_buffer_destroy(d)
+
+    def forwardFFTAbs(self, data) -> np.ndarray:
+        """|FFT(data)|^2 (the squared modulus, as PsFFTW::forwardFFTAbs)."""
+        return self._call(self._L.psx_fft_forward_abs, [data])[-1]
+
+    def convolveRe(self, data1, data2) -> np.ndarray:
+        """Re(IFFT(FFT(data1) FFT(data2))), unnormalised."""
+        return self._call(self._L.psx_fft_convolve_re, [data1, data2])[-1]
+
+    def scaledFFTPairIm(self, data, kdata) -> np.ndarray:
+        """Re(IFFT(kdata FFT(0 + i data))), unnormalised."""
+        return self._call(self._L.psx_fft_scaled_pair_im, [data, kdata])[-1]
+
+    def calcForwardFFT(self, data) -> np.ndarray:
+        """Re(FFT(data)), unnormalised."""
+        return self._call(self._L.psx_fft_forward_re, [data])[-1]
+
+    def calcBackwardFFT(self, data) -> np.ndarray:
+        """Re(IFFT(data + 0i)), unnormalised."""
+        return self._call(self._L.psx_fft_backward_re, [data])[-1]
+
+    def specFilter(self, w, cutoffFactor: float, filterStrength: float, specCellSizes=None,
+                   wallVolume: float = 0.0) -> np.ndarray:
+        """The filtered field (PsSimpleSpecFilter, or PsMultiSpecFilter with
+        specCellSizes): mean removed, spectral mask, FFT pair / N."""
+        cells = None if specCellSizes is None else (ctypes.c_int * 3)(*[int(c) for c in specCellSizes])
+
+        def fn(h, d):
+            return self._L.psx_fft_spec_filter(h, d, float(cutoffFactor), float(filterStrength), cells,
+                                               float(wallVolume))
+        return self._call(fn, [w], n_out=0)[0]
+
+    def poissonUpdate(self, charge, psi, bjerrumLen: float, cell):
+        """(charge with its mean removed, new psi) of PsPoissonUpdater::update."""
+        dx, dy, dz = (float(c) for c in cell)
+
+        def fn(h, dc, dp):
+            return self._L.psx_fft_poisson_update(h, dc, dp, float(bjerrumLen), dx, dy, dz)
+        c, p = self._call(fn, [charge, psi], n_out=0)
+        return c, p
+
     def close(self):
         if getattr(self, "_h", None):
             self._L.psx_fft_plan_destroy(self._h)
@@ -144,6 +214,15 @@ class DiblockSCFT:
 
     def setPrecision(self, prec) -> None:
         check(self._L.psx_scft_set_precision(self._h, _prec(prec)))
+
+    def specFilter(self, fields: str, cutoffFactor: float, filterStrength: float,
+                   specCellSizes=None) -> None:
+        """Spectral filter updater on the conjugate fields in place: fields
+        "A", "B" or "AB" (PsSimpleSpecFilter, or PsMultiSpecFilter with
+        specCellSizes)."""
+        mask = (1 if "A" in fields else 0) | (2 if "B" in fields else 0)
+        cells = None if specCellSizes is None else (ctypes.c_int * 3)(*[int(c) for c in specCellSizes])
+        check(self._L.psx_scft_spec_filter(self._h, mask, float(cutoffFactor), float(filterStrength), cells))
 
     # -- polymer: PsBlockCopolymer::update split into its reference steps ----
     def solveQ(self) -> None:

@@ -123,10 +123,10 @@ def test_reference_example_with_expressions():
 
 @needs_ref
 @pytest.mark.parametrize("name", ["chargedAB.pre", "abSolventMix.pre", "abHomopMix.pre",
-                                  "polydBulk.pre", "multispecf.pre", "pspi.pre", "tri3abc.pre"])
+                                  "polydBulk.pre", "pspi.pre", "tri3abc.pre"])
 def test_reference_decks_outside_the_subset_raise(name):
-    """Decks with solvents, blends, charges, polydispersity, filters or three
-    species parse, then stop with a DeckError naming the feature."""
+    """Decks with solvents, blends, charges, polydispersity, chi functions or
+    three species parse, then stop with a DeckError naming the feature."""
     path = os.path.join(REF_DECKS, name)
     try:
         root = deck.load(path)
@@ -134,3 +134,20 @@ def test_reference_decks_outside_the_subset_raise(name):
         return
     with pytest.raises(DeckError):
         deck.configure(root)
+
+
+@needs_ref
+def test_reference_spectral_filter_decks():
+    """multispecf.pre: steepest descent, then a multiSpecFilter on both
+    fields every 100 steps from step 200 (pstests/tests/diblock/multispecf.pre:
+    73-103)."""
+    cfg = deck.configure(deck.load(os.path.join(REF_DECKS, "multispecf.pre")))
+    assert cfg.shape == (128, 128, 1) and cfg.chiN == pytest.approx(18.0)
+    assert cfg.lambdas == (0.2, 0.1) and cfg.noise == 0.02
+    (f,) = cfg.filters
+    assert (f.fields, f.cutoffFactor, f.filterStrength, f.specCellSizes) == ("AB", 0.6, 0.0, (4, 4, 1))
+    s = f.schedule.resolve(cfg.nsteps)
+    assert [t for t in range(1, cfg.nsteps + 1) if s.due(t)] == [200, 300, 400]
+    path = os.path.join(REF, "psexamples", "PSimBase", "spectralFilter", "spectralFilter.pre")
+    ex = deck.configure(deck.load(path))
+    assert len(ex.filters) == 1 and ex.filters[0].specCellSizes is not None
