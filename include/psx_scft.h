@@ -200,6 +200,27 @@ psx_status psx_scft_set_fields_host(psx_scft* e, const double* h_wA,
                                     const double* h_wB, const double* h_p);
 psx_status psx_scft_get_fields_host(psx_scft* e, double* h_wA, double* h_wB,
                                     double* h_p);
+/* The reference's host random stream theRng (PsRandom, psstd/PsRandom.h:
+ * 147-170,305-373, psstd/PsRandom.cpp:58-95): MT19937 seeded by
+ * init_genrand(seed mod 2^32) (PsDomain seeds it with randomSeed + rank,
+ * pstrol/PsDomain.cpp:213); uniform = psRandom (y / 2^32), gaussian =
+ * psRandomGaussian (Marsaglia polar, the second deviate kept for the next
+ * call).  Host arrays; one stream may feed the initial fields and an
+ * engine's noise, in the reference's order. */
+typedef struct psx_rng psx_rng;
+psx_status psx_rng_create(unsigned long long seed, psx_rng** out);
+psx_status psx_rng_destroy(psx_rng* rng);
+psx_status psx_rng_uniform(psx_rng* rng, long long n, double* h_out);
+psx_status psx_rng_gaussian(psx_rng* rng, long long n, double* h_out);
+/* Steepest-descent noise (PsPolymerUpdater::addRandUpdateConjgFields,
+ * pseffhamil/PsPolymerUpdater.cpp:264-295, called from PsSteepDUpdater at
+ * pseffhamil/PsSteepDUpdater.cpp:283): every steepest-descent step draws 2N
+ * deviates from rng (w_A's cells, then w_B's) and adds strength * g to the
+ * updated fields before the pressure and F.  rng NULL: no noise (and no
+ * draws); strength 0 with an rng draws and adds nothing, as the reference.
+ * The engine keeps the pointer: rng must outlive its use.  Anderson
+ * mixing with noise on is PSX_ERR_INVALID. */
+psx_status psx_scft_set_noise(psx_scft* e, double strength, psx_rng* rng);
 /* Spectral filter updater (PsSimpleSpecFilter / PsMultiSpecFilter::update,
  * pseffhamil/PsSimpleSpecFilter.cpp:74-103) on the engine's conjugate
  * fields in place: fields is a mask of 1 (w_A) and 2 (w_B); the other

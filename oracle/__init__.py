@@ -157,6 +157,17 @@ def lib():
         L.orc_spec_filter.argtypes = [ctypes.c_int] * 3 + [_dp, ctypes.c_double, ctypes.c_double,
                                                            ctypes.POINTER(ctypes.c_int), ctypes.c_double]
         L.orc_poisson_update.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 4 + [_dp] * 2
+        L.orc_rng_create.restype = ctypes.c_void_p
+        L.orc_rng_create.argtypes = [ctypes.c_ulong]
+        L.orc_rng_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_rng_destroy.restype = None
+        L.orc_rng_raw32.argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.POINTER(ctypes.c_uint)]
+        L.orc_rng_raw32.restype = None
+        for nm in ("orc_rng_uniform", "orc_rng_gaussian"):
+            getattr(L, nm).argtypes = [ctypes.c_void_p, ctypes.c_long, _dp]
+            getattr(L, nm).restype = None
+        L.orc_scft_run_noise.argtypes = ([ctypes.POINTER(OrcParams)] + [_dp] * 3 + [ctypes.c_int] +
+                                         [_dp] * 3 + [ctypes.c_double, ctypes.c_void_p])
         _lib = L
     return _lib
 
@@ -398,3 +409,47 @@ def poisson_update(charge, psi, lB: float, cell):
     c, p = _f64(charge).copy(), _f64(psi).copy()
     lib().orc_poisson_update(*c.shape, *[float(x) for x in cell], float(lB), _p(c), _p(p))
     return c, p
+
+
+# -- the reference's random stream ----------------------------------------------
+class Rng:
+    """PsRandom (psstd/PsRandom.h:147-170,305-373; PsRandom.cpp:58-95):
+    MT19937 seeded by init_genrand, uniform y / 2^32, Marsaglia polar
+    Gaussians."""
+
+    def __init__(self, seed: int):
+        self._h = lib().orc_rng_create(int(seed))
+
+    def raw32(self, n: int) -> np.ndarray:
+        out = np.empty(int(n), dtype=np.uint32)
+        lib().orc_rng_raw32(self._h, int(n), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)))
+        return out
+
+    def uniform(self, n: int) -> np.ndarray:
+        out = np.empty(int(n))
+        lib().orc_rng_uniform(self._h, int(n), _p(out))
+        return out
+
+    def gaussian(self, n: int) -> np.ndarray:
+        out = np.empty(int(n))
+        lib().orc_rng_gaussian(self._h, int(n), _p(out))
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_rng_destroy(self._h)
+            self._h = None
+
+
+def scft_run_noise(P: Params, wA, wB, p=None, n_iters=1, strength=0.0, rng: Rng = None):
+    """Steepest descent with the reference's noise term drawn from rng
+    (pseffhamil/PsPolymerUpdater.cpp:264-295)."""
+    wA, wB = _f64(wA).copy(), _f64(wB).copy()
+    p = np.zeros_like(wA) if p is None else _f64(p).copy()
+    F = np.zeros(n_iters); Qh = np.zeros(n_iters); err = np.zeros(n_iters)
+    cp = P.c()
+    done = lib().orc_scft_run_noise(ctypes.byref(cp), _p(wA), _p(wB), _p(p), n_iters, _p(F), _p(Qh),
+                                    _p(err), float(strength), rng._h)
+    if done < 0:
+        raise RuntimeError("oracle scft_run_noise failed")
+    return dict(wA=wA, wB=wB, p=p, F=F, Q=Qh, err=err, iters=done)

@@ -995,3 +995,129 @@ extern "C" int orc_poisson_update(int nx, int ny, int nz, double dx, double dy, 
   }
   return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* The reference's random stream and the noisy steepest descent.            */
+/* ------------------------------------------------------------------------ */
+// MT19937 (Matsumoto & Nishimura 1998) as PsRandom runs it
+// (psstd/PsRandom.cpp:58-95, psstd/PsRandom.h:147-170,305-373): state
+// words, one block regeneration per 624 outputs, tempering; real() = y/2^32;
+// psRandomGaussian = Marsaglia polar, v2*fac returned, v1*fac kept.
+struct orc_rng {
+  unsigned long st[624];
+  int left = 1;
+  int pos = 0;
+  int iset = 0;
+  double gset = 0.0;
+};
+
+namespace {
+void orc_mt_regen(orc_rng* g) {
+  const unsigned long UP = 0x80000000UL, LO = 0x7fffffffUL, A = 0x9908b0dfUL;
+  for (int i = 0; i < 624; ++i) {
+    const unsigned long u = g->st[i], v = g->st[(i + 1) % 624];
+    const unsigned long mix = (u & UP) | (v & LO);
+    g->st[i] = g->st[(i + 397) % 624] ^ (mix >> 1) ^ ((v & 1UL) ? A : 0UL);
+  }
+  g->pos = 0;
+  g->left = 624;
+}
+
+unsigned long orc_mt_next(orc_rng* g) {
+  if (--g->left == 0) orc_mt_regen(g);
+  unsigned long y = g->st[g->pos++];
+  y ^= (y >> 11);
+  y ^= (y << 7) & 0x9d2c5680UL;
+  y ^= (y << 15) & 0xefc60000UL;
+  y ^= (y >> 18);
+  return y & 0xffffffffUL;
+}
+
+double orc_mt_real(orc_rng* g) { return (double)(long)orc_mt_next(g) * (1.0 / 4294967296.0); }
+
+double orc_mt_gauss(orc_rng* g) {
+  if (g->iset != 0) {
+    g->iset = 0;
+    return g->gset;
+  }
+  double v1, v2, r;
+  do {
+    double u = orc_mt_real(g);
+    v1 = (2.0 * u) - 1.0;
+    u = orc_mt_real(g);
+    v2 = (2.0 * u) - 1.0;
+    r = (v1 * v1) + (v2 * v2);
+  } while ((r >= 1.0) || (r == 0.0));
+  const double fac = sqrt(-2.0 * log(r) / r);
+  g->gset = v1 * fac;
+  g->iset = 1;
+  return v2 * fac;
+}
+}  // namespace
+
+extern "C" orc_rng* orc_rng_create(unsigned long seed) {
+  orc_rng* g = new orc_rng();
+  g->st[0] = seed & 0xffffffffUL;
+  for (int j = 1; j < 624; ++j)
+    g->st[j] = (1812433253UL * (g->st[j - 1] ^ (g->st[j - 1] >> 30)) + (unsigned long)j) & 0xffffffffUL;
+  g->left = 1;
+  g->pos = 0;
+  return g;
+}
+
+extern "C" void orc_rng_destroy(orc_rng* g) { delete g; }
+
+extern "C" void orc_rng_raw32(orc_rng* g, long n, unsigned int* out) {
+  for (long i = 0; i < n; ++i) out[i] = (unsigned int)orc_mt_next(g);
+}
+
+extern "C" void orc_rng_uniform(orc_rng* g, long n, double* out) {
+  for (long i = 0; i < n; ++i) out[i] = orc_mt_real(g);
+}
+
+extern "C" void orc_rng_gaussian(orc_rng* g, long n, double* out) {
+  for (long i = 0; i < n; ++i) out[i] = orc_mt_gauss(g);
+}
+
+extern "C" int orc_scft_run_noise(const orc_params* P, double* wA, double* wB, double* p,
+                                  int n_iters, double* F_hist, double* Q_hist, double* err_hist,
+                                  double strength, orc_rng* g) {
+  // Steepest descent with PsPolymerUpdater::addRandUpdateConjgFields
+  // (pseffhamil/PsPolymerUpdater.cpp:264-295) after the lambda update
+  // (pseffhamil/PsSteepDUpdater.cpp:255-283): per step N deviates for w_A's
+  // noise field, then N for w_B's, each value (0 + strength g) * 1.
+  if (P->mix != 0) return -1;
+  const long N = ncells(P);
+  std::vector<double> phiA(N), phiB(N), nA(N), nB(N);
+  double fA, fB;
+  orc_species_fracs(P, &fA, &fB);
+  for (int it = 0; it < n_iters; ++it) {
+    double Q;
+    int rc = P->nblocks > 0 ? orc_graph_densities(P, wA, wB, phiA.data(), phiB.data(), &Q, nullptr)
+                            : orc_diblock_densities(P, wA, wB, phiA.data(), phiB.data(), &Q, nullptr);
+    if (rc) return -1;
+    double ff = 0.0;
+    std::vector<double> dHA(N), dHB(N);
+    for (long r = 0; r < N; ++r) {
+      dHA[r] = p[r] - wA[r] + P->chiN * (phiB[r] - fB);
+      dHB[r] = p[r] - wB[r] + P->chiN * (phiA[r] - fA);
+      ff += dHA[r] * dHA[r] + dHB[r] * dHB[r];
+    }
+    for (long r = 0; r < N; ++r) {
+      wA[r] += (dHA[r] + (-1.0 * P->lambda1) * dHB[r]) * P->lambda0;
+      wB[r] += (dHB[r] + (-1.0 * P->lambda1) * dHA[r]) * P->lambda0;
+    }
+    for (long r = 0; r < N; ++r) nA[r] = (0.0 + strength * orc_mt_gauss(g)) * 1.0;
+    for (long r = 0; r < N; ++r) nB[r] = (0.0 + strength * orc_mt_gauss(g)) * 1.0;
+    for (long r = 0; r < N; ++r) { wA[r] += nA[r]; wB[r] += nB[r]; }
+    double sum = 0.0;
+    for (long r = 0; r < N; ++r) { p[r] = (wA[r] + wB[r]) * 0.5; sum += p[r]; }
+    const double avg = sum / (double)N;
+    for (long r = 0; r < N; ++r) p[r] -= avg;
+    const double F = orc_free_energy(P, N, wA, wB, phiA.data(), phiB.data(), Q);
+    if (F_hist) F_hist[it] = F;
+    if (Q_hist) Q_hist[it] = Q;
+    if (err_hist) err_hist[it] = std::sqrt(ff / (double)N);
+  }
+  return n_iters;
+}

@@ -169,6 +169,18 @@ int orc_fft_backward_re(int nx, int ny, int nz, const double* data, double* res)
  * not divide its grid size. */
 int orc_spec_filter(int nx, int ny, int nz, double* w, double cutoff, double strength,
                     const int* cells, double vwall);
+/* The reference's random stream (PsRandom: MT19937 with init_genrand
+ * seeding, real() = y / 2^32, Marsaglia polar Gaussians). */
+typedef struct orc_rng orc_rng;
+orc_rng* orc_rng_create(unsigned long seed);
+void orc_rng_destroy(orc_rng* g);
+void orc_rng_raw32(orc_rng* g, long n, unsigned int* out);
+void orc_rng_uniform(orc_rng* g, long n, double* out);
+void orc_rng_gaussian(orc_rng* g, long n, double* out);
+/* Steepest descent (P->mix == 0) with the noise term drawn from g. */
+int orc_scft_run_noise(const orc_params* P, double* wA, double* wB, double* p, int n_iters,
+                       double* F_hist, double* Q_hist, double* err_hist, double strength,
+                       orc_rng* g);
 /* PsPoissonUpdater::update in place on charge and psi. */
 int orc_poisson_update(int nx, int ny, int nz, double dx, double dy, double dz, double lB,
                        double* charge, double* psi);

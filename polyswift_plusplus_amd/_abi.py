@@ -88,6 +88,11 @@ EXPORTS = {
     "psx_scft_set_fields_host": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
     "psx_scft_get_fields_host": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
     "psx_scft_spec_filter": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _ip]),
+    "psx_rng_create": (ctypes.c_int, [ctypes.c_ulonglong, ctypes.POINTER(_vp)]),
+    "psx_rng_destroy": (ctypes.c_int, [_vp]),
+    "psx_rng_uniform": (ctypes.c_int, [_vp, ctypes.c_longlong, _dp]),
+    "psx_rng_gaussian": (ctypes.c_int, [_vp, ctypes.c_longlong, _dp]),
+    "psx_scft_set_noise": (ctypes.c_int, [_vp, ctypes.c_double, _vp]),
     "psx_scft_propagate": (ctypes.c_int, [_vp]),
     "psx_scft_Q": (ctypes.c_int, [_vp, _dp]),
     "psx_scft_densities": (ctypes.c_int, [_vp, _vp, _vp]),

@@ -412,6 +412,7 @@ struct QuadArgs {
   double* part;                  // [gridDim.x][kNPart]
   int mode, f_order;
   int acc_in;                    // add the raw sums of earlier chunks held in phiA/phiB
+  const double* noise;           // steepest descent: noise added to w_A, w_B (2N), or null
 };
 
 // V consecutive cells of T in one aligned load (16 bytes for V = 16 / sizeof(T)).
@@ -442,8 +443,12 @@ __device__ __forceinline__ void quad_tail(const QuadArgs& a, long long r, double
     ff[r] = dHA; ff[N + r] = dHB;
     return;
   }
-  const double nA = wA + (dHA + (-a.lam1) * dHB) * a.lam0;
-  const double nB = wB + (dHB + (-a.lam1) * dHA) * a.lam0;
+  double nA = wA + (dHA + (-a.lam1) * dHB) * a.lam0;
+  double nB = wB + (dHB + (-a.lam1) * dHA) * a.lam0;
+  if (a.noise) {   // PsPolymerUpdater::addRandUpdateConjgFields, before the pressure
+    nA = nA + a.noise[r];
+    nB = nB + a.noise[N + r];
+  }
   const double pr = (nA + nB) * 0.5;
   a.wA[r] = nA; a.wB[r] = nB; a.praw[r] = pr;
   ((T*)a.wfA)[r] = (T)exp(-a.halfds * nA);

@@ -24,6 +24,7 @@
 #include "psx_kernels.hpp"
 #include "psx_fft_reg.hpp"
 #include "psx_filter.hpp"
+#include "psx_rng.hpp"
 
 using namespace psx;
 
@@ -877,6 +878,13 @@ struct psx_scft {
   std::vector<int> btab_axis;
   bool in_batch = false;                // owned by a psx_batch (its tables point at our slices)
   psx_fft* fplan = nullptr;             // FP64 plan of the spectral filters (first use)
+  // steepest-descent noise (PsPolymerUpdater::addRandUpdateConjgFields)
+  psx_rng* noise_rng = nullptr;         // caller's stream; null: no noise drawn
+  double noise_strength = 0.0;
+  double* noise = nullptr;              // device, 2N: the current iteration's noise
+  double* noise_host[2] = {nullptr, nullptr};   // pinned staging, alternating
+  hipEvent_t noise_ev[2] = {nullptr, nullptr};
+  int noise_k = 0;
   int prec;
   std::vector<Axis> axes;
   int variant[2][3] = {{-1, -1, -1}, {-1, -1, -1}};   // line-kernel choice per precision, pass
@@ -1389,7 +1397,31 @@ QuadArgs quad_args(psx_scft* e, int mode, const psx_mixing* mix) {
   a.part = e->part;
   a.mode = mode;
   a.f_order = mix ? mix->f_order : 0;
+  if ((mode == QM_SD || mode == QM_SD_PHI) && e->noise_rng && e->noise_strength != 0.0) a.noise = e->noise;
   return a;
+}
+
+// Draw one steepest-descent step's noise on the host, in the reference's
+// order (all of w_A's cells, then w_B's: PsPolymerUpdater.cpp:282-293, each
+// value (0 + strength g) * 1), and queue its copy to e->noise ahead of the
+// step's kernels.  Strength 0 still draws, as the reference does, but adds
+// nothing, so nothing is copied.
+psx_status stage_noise(psx_scft* e) {
+  if (!e->noise_rng) return PSX_OK;
+  const long long n2 = 2 * e->N;
+  if (e->noise_strength == 0.0) {
+    for (long long i = 0; i < n2; ++i) (void)e->noise_rng->gaussian();
+    return PSX_OK;
+  }
+  const int k = e->noise_k;
+  e->noise_k ^= 1;
+  PSX_HIP(hipEventSynchronize(e->noise_ev[k]));   // its previous copy has finished
+  double* h = e->noise_host[k];
+  const double s = e->noise_strength;
+  for (long long i = 0; i < n2; ++i) h[i] = (0.0 + s * e->noise_rng->gaussian()) * 1.0;
+  PSX_HIP(hipMemcpyAsync(e->noise, h, (size_t)n2 * sizeof(double), hipMemcpyHostToDevice, e->stream));
+  PSX_HIP(hipEventRecord(e->noise_ev[k], e->stream));
+  return PSX_OK;
 }
 
 // The quadrature kernel: FP32 slices as 16-byte vectors of 4 cells when N
@@ -1587,6 +1619,11 @@ extern "C" psx_status psx_scft_destroy(psx_scft* e) {
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   drop_graphs(e);
   psx_fft_plan_destroy(e->fplan);
+  dfree((void*&)e->noise);
+  for (int k = 0; k < 2; ++k) {
+    if (e->noise_host[k]) (void)hipHostFree(e->noise_host[k]);
+    if (e->noise_ev[k]) (void)hipEventDestroy(e->noise_ev[k]);
+  }
   for (auto& s : e->sw) free_sweep(s);
   for (auto& a : e->axes) { dfree(a.tw[0]); dfree(a.tw[1]); }
   dfree((void*&)e->wA); dfree((void*&)e->wB); dfree((void*&)e->praw);
@@ -2047,6 +2084,51 @@ extern "C" psx_status psx_scft_spec_filter(psx_scft* e, int fields, double cutof
   return PSX_OK;
 }
 
+extern "C" psx_status psx_rng_create(unsigned long long seed, psx_rng** out) {
+  if (!out) return fail(PSX_ERR_INVALID, "null argument");
+  psx_rng* r = new psx_rng();
+  r->seed((uint32_t)(seed & 0xffffffffull));
+  *out = r;
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_rng_destroy(psx_rng* r) {
+  delete r;
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_rng_uniform(psx_rng* r, long long n, double* h_out) {
+  if (!r || (n > 0 && !h_out) || n < 0) return fail(PSX_ERR_INVALID, "bad argument");
+  for (long long i = 0; i < n; ++i) h_out[i] = r->uniform();
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_rng_gaussian(psx_rng* r, long long n, double* h_out) {
+  if (!r || (n > 0 && !h_out) || n < 0) return fail(PSX_ERR_INVALID, "bad argument");
+  for (long long i = 0; i < n; ++i) h_out[i] = r->gaussian();
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_scft_set_noise(psx_scft* e, double strength, psx_rng* rng) {
+  if (!e) return fail(PSX_ERR_INVALID, "null engine");
+  if (e->in_batch) return fail(PSX_ERR_STATE, "engine belongs to a batch");
+  if (!std::isfinite(strength)) return fail(PSX_ERR_INVALID, "noise strength must be finite");
+  const bool was = e->noise_rng && e->noise_strength != 0.0;
+  const bool now = rng && strength != 0.0;
+  if (now && !e->noise) {
+    PSX_TRY(dalloc((void**)&e->noise, 2 * (size_t)e->N * sizeof(double)));
+    for (int k = 0; k < 2; ++k) {
+      PSX_HIP(hipHostMalloc((void**)&e->noise_host[k], 2 * (size_t)e->N * sizeof(double), hipHostMallocDefault));
+      PSX_HIP(hipEventCreateWithFlags(&e->noise_ev[k], hipEventDisableTiming));
+      PSX_HIP(hipEventRecord(e->noise_ev[k], e->stream));
+    }
+  }
+  if (was != now) drop_graphs(e);
+  e->noise_rng = rng;
+  e->noise_strength = rng ? strength : 0.0;
+  return PSX_OK;
+}
+
 static psx_status get_fields_impl(psx_scft* e, double* wA, double* wB, double* p, hipMemcpyKind k) {
   const size_t fb = e->N * sizeof(double);
   if (wA) PSX_HIP(hipMemcpyAsync(wA, e->wA, fb, k, e->stream));
@@ -2161,6 +2243,7 @@ extern "C" psx_status psx_scft_update_steepest(psx_scft* e, double lambda0, doub
   psx_mixing mix{};
   mix.kind = 0; mix.lambda0 = lambda0; mix.lambda1 = lambda1; mix.f_order = 0;
   hipLaunchKernelGGL(k_state_set_call, dim3(1), dim3(1), 0, e->stream, e->st);
+  PSX_TRY(stage_noise(e));
   PSX_TRY(launch_quad(e, QM_SD_PHI, &mix));
   PSX_TRY(launch_finish(e, FIN_ITER_SD, nullptr));
   DevState h;
@@ -2242,6 +2325,8 @@ extern "C" psx_status psx_scft_iterate(psx_scft* e, const psx_mixing* mix, int n
   if (!e || !mix) return fail(PSX_ERR_INVALID, "null argument");
   if (done) *done = 0;
   if (n_iters <= 0) return PSX_OK;
+  if (mix->kind != 0 && e->noise_rng && e->noise_strength != 0.0)
+    return fail(PSX_ERR_INVALID, "noise belongs to the steepest-descent update (mixing kind 0)");
   PSX_TRY(prepare_iterate(e, mix, n_iters));
 
   hipGraphExec_t exec = nullptr;
@@ -2266,6 +2351,7 @@ extern "C" psx_status psx_scft_iterate(psx_scft* e, const psx_mixing* mix, int n
   DevState h;
   psx_status result = PSX_OK;
   for (; it < n_iters; ++it) {
+    if (mix->kind == 0) PSX_TRY(stage_noise(e));
     if (exec) PSX_HIP(hipGraphLaunch(exec, e->stream));
     else PSX_TRY(enqueue_iteration(e, mix));
     if (tol > 0.0) {

@@ -386,6 +386,9 @@ class MeltConfig:
     constraint_field: str = "defaultPressure"
     dump_period: int | None = None
     filters: list = field(default_factory=list)   # SpecFilterUpdater, in sequence order
+    # monomerDens PhysFields in deck order: each draws N uniforms for its
+    # random initial conjugate field (psphysf/PsMonomerDens.cpp:116-126)
+    init_fields: tuple = ()
 
 
 _SUPPORTED_TOP = {"Grid", "Decomp", "Comm", "FFT", "PhysField", "EffHamil", "Polymer", "History",
@@ -524,12 +527,21 @@ def configure(root: Node) -> MeltConfig:
     period = int(hist[0].get("updatePeriodicity", 1)) if hist else None
     seed = root.get("randomSeed")
     cons = [pf.name for pf in root.all("PhysField") if pf.get("kind") == "constraint"]
+    init_fields = []
+    for pf in root.all("PhysField"):
+        if pf.get("kind") != "monomerDens":
+            continue
+        if pf.get("initOption", "random") != "random":
+            raise DeckError(f"PhysField {pf.name}: initOption {pf.get('initOption')!r} unsupported "
+                            "(random only)")
+        init_fields.append(pf.name)
     dump = root.get("dumpPeriodicity")
     return MeltConfig(shape=shape, cell=cell, blocks=blocks, block_names=names, ds=ds_vals.pop(),
                       chiN=chiN, lambdas=lam, noise=noise, nsteps=int(root.get("nsteps", 1)),
                       seed=None if seed is None else int(seed), species_fields=tuple(fields),
                       history_period=period, constraint_field=cons[0] if cons else "defaultPressure",
-                      dump_period=None if dump is None else int(dump), filters=filters)
+                      dump_period=None if dump is None else int(dump), filters=filters,
+                      init_fields=tuple(init_fields))
 
 
 def build(cfg: MeltConfig, prec="fp64", stream=None):
@@ -539,35 +551,64 @@ def build(cfg: MeltConfig, prec="fp64", stream=None):
                               stream=stream)
 
 
+def reference_initial_fields(cfg: MeltConfig, rng) -> tuple:
+    """(w_A, w_B) as the reference initialises them: every monomerDens
+    PhysField in deck order takes N uniforms u of theRng and sets its
+    conjugate field to 1.0 * (u + (-0.5)) (PsGridField::resetRandom,
+    psbase/PsGridField.cpp:389-402; psphysf/PsMonomerDens.cpp:116-126)."""
+    n = int(np.prod(cfg.shape))
+    out = {}
+    for name in cfg.init_fields:
+        out[name] = (1.0 * (rng.uniform(n) + (-0.5))).reshape(cfg.shape)
+    a, b = cfg.species_fields
+    if a not in out or b not in out:
+        raise DeckError("the flory scfields must be monomerDens PhysFields")
+    return out[a], out[b]
+
+
 def run(deck, iters: int | None = None, prec="fp64", sigma: float = 0.1, fields_seed=None,
         use_graph: bool = True, tol: float = 0.0, prefix: str | None = None,
-        restore_step: int | None = None, search_path=()):
-    """Run a deck: its steepest-descent relaxation for ``nsteps`` (or
-    ``iters``) iterations, from seeded N(0, sigma) fields (Philox keyed by the
-    deck's randomSeed, or fields_seed) or, with ``restore_step``, from the
-    PhysField files ``<prefix>_<field>_<1000000+step>.h5`` of that step (the
-    reference's ``-r`` restart).  With ``prefix`` the PhysFields are dumped
-    every ``dumpPeriodicity`` steps, after the update, as the reference's main
-    loop does (polyswift/polyswift.cxx:544-588).  The reference's noise term
-    is not applied (noise-free parity runs); the deck's value is returned
-    beside the histories."""
+        restore_step: int | None = None, search_path=(), init: str = "reference",
+        noise: bool = True):
+    """Run a deck: its steepest-descent relaxation (with the deck's noise and
+    spectral filters) for ``nsteps`` (or ``iters``) iterations.
+
+    The reference's random stream (MT19937 seeded with ``randomSeed``,
+    0 when the deck sets none) feeds the initial fields and the noise, in the
+    reference's order.  Initial fields: ``init="reference"`` draws them as the
+    reference does (U(-0.5, 0.5) per monomerDens PhysField); ``init="gaussian"``
+    uses the project's seeded N(0, sigma) fields (Philox keyed by
+    ``fields_seed`` or randomSeed); ``restore_step`` reads the PhysField files
+    ``<prefix>_<field>_<1000000+step>.h5`` of that step instead (the
+    reference's ``-r`` restart, which draws no initial fields).  With
+    ``prefix`` the PhysFields are dumped every ``dumpPeriodicity`` steps,
+    after the update, as the reference's main loop does
+    (polyswift/polyswift.cxx:544-588).  ``noise=False`` leaves the noise term
+    out (noise-free runs)."""
     from . import fields as F
     from . import physfield
-    from .scft import Mixing
+    from .scft import Mixing, Rng
     root = load(deck, search_path) if isinstance(deck, str) else deck
     cfg = configure(root)
     e = build(cfg, prec)
     names = (*cfg.species_fields, cfg.constraint_field)
+    rng = Rng(cfg.seed if cfg.seed is not None else 0)
     step0 = 0
     if restore_step is not None:
         if prefix is None:
             raise DeckError("restore_step needs the dump prefix")
         physfield.restore(e, prefix, restore_step, names)
         step0 = int(restore_step)
-    else:
+    elif init == "reference":
+        e.setFields(*reference_initial_fields(cfg, rng))
+    elif init == "gaussian":
         seed = fields_seed if fields_seed is not None else (cfg.seed or 0)
         wA, wB = F.seeded_fields(cfg.shape, cfg.cell, sigma=sigma, seed=seed)
         e.setFields(wA, wB)
+    else:
+        raise DeckError(f"init must be 'reference' or 'gaussian', not {init!r}")
+    if noise and cfg.noise != 0.0:
+        e.setNoise(cfg.noise, rng)
     n = int(iters or cfg.nsteps)
     mix = Mixing("steepest", *cfg.lambdas)
     period = cfg.dump_period if prefix else None
@@ -596,5 +637,6 @@ def run(deck, iters: int | None = None, prec="fp64", sigma: float = 0.1, fields_
         if h["iters"] < chunk:     # converged
             break
     out = {k: np.asarray(v) for k, v in hist.items()}
-    out.update(iters=done, step=t, config=cfg, engine=e, noise_ignored=cfg.noise)
+    out.update(iters=done, step=t, config=cfg, engine=e, rng=rng,
+               noise=cfg.noise if noise else 0.0)
     return out

@@ -153,6 +153,34 @@ class PsFFT:
     __del__ = close
 
 
+class Rng:
+    """The reference's host random stream (PsRandom: MT19937, init_genrand
+    seeding, uniform y / 2^32, Marsaglia polar Gaussians)."""
+
+    def __init__(self, seed: int):
+        self._L = _abi.load()
+        h = ctypes.c_void_p()
+        check(self._L.psx_rng_create(int(seed), ctypes.byref(h)))
+        self._h = h
+
+    def uniform(self, n: int) -> np.ndarray:
+        out = np.empty(int(n))
+        check(self._L.psx_rng_uniform(self._h, int(n), _ptr(out)))
+        return out
+
+    def gaussian(self, n: int) -> np.ndarray:
+        out = np.empty(int(n))
+        check(self._L.psx_rng_gaussian(self._h, int(n), _ptr(out)))
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.psx_rng_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+
 @dataclass
 class Mixing:
     """Field-update method (``steepestDescent`` [lambda0, lambda1] or Anderson)."""
@@ -214,6 +242,12 @@ class DiblockSCFT:
 
     def setPrecision(self, prec) -> None:
         check(self._L.psx_scft_set_precision(self._h, _prec(prec)))
+
+    def setNoise(self, strength: float, rng: "Rng | None") -> None:
+        """Steepest-descent noise: strength * N(0, 1) per cell and step from
+        rng (the reference's theRng); rng None turns it off."""
+        self._noise_rng = rng       # the engine keeps the stream's pointer
+        check(self._L.psx_scft_set_noise(self._h, float(strength), rng._h if rng else None))
 
     def specFilter(self, fields: str, cutoffFactor: float, filterStrength: float,
                    specCellSizes=None) -> None:

@@ -134,7 +134,7 @@ def test_filter_deck_matches_oracle():
     """melt_filter.pre: multiSpecFilter every 4 steps from step 4."""
     root = deck.load(os.path.join(DECKS, "melt_filter.pre"), search_path=[os.path.join(DECKS, "macros")])
     cfg = deck.configure(root)
-    h = deck.run(root)
+    h = deck.run(root, init="gaussian")
     (f,) = cfg.filters
     P = O.Params(*cfg.shape, *cfg.cell, cfg.ds, 0.3, cfg.chiN, lambda0=cfg.lambdas[0],
                  lambda1=cfg.lambdas[1])
