@@ -31,25 +31,106 @@ inline void check(psx_status s) {
   throw Error(s, std::string("psx status ") + std::to_string(s) + ": " + buf);
 }
 
-// FFT pair plugin: scaledFFTPair(data, kdata, res) on host arrays of
-// getFFTSize() elements, unnormalised, real part kept.
+// A device buffer (psx_buffer_*), for host-array forms of device calls.
+class DevBuf {
+ public:
+  explicit DevBuf(size_t bytes) : n_(bytes) { check(psx_buffer_create(bytes, &d_)); }
+  ~DevBuf() { psx_buffer_destroy(d_); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  void* get() const { return d_; }
+  void upload(const void* h) { check(psx_buffer_upload(d_, h, n_)); }
+  void download(void* h) const { check(psx_buffer_download(h, d_, n_)); }
+
+ private:
+  void* d_ = nullptr;
+  size_t n_ = 0;
+};
+
+// FFT plugin: every PsFFTBase transform on host arrays of getFFTSize()
+// elements of the plan's type, unnormalised (psfft/PsFFTW.cpp:121-296).
 class HipFFT {
  public:
-  HipFFT(int nx, int ny, int nz, psx_prec prec = PSX_FP64) { check(psx_fft_plan_create(nx, ny, nz, prec, nullptr, &p_)); }
+  HipFFT(int nx, int ny, int nz, psx_prec prec = PSX_FP64) : prec_(prec) {
+    check(psx_fft_plan_create(nx, ny, nz, prec, nullptr, &p_));
+  }
   ~HipFFT() { psx_fft_plan_destroy(p_); }
   HipFFT(const HipFFT&) = delete;
   HipFFT& operator=(const HipFFT&) = delete;
   size_t getFFTSize() const { return psx_fft_size(p_); }
-  void scaledFFTPair(const double* data, const double* kdata, double* resPtr) {
+  void scaledFFTPair(const void* data, const void* kdata, void* resPtr) {
     check(psx_fft_scaled_pair_host(p_, data, kdata, resPtr));
   }
   // device-pointer form (asynchronous on the plan's stream)
   void scaledFFTPairDevice(const void* d_data, const void* d_kdata, void* d_res) {
     check(psx_fft_scaled_pair(p_, d_data, d_kdata, d_res));
   }
+  void forwardFFTAbs(const void* data, void* resPtr) { run1(psx_fft_forward_abs, data, resPtr); }
+  void calcForwardFFT(const void* data, void* resPtr) { run1(psx_fft_forward_re, data, resPtr); }
+  void calcBackwardFFT(const void* data, void* resPtr) { run1(psx_fft_backward_re, data, resPtr); }
+  void convolveRe(const void* data1, const void* data2, void* resPtr) {
+    run2(psx_fft_convolve_re, data1, data2, resPtr);
+  }
+  void scaledFFTPairIm(const void* data, const void* kdata, void* resPtr) {
+    run2(psx_fft_scaled_pair_im, data, kdata, resPtr);
+  }
+  // PsSimpleSpecFilter (specCells null) / PsMultiSpecFilter::applyFilter
+  // after subtractAverage, in place on a host field
+  void specFilter(void* w, double cutoffFactor, double filterStrength, const int* specCells = nullptr,
+                  double wallVolume = 0.0) {
+    DevBuf d(bytes());
+    d.upload(w);
+    check(psx_fft_spec_filter(p_, d.get(), cutoffFactor, filterStrength, specCells, wallVolume));
+    check(psx_fft_synchronize(p_));
+    d.download(w);
+  }
+  // PsPoissonUpdater::update, in place on host charge and psi fields
+  void poissonUpdate(void* charge, void* psi, double bjerrumLen, double dx, double dy, double dz) {
+    DevBuf c(bytes()), q(bytes());
+    c.upload(charge);
+    q.upload(psi);
+    check(psx_fft_poisson_update(p_, c.get(), q.get(), bjerrumLen, dx, dy, dz));
+    check(psx_fft_synchronize(p_));
+    c.download(charge);
+    q.download(psi);
+  }
+  psx_fft* handle() { return p_; }
 
  private:
+  size_t bytes() const { return getFFTSize() * (prec_ == PSX_FP64 ? 8 : 4); }
+  void run1(psx_status (*fn)(psx_fft*, const void*, void*), const void* in, void* out) {
+    DevBuf a(bytes()), r(bytes());
+    a.upload(in);
+    check(fn(p_, a.get(), r.get()));
+    check(psx_fft_synchronize(p_));
+    r.download(out);
+  }
+  void run2(psx_status (*fn)(psx_fft*, const void*, const void*, void*), const void* in1,
+            const void* in2, void* out) {
+    DevBuf a(bytes()), b(bytes()), r(bytes());
+    a.upload(in1);
+    b.upload(in2);
+    check(fn(p_, a.get(), b.get(), r.get()));
+    check(psx_fft_synchronize(p_));
+    r.download(out);
+  }
   psx_fft* p_ = nullptr;
+  psx_prec prec_;
+};
+
+// The reference's random stream theRng (psstd/PsRandom.h).
+class Rng {
+ public:
+  explicit Rng(unsigned long long seed) { check(psx_rng_create(seed, &r_)); }
+  ~Rng() { psx_rng_destroy(r_); }
+  Rng(const Rng&) = delete;
+  Rng& operator=(const Rng&) = delete;
+  void uniform(long long n, double* out) { check(psx_rng_uniform(r_, n, out)); }
+  void gaussian(long long n, double* out) { check(psx_rng_gaussian(r_, n, out)); }
+  psx_rng* handle() { return r_; }
+
+ private:
+  psx_rng* r_ = nullptr;
 };
 
 // Bulk AB diblock: reset + solveQ of both blocks, calcBigQ, the quadrature,
@@ -81,6 +162,14 @@ class DiblockSolver {
     phiA.resize(n_); phiB.resize(n_);
     check(psx_scft_densities(e_, nullptr, nullptr));
     check(psx_scft_get_densities_host(e_, phiA.data(), phiB.data()));
+  }
+  // steepest-descent noise from the reference's stream (nullptr: off)
+  void setNoise(double strength, Rng* rng) {
+    check(psx_scft_set_noise(e_, strength, rng ? rng->handle() : nullptr));
+  }
+  // spectral filter updater on w_A (fields & 1) and w_B (fields & 2)
+  void specFilter(int fields, double cutoffFactor, double filterStrength, const int* specCells = nullptr) {
+    check(psx_scft_spec_filter(e_, fields, cutoffFactor, filterStrength, specCells));
   }
   double steepestDescent(double lambda0, double lambda1) {
     double F;
