@@ -74,19 +74,22 @@ def time_iters(e, mix, warmup, steps):
 
 def accuracy_leg(w, mix, tol, max_iters, switch_tol, max_fp32):
     """FP64 run to tol vs the mixed protocol (FP32 sweeps until the residual
-    falls below switch_tol, then FP64 iterations to tol) from the same seed."""
+    falls below switch_tol, then FP64 iterations to tol with a fresh, deeper
+    Anderson history) from the same seed."""
     from polyswift_plusplus_amd import Mixing
     e64 = make_engine("fp64", w)
     t0 = time.perf_counter()
     h64 = e64.iterate(max_iters, mix, tol=tol)
     t64 = time.perf_counter() - t0
     e64.close()
+    from polyswift_plusplus_amd.relax import tail_mixing
+    tail = tail_mixing(mix)
     em = make_engine("fp32", w)
     t0 = time.perf_counter()
     h32 = em.iterate(max_fp32, mix, tol=switch_tol)
     em.setPrecision("fp64")
     em.resetHistory()
-    hfin = em.iterate(max_iters, mix, tol=tol)
+    hfin = em.iterate(max_iters, tail, tol=tol)
     tm = time.perf_counter() - t0
     em.close()
     F64 = float(h64["F"][-1])
@@ -95,6 +98,7 @@ def accuracy_leg(w, mix, tol, max_iters, switch_tol, max_fp32):
                 fp64_iters=int(h64["iters"]), fp64_err=float(h64["err"][-1]),
                 mixed_fp32_iters=int(h32["iters"]), mixed_fp64_iters=int(hfin["iters"]),
                 mixed_err=float(hfin["err"][-1]), fp64_seconds=t64, mixed_seconds=tm,
+                anderson_depth=mix.depth, mixed_tail_depth=tail.depth,
                 converged=bool(h64["err"][-1] < tol and hfin["err"][-1] < tol))
 
 

@@ -262,7 +262,7 @@ def test_cpp_host_program():
     assert abs(F_cpp - h["F"][-1]) < 1e-11
     # the C++ FFT plugin's filter and |F|^2 against the oracle
     line = next(ln for ln in r.stdout.splitlines() if ln.startswith("filter"))
-    ss, mx = float(line.split()[3]), float(line.split()[5])
+    ss, mx = float(line.split()[3]), float(line.split()[6])
     filt = O.spec_filter(e.getFields()[0], 0.5, 0.0)
     assert abs(ss / np.sum(filt ** 2) - 1) < 1e-10
     assert abs(mx / O.forward_fft_abs(filt).max() - 1) < 1e-10
