@@ -102,7 +102,7 @@ def accuracy_leg(w, mix, tol, max_iters, switch_tol, max_fp32):
                 converged=bool(h64["err"][-1] < tol and hfin["err"][-1] < tol))
 
 
-def batch_leg(n_inst=64, steps=5, warmup=2, seed=4):
+def batch_leg(n_inst=64, steps=5, warmup=2, seed=4, sequential=True):
     """Parameter sweep: n_inst instances, grid drawn from {32,48,64,96}^3,
     chiN ~ U[12, 30], fA ~ U[0.20, 0.50] rounded to 0.01 (the reference
     needs an integral fA/ds), box = grid x 0.08 Rg, Ns = 100, FP32 sweeps,
@@ -127,6 +127,10 @@ def batch_leg(n_inst=64, steps=5, warmup=2, seed=4):
     b.iterate(steps, mix)
     dt = time.perf_counter() - t0
     b.close()
+    sizes = {n: sum(1 for sp in specs if sp["shape"][0] == n) for n in (32, 48, 64, 96)}
+    if not sequential:
+        return dict(instances=n_inst, sizes=sizes, steps=steps, aggregate_iter_per_s=n_inst * steps / dt,
+                    autotune_choice=choice)
     seq = 0.0
     for i, sp in enumerate(specs):
         e = DiblockSCFT(sp["shape"], sp["cell"], sp["fA"], sp["ds"], sp["chiN"], prec="fp64")
@@ -136,7 +140,6 @@ def batch_leg(n_inst=64, steps=5, warmup=2, seed=4):
         e.iterate(steps, mix)
         seq += time.perf_counter() - t0
         e.close()
-    sizes = {n: sum(1 for sp in specs if sp["shape"][0] == n) for n in (32, 48, 64, 96)}
     return dict(instances=n_inst, sizes=sizes, steps=steps,
                 aggregate_iter_per_s=n_inst * steps / dt,
                 sequential_fp64_iter_per_s=n_inst * steps / seq,
@@ -174,6 +177,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default line kernels (no per-pass autotuning)")
+    ap.add_argument("--batch-profile", action="store_true",
+                    help="only the batched half of the batch leg (for rocprofv3)")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3 counters: the dual contour step of "
                          "both precisions (5 reps each) and one FP64 copy of known size")
@@ -189,6 +194,9 @@ def main():
     w = dict(WORKLOAD)
     w["shape"] = (args.grid,) * 3
     N = int(np.prod(w["shape"]))
+    if args.batch_profile:
+        print(json.dumps(batch_leg(sequential=False)))
+        return
     if args.profile:
         # Counter calibration: one k_to_f64<double> over N cells moves exactly
         # 8N bytes in and 8N bytes out (psx_scft_get_slice_host on FP64).

@@ -99,3 +99,20 @@ def test_anderson_with_noise_is_refused():
             e.iterate(2, Mixing("anderson", 0.1, 0.0, depth=4, beta=0.5, anderson_start=1.0))
     finally:
         e.close()
+
+
+def test_noise_with_contour_checkpointing():
+    """The checkpointed sweep (K = 7) adds the same noise in its last chunk."""
+    P, wA, wB = _setup(34)
+    ref = O.scft_run_noise(P, wA, wB, n_iters=4, strength=0.05, rng=O.Rng(9))
+    e = DiblockSCFT(SHAPE, CELL, 0.3, 0.05, 15.0)
+    try:
+        e.setFields(wA, wB)
+        e.setCheckpointing(7)
+        e.setNoise(0.05, Rng(9))
+        h = e.iterate(4, Mixing("steepest", 0.1, 0.05))
+        assert rel(h["F"], ref["F"]) < 1e-10
+        gA, gB, _ = e.getFields()
+        assert rel(gA, ref["wA"]) < 1e-10 and rel(gB, ref["wB"]) < 1e-10
+    finally:
+        e.close()

@@ -58,6 +58,20 @@ def test_spec_filter_fp64(cells, strength, vwall):
     assert rel(out, O.spec_filter(w, 0.2, strength, cells, vwall)) < 1e-12
 
 
+@pytest.mark.parametrize("shape,cells", [((32, 32, 1), (4, 4, 1)), ((32, 16, 1), (4, 2, 1)), ((64, 1, 1), (8, 1, 1))])
+def test_spec_filter_2d_1d(shape, cells):
+    """2D and 1D grids (the reference's filter decks are 2D, specCellSizes
+    [4 4 1]), simple and multi."""
+    print(f"spec filter {shape} {cells}", flush=True)
+    w = np.random.default_rng(15).standard_normal(shape)
+    p = PsFFT(shape)
+    try:
+        for c in (None, cells):
+            assert rel(p.specFilter(w, 0.3, 0.1, c), O.spec_filter(w, 0.3, 0.1, c)) < 1e-12
+    finally:
+        p.close()
+
+
 def test_spec_filter_fp32_two_modes():
     shape = (32, 8, 8)
     x = np.arange(32)[:, None, None] * np.ones(shape)
