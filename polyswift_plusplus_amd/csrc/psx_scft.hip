@@ -878,6 +878,7 @@ struct psx_scft {
   std::vector<int> btab_axis;
   bool in_batch = false;                // owned by a psx_batch (its tables point at our slices)
   psx_fft* fplan = nullptr;             // FP64 plan of the spectral filters (first use)
+  long long graph_gen = 0;              // bumped whenever captured graphs go stale
   // steepest-descent noise (PsPolymerUpdater::addRandUpdateConjgFields)
   psx_rng* noise_rng = nullptr;         // caller's stream; null: no noise drawn
   double noise_strength = 0.0;
@@ -918,6 +919,7 @@ namespace {
 void drop_graphs(psx_scft* e) {
   for (auto& kv : e->graphs) (void)hipGraphExecDestroy(kv.second);
   e->graphs.clear();
+  ++e->graph_gen;   // a batch holding this engine's buffers in its graphs re-captures
 }
 
 // exp(-ds b^2 k_d^2) / n_d for every axis and block; the product over axes is
@@ -2279,11 +2281,12 @@ static psx_status prepare_iterate(psx_scft* e, const psx_mixing* mix, int n_iter
   if (mix->kind == 1 && (mix->depth < 0 || mix->depth > kMaxDepth))
     return fail(PSX_ERR_INVALID, "Anderson depth must be in [0, 32]");
   if (mix->kind == 1) PSX_TRY(ensure_ring(e, mix->depth));
-  if (n_iters > e->hist_cap) {
+  if (n_iters > e->hist_cap) {   // grows rarely: captured graphs hold the pointer
+    const int cap = std::max(n_iters, std::max(2 * e->hist_cap, 1024));
     drop_graphs(e);
     dfree((void*&)e->hist);
-    PSX_TRY(dalloc((void**)&e->hist, (size_t)n_iters * 3 * sizeof(double)));
-    e->hist_cap = n_iters;
+    PSX_TRY(dalloc((void**)&e->hist, (size_t)cap * 3 * sizeof(double)));
+    e->hist_cap = cap;
   }
   if (!e->sw[e->prec].wfac_valid) PSX_TRY(compute_wfac(e));
   hipLaunchKernelGGL(k_state_set_call, dim3(1), dim3(1), 0, e->stream, e->st);
@@ -2737,6 +2740,7 @@ struct psx_batch {
   int Ns = 0;
   std::map<GraphKey, hipGraphExec_t> graphs;
   hipStream_t ss[kBatchStreams] = {};   // ss[0] = stream
+  std::vector<long long> graph_gen;     // engines' graph_gen when the graphs were captured
   hipEvent_t ev_fork = nullptr;
   hipEvent_t ev_join[kBatchStreams] = {};
   std::vector<hipEvent_t> ev_cls;       // per class: its sweep is done
@@ -2984,6 +2988,14 @@ extern "C" psx_status psx_batch_iterate(psx_batch* b, const psx_mixing* mix, int
   if (done) *done = 0;
   if (n_iters <= 0) return PSX_OK;
   for (auto* e : b->eng) PSX_TRY(prepare_iterate(e, mix, n_iters));
+  // prepare_iterate may have reallocated an engine's history, ring or
+  // partials; graphs captured with the old pointers must not replay
+  std::vector<long long> gen;
+  for (auto* e : b->eng) gen.push_back(e->graph_gen);
+  if (gen != b->graph_gen) {
+    batch_drop_graphs(b);
+    b->graph_gen = gen;
+  }
   hipGraphExec_t exec = nullptr;
   if (use_graph) {
     GraphKey key = graph_key(b->prec, mix);

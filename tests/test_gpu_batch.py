@@ -19,27 +19,59 @@ def _fields(i, sp):
     return fields.seeded_fields(sp["shape"], sp["cell"], sigma=0.3, seed=100 + i, seed_kind="lamellar")
 
 
-def _single(sp, i, prec, n):
+def _single(sp, i, prec, n, fields_out=False):
     e = DiblockSCFT(sp["shape"], sp["cell"], sp["fA"], sp["ds"], sp["chiN"], prec=prec)
     e.setFields(*_fields(i, sp))
-    return e.iterate(n, MIX)
+    h = e.iterate(n, MIX)
+    if fields_out:
+        h["fields"] = e.getFields()
+    e.close()
+    return h
 
 
+@pytest.mark.parametrize("use_graph", [True, False])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_batch_matches_single_instances(prec, monkeypatch):
+def test_batch_matches_single_instances(prec, use_graph, monkeypatch):
     """Like for like: the batch runs its quadrature after the sweeps, so the
-    single-instance references do too (PSX_OVERLAP_QUAD=0, the default)."""
-    print(f"batch {prec}", flush=True)
+    single-instance references do too (PSX_OVERLAP_QUAD=0, the default).  The
+    batch's four streams change no arithmetic: F and the fields agree bit for
+    bit, across two iterate calls, with and without the graph."""
+    print(f"batch {prec} graph={use_graph}", flush=True)
     monkeypatch.setenv("PSX_OVERLAP_QUAD", "0")
     b = BatchSCFT(SPECS, prec=prec)
     assert b.nclasses == 3
     for i, sp in enumerate(SPECS):
         b.instances[i].setFields(*_fields(i, sp))
-    h = b.iterate(8, MIX)
+    h1 = b.iterate(3, MIX, use_graph=use_graph)
+    h2 = b.iterate(5, MIX, use_graph=use_graph)
     for i, sp in enumerate(SPECS):
-        ref = _single(sp, i, prec, 8)
-        assert np.max(np.abs(h["F"][i] - ref["F"])) <= 1e-14 * np.max(np.abs(ref["F"])), i
-        wA, _, _ = b.instances[i].getFields()
+        ref = _single(sp, i, prec, 8, fields_out=True)
+        F = np.r_[h1["F"][i], h2["F"][i]]
+        assert np.max(np.abs(F - ref["F"])) <= 1e-14 * np.max(np.abs(ref["F"])), i
+        for got, want in zip(b.instances[i].getFields(), ref["fields"]):
+            assert np.array_equal(got, want), i
+    b.close()
+
+
+def test_batch_recaptures_after_engine_buffers_change(monkeypatch):
+    """A deeper Anderson history on the second call reallocates every
+    instance's ring: the batch graph is captured again, not replayed with the
+    old pointers."""
+    monkeypatch.setenv("PSX_OVERLAP_QUAD", "0")
+    deep = Mixing("anderson", lambda0=0.2, depth=9, beta=1.0, anderson_start=0.05)
+    b = BatchSCFT(SPECS, prec="fp64")
+    for i, sp in enumerate(SPECS):
+        b.instances[i].setFields(*_fields(i, sp))
+    h1 = b.iterate(3, MIX)
+    h2 = b.iterate(4, deep)
+    for i, sp in enumerate(SPECS):
+        e = DiblockSCFT(sp["shape"], sp["cell"], sp["fA"], sp["ds"], sp["chiN"])
+        e.setFields(*_fields(i, sp))
+        r1 = e.iterate(3, MIX)
+        r2 = e.iterate(4, deep)
+        F, ref = np.r_[h1["F"][i], h2["F"][i]], np.r_[r1["F"], r2["F"]]
+        assert np.max(np.abs(F - ref)) <= 1e-14 * np.max(np.abs(ref)), i
+        e.close()
     b.close()
 
 
