@@ -2439,6 +2439,7 @@ static psx_status ensure_stress(psx_scft* e) {
   PSX_TRY(write_stress_tables(e, true));
   const int need = e->nb * 3;
   if (need > e->part_cap) {
+    drop_graphs(e);   // captured iterations hold the partials' pointer
     dfree((void*&)e->part);
     PSX_TRY(dalloc((void**)&e->part, need * sizeof(double)));
     e->part_cap = need;
