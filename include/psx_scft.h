@@ -183,7 +183,9 @@ psx_status psx_scft_graph_info(const psx_scft* e, int* h_nblocks,
  * (the reference's qs / qts of PsBlock, pspolymer/PsBlock.cpp:256-300). */
 psx_status psx_scft_get_block_slice_host(psx_scft* e, int block,
                                          int from_tail, int s, double* h_out);
-/* Switch the sweep precision (allocates the other slice set on first use). */
+/* Switch the sweep precision (allocates the other slice set on first use).
+ * The Anderson history is stored in the sweep's type, so a switch restarts
+ * it (as psx_scft_reset_history). */
 psx_status psx_scft_set_precision(psx_scft* e, psx_prec prec);
 /* Contour steps of the A blocks and of the B blocks (for a diblock, the
  * block steps of A and B). */

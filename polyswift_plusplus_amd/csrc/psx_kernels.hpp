@@ -405,7 +405,7 @@ struct QuadArgs {
   double* phiA; double* phiB;
   double* wA; double* wB; double* praw;
   void* wfA; void* wfB;          // T
-  double* ringx; double* ringf;  // Anderson ring: slot s at s*2N (A then B)
+  void* ringx; void* ringf;      // Anderson ring, T: slot s at s*2N (A then B)
   int m1;                        // ring slots (depth + 1)
   double chiN, fA, fB, lam0, lam1, halfds, dens;
   DevState* st;
@@ -437,10 +437,10 @@ __device__ __forceinline__ void quad_tail(const QuadArgs& a, long long r, double
   const double dHB = p - wB + a.chiN * (phA - a.fA);
   acc[3] += dHA * dHA + dHB * dHB;
   if (a.mode == QM_RESID) {
-    double* fx = a.ringx + slot * 2 * N;
-    double* ff = a.ringf + slot * 2 * N;
-    fx[r] = wA; fx[N + r] = wB;
-    ff[r] = dHA; ff[N + r] = dHB;
+    T* fx = (T*)a.ringx + slot * 2 * N;
+    T* ff = (T*)a.ringf + slot * 2 * N;
+    fx[r] = (T)wA; fx[N + r] = (T)wB;
+    ff[r] = (T)dHA; ff[N + r] = (T)dHB;
     return;
   }
   double nA = wA + (dHA + (-a.lam1) * dHB) * a.lam0;
@@ -566,21 +566,23 @@ __global__ void k_finish(const double* __restrict__ part, int nb, double invN,
 
 // Anderson: per-block partial dots <f_cur, f_slot> for all ring slots
 // (MAXM1 >= m1 is a compile-time bound so the accumulators stay in VGPRs).
-template <int MAXM1>
+// The ring holds the sweep's type R (FP32 sweeps keep an FP32 history);
+// the dots accumulate in FP64.
+template <int MAXM1, typename R>
 __global__ void __launch_bounds__(kThreads)
-k_gram(const double* __restrict__ ringf, long long N, int m1, const DevState* st,
+k_gram(const R* __restrict__ ringf, long long N, int m1, const DevState* st,
        double* __restrict__ part) {
   const long long cur = st->iter % m1;
-  const double* fc = ringf + cur * 2 * N;
+  const R* fc = ringf + cur * 2 * N;
   double acc[MAXM1];
 #pragma unroll
   for (int j = 0; j < MAXM1; ++j) acc[j] = 0.0;
   for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < 2 * N;
        r += (long long)gridDim.x * blockDim.x) {
-    const double x = fc[r];
+    const double x = (double)fc[r];
 #pragma unroll
     for (int j = 0; j < MAXM1; ++j)
-      if (j < m1) acc[j] += x * ringf[(long long)j * 2 * N + r];
+      if (j < m1) acc[j] += x * (double)ringf[(long long)j * 2 * N + r];
   }
   __shared__ double sh[MAXM1][kThreads / 64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -679,7 +681,9 @@ k_mix(QuadArgs a) {
   const long long cur = st->iter % m1;
   const int mk = st->mk;
   const double beta = st->beta_used;
-  const double* fc = a.ringf + cur * 2 * N;
+  const T* rx = (const T*)a.ringx;
+  const T* rf = (const T*)a.ringf;
+  const T* fc = rf + cur * 2 * N;
   double acc[kNPart] = {0.0, 0.0, 0.0, 0.0};
   for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < N;
        r += (long long)gridDim.x * blockDim.x) {
@@ -687,13 +691,13 @@ k_mix(QuadArgs a) {
     for (int h = 0; h < 2; ++h) {
       const long long o = h * N + r;
       const double x = h == 0 ? a.wA[r] : a.wB[r];
-      const double f = fc[o];
+      const double f = (double)fc[o];
       double xs = x, fs = f;
       for (int i = 0; i < mk; ++i) {
         const double ci = st->coef[i];
         const long long sl = (cur - 1 - i + m1) % m1;
-        xs += ci * (a.ringx[sl * 2 * N + o] - x);
-        fs += ci * (a.ringf[sl * 2 * N + o] - f);
+        xs += ci * ((double)rx[sl * 2 * N + o] - x);
+        fs += ci * ((double)rf[sl * 2 * N + o] - f);
       }
       old[h] = x;
       xb[h] = xs;

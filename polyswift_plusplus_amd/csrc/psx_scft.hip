@@ -1570,6 +1570,22 @@ psx_status enqueue_iteration(psx_scft* e, const psx_mixing* mix) {
   return st;
 }
 
+// The Anderson ring holds the sweep's type R.
+template <typename R>
+void launch_gram(psx_scft* e, int m1) {
+  const R* rf = (const R*)e->ringf;
+  const DevState* st = e->st;
+  if (m1 <= 4)
+    hipLaunchKernelGGL((k_gram<4, R>), dim3(e->nb), dim3(kThreads), 0, e->stream, rf, e->N, m1, st, e->part);
+  else if (m1 <= 11)
+    hipLaunchKernelGGL((k_gram<11, R>), dim3(e->nb), dim3(kThreads), 0, e->stream, rf, e->N, m1, st, e->part);
+  else if (m1 <= 21)
+    hipLaunchKernelGGL((k_gram<21, R>), dim3(e->nb), dim3(kThreads), 0, e->stream, rf, e->N, m1, st, e->part);
+  else
+    hipLaunchKernelGGL((k_gram<kMaxDepth + 1, R>), dim3(e->nb), dim3(kThreads), 0, e->stream, rf, e->N, m1, st,
+                       e->part);
+}
+
 // Q, quadrature and the field update after both sweeps (capturable).
 psx_status enqueue_post_sweep(psx_scft* e, const psx_mixing* mix) {
   PSX_TRY(launch_Q(e));
@@ -1582,18 +1598,8 @@ psx_status enqueue_post_sweep(psx_scft* e, const psx_mixing* mix) {
   const int m1 = m + 1;
   PSX_TRY(launch_quad(e, QM_RESID, mix));
   PSX_TRY(launch_finish(e, FIN_ERR, nullptr));
-  if (m1 <= 4)
-    hipLaunchKernelGGL(k_gram<4>, dim3(e->nb), dim3(kThreads), 0, e->stream, (const double*)e->ringf, e->N, m1,
-                       (const DevState*)e->st, e->part);
-  else if (m1 <= 11)
-    hipLaunchKernelGGL(k_gram<11>, dim3(e->nb), dim3(kThreads), 0, e->stream, (const double*)e->ringf, e->N, m1,
-                       (const DevState*)e->st, e->part);
-  else if (m1 <= 21)
-    hipLaunchKernelGGL(k_gram<21>, dim3(e->nb), dim3(kThreads), 0, e->stream, (const double*)e->ringf, e->N, m1,
-                       (const DevState*)e->st, e->part);
-  else
-    hipLaunchKernelGGL(k_gram<kMaxDepth + 1>, dim3(e->nb), dim3(kThreads), 0, e->stream, (const double*)e->ringf,
-                       e->N, m1, (const DevState*)e->st, e->part);
+  if (e->prec == PSX_FP64) launch_gram<double>(e, m1);
+  else launch_gram<float>(e, m1);
   hipLaunchKernelGGL(k_gram_finish, dim3(m1), dim3(kThreads), 0, e->stream, (const double*)e->part, e->nb, m1,
                      e->st);
   hipLaunchKernelGGL(k_anderson_solve, dim3(1), dim3(1), 0, e->stream, m, mix->beta, mix->lambda0,
@@ -2012,6 +2018,12 @@ extern "C" psx_status psx_scft_set_precision(psx_scft* e, psx_prec prec) {
   if (e->prec == (int)prec) return PSX_OK;
   PSX_TRY(ensure_sweep(e, prec));
   e->prec = prec;
+  // the Anderson ring is kept in the sweep's type: its old entries are not
+  // readable in the new one, so the history starts again
+  if (e->ringx) {
+    hipLaunchKernelGGL(k_state_reset_hist, dim3(1), dim3(1), 0, e->stream, e->st);
+    PSX_HIP(hipGetLastError());
+  }
   e->sw[prec].wfac_valid = false;
   e->swept = false;
   return PSX_OK;
