@@ -56,7 +56,7 @@ def mixed_precision_solve(e: DiblockSCFT, tol: float = 1e-8, switch_tol: float =
 
 def relax_box(e: DiblockSCFT, tol_stress: float = 1e-7, tol_scft: float = 1e-11,
               max_outer: int = 40, first_step: float = 5.0, axes=None,
-              mixing: Mixing | None = None) -> dict:
+              mixing: Mixing | None = None, log=None) -> dict:
     """Relax box lengths L_d = n_d * cell_d to dF/dL_d = 0 by a secant step
     per axis (the orthorhombic cell keeps its angles).  Each outer step
     converges the fields from the previous ones, then takes the stress from
@@ -67,11 +67,13 @@ def relax_box(e: DiblockSCFT, tol_stress: float = 1e-7, tol_scft: float = 1e-11,
     hist = []
     L_prev = g_prev = None
     for outer in range(max_outer):
-        converge(e, tol=tol_scft, mixing=mixing)
+        h = converge(e, tol=tol_scft, mixing=mixing)
         g = e.boxStress()
         F = e.getFreeE()
         L = n * cell
-        hist.append(dict(L=L.copy(), dFdL=g.copy(), F=F))
+        hist.append(dict(L=L.copy(), dFdL=g.copy(), F=F, iters=int(h["iters"])))
+        if log:
+            log(f"outer {outer}: {h['iters']} iterations, L = {L}, dF/dL = {g}, F = {F:.12f}")
         if np.max(np.abs(g[active])) < tol_stress:
             break
         step = np.zeros(3)
