@@ -86,12 +86,27 @@ __device__ __forceinline__ void dft_reg(C* v, const C* __restrict__ tw, int twst
   }
 }
 
+// Hand-over of LDS data between the lanes of the transposes: a block barrier,
+// or (WAVE) wave-scope ordering only -- a wave's LDS operations complete in
+// issue order, so fences that keep the compiler from moving them across the
+// hand-over are enough when no other wave touches the buffer.
+template <bool WAVE>
+__device__ __forceinline__ void lds_sync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 // One round trip of a wave's lines held in registers (lane t of line p owns
 // points t + N2 r):  v <- IFFT(emul * FFT(v)), result in the same layout.
 // L is the wave's LDS transpose buffer (p-th line at L + p n).
 // tw: global twiddle table (uniform indices, scalar loads); twl, emul: LDS
 // copies of the twiddles and the spectral multiplier (per-lane indices).
-template <typename T, int N1, int N2, typename C>
+template <typename T, int N1, int N2, bool WS, typename C>
 __device__ __forceinline__ void reg_roundtrip(C* v, C* L, const C* __restrict__ tw,
                                               const C* twl, const T* emul, int t) {
   constexpr int U = N1 / N2;
@@ -100,10 +115,10 @@ __device__ __forceinline__ void reg_roundtrip(C* v, C* L, const C* __restrict__ 
 #pragma unroll
   for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], twiddle<-1>(twl, t * k1));
   // transpose: element (k1, t) at k1*N2 + (t ^ (k1 % N2)) (XOR swizzle)
-  __syncthreads();
+  lds_sync<WS>();
 #pragma unroll
   for (int k1 = 0; k1 < N1; ++k1) L[k1 * N2 + (t ^ (k1 % N2))] = v[k1];
-  __syncthreads();
+  lds_sync<WS>();
   // lane t now takes rows k1 = t + N2 u, u < U; y[u][s] = element (k1, s)
 #pragma unroll
   for (int u = 0; u < U; ++u)
@@ -122,12 +137,12 @@ __device__ __forceinline__ void reg_roundtrip(C* v, C* L, const C* __restrict__ 
     }
     dft_reg<N2, +1, T>(y, tw, N1);
   }
-  __syncthreads();
+  lds_sync<WS>();
 #pragma unroll
   for (int u = 0; u < U; ++u)
 #pragma unroll
     for (int s = 0; s < N2; ++s) L[(t + N2 * u) * N2 + (s ^ t)] = v[u * N2 + s];
-  __syncthreads();
+  lds_sync<WS>();
 #pragma unroll
   for (int k1 = 0; k1 < N1; ++k1) v[k1] = L[k1 * N2 + (t ^ (k1 % N2))];
   // inverse: conjugate twiddle, inverse DFT_N1; lanes hold x[t + N2 r]
@@ -157,17 +172,17 @@ __device__ __forceinline__ double pair_swap(double x) {
 //   radix 2 across the pair: lane h keeps X[k1 + N1 (k2 + N1 h)] =
 //   E_0[k2] + (-1)^h W_N2^{k2} E_1[k2], k2 < N1.
 // The spectral multiply follows, then the same steps inverted.
-template <typename T, int N1, typename C>
+template <typename T, int N1, bool WS, typename C>
 __device__ __forceinline__ void pair_roundtrip(C* v, C* L, const C* __restrict__ tw,
                                                const C* twl, const T* emul, int t) {
   constexpr int N2 = 2 * N1;
   dft_reg<N1, -1, T>(v, tw, N2);
 #pragma unroll
   for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], twiddle<-1>(twl, t * k1));
-  __syncthreads();
+  lds_sync<WS>();
 #pragma unroll
   for (int k1 = 0; k1 < N1; ++k1) L[k1 * N2 + (t ^ ((2 * k1) % N2))] = v[k1];
-  __syncthreads();
+  lds_sync<WS>();
   const int k1 = t >> 1, h = t & 1;
   const int sw = (2 * k1) % N2;
 #pragma unroll
@@ -207,10 +222,10 @@ __device__ __forceinline__ void pair_roundtrip(C* v, C* L, const C* __restrict__
     v[k2] = e;
   }
   dft_reg<N1, +1, T>(v, tw, N2);
-  __syncthreads();
+  lds_sync<WS>();
 #pragma unroll
   for (int m = 0; m < N1; ++m) L[k1 * N2 + ((h + 2 * m) ^ sw)] = v[m];
-  __syncthreads();
+  lds_sync<WS>();
 #pragma unroll
   for (int j = 0; j < N1; ++j) v[j] = L[j * N2 + (t ^ ((2 * j) % N2))];
 #pragma unroll
@@ -225,7 +240,7 @@ __device__ __forceinline__ void pair_roundtrip(C* v, C* L, const C* __restrict__
 // pair, as in pair_roundtrip (its P = 1 case).  Registers: column u holds
 // v[u H .. u H + H - 1].  With N1 = 12, N2 = 8 a line of 96 takes 8 lanes
 // of 12 points: twice the lines per wave of the 24 x 4 split.
-template <typename T, int N1, int N2, typename C>
+template <typename T, int N1, int N2, bool WS, typename C>
 __device__ __forceinline__ void pairs_roundtrip(C* v, C* L, const C* __restrict__ tw,
                                                 const C* twl, const T* emul, int t) {
   constexpr int n = N1 * N2;
@@ -235,10 +250,10 @@ __device__ __forceinline__ void pairs_roundtrip(C* v, C* L, const C* __restrict_
   dft_reg<N1, -1, T>(v, tw, N2);
 #pragma unroll
   for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], twiddle<-1>(twl, t * k1));
-  __syncthreads();
+  lds_sync<WS>();
 #pragma unroll
   for (int k1 = 0; k1 < N1; ++k1) L[k1 * N2 + (t ^ ((2 * k1) % N2))] = v[k1];
-  __syncthreads();
+  lds_sync<WS>();
   const int k = t >> 1, h = t & 1;
 #pragma unroll
   for (int u = 0; u < P; ++u) {
@@ -281,14 +296,14 @@ __device__ __forceinline__ void pairs_roundtrip(C* v, C* L, const C* __restrict_
   }
 #pragma unroll
   for (int u = 0; u < P; ++u) dft_reg<H, +1, T>(v + u * H, tw, n / H);
-  __syncthreads();
+  lds_sync<WS>();
 #pragma unroll
   for (int u = 0; u < P; ++u) {
     const int k1 = k + H * u, sw = (2 * k1) % N2;
 #pragma unroll
     for (int m = 0; m < H; ++m) L[k1 * N2 + ((h + 2 * m) ^ sw)] = v[u * H + m];
   }
-  __syncthreads();
+  lds_sync<WS>();
 #pragma unroll
   for (int j = 0; j < N1; ++j) v[j] = L[j * N2 + (t ^ ((2 * j) % N2))];
 #pragma unroll
@@ -298,12 +313,15 @@ __device__ __forceinline__ void pairs_roundtrip(C* v, C* L, const C* __restrict_
 
 // Round trip on the lanes of the split (N1, N2): lane-pair form when
 // N2 = 2 N1, the N1 % N2 == 0 form, else the general lane-pair form.
-template <typename T, int N1, int N2, typename C>
+// WS: the transpose buffer L is private to the wave, so its write/read
+// hand-overs need only wave-level ordering (multi-wave blocks whose callers
+// separate the shared uses of that memory with block barriers).
+template <typename T, int N1, int N2, bool WS = false, typename C>
 __device__ __forceinline__ void line_roundtrip(C* v, C* L, const C* __restrict__ tw,
                                                const C* twl, const T* emul, int t) {
-  if constexpr (N2 == 2 * N1) pair_roundtrip<T, N1>(v, L, tw, twl, emul, t);
-  else if constexpr (N1 % N2 == 0) reg_roundtrip<T, N1, N2>(v, L, tw, twl, emul, t);
-  else pairs_roundtrip<T, N1, N2>(v, L, tw, twl, emul, t);
+  if constexpr (N2 == 2 * N1) pair_roundtrip<T, N1, WS>(v, L, tw, twl, emul, t);
+  else if constexpr (N1 % N2 == 0) reg_roundtrip<T, N1, N2, WS>(v, L, tw, twl, emul, t);
+  else pairs_roundtrip<T, N1, N2, WS>(v, L, tw, twl, emul, t);
 }
 
 // Round trip(s) along one axis for 2*64/N2 real lines per wave:
@@ -580,7 +598,7 @@ k_line_tile(LineGeom g, const PairIO* __restrict__ fio) {
     v[r].y = S[k * LS + j0 + 1];
   }
   __syncthreads();
-  line_roundtrip<T, N1, N2>(v, L, tw, twl, e1, t);
+  line_roundtrip<T, N1, N2, true>(v, L, tw, twl, e1, t);
 
   for (int stage = 0; stage < 2; ++stage) {
     const bool last = stage == 1 || !fused;
@@ -616,7 +634,7 @@ k_line_tile(LineGeom g, const PairIO* __restrict__ fio) {
       v[r].y = S[k * LS + j0 + 1];
     }
     __syncthreads();
-    line_roundtrip<T, N1, N2>(v, L, tw, twl, e2, t);
+    line_roundtrip<T, N1, N2, true>(v, L, tw, twl, e2, t);
   }
 }
 
