@@ -23,6 +23,7 @@
 #include "../../include/psx_scft.h"
 #include "psx_kernels.hpp"
 #include "psx_fft_reg.hpp"
+#include "psx_fft_cline.hpp"
 #include "psx_filter.hpp"
 #include "psx_rng.hpp"
 
@@ -337,8 +338,45 @@ psx_status launch_pair(const Axis& a, int prec, const PairIO* fio,
   return PSX_OK;
 }
 
+// Register-resident complex line pass (k_cline): 64 / N2 lines per wave.
+template <typename T, int N1, int N2>
+void launch_cline(const Axis& a, int prec, const CplxIO& f, hipStream_t s) {
+  constexpr int CL = 64 / N2;
+  LineGeom g = geom(a, prec, CL);
+  g.ntiles = (a.nlines + CL - 1) / CL;
+  const size_t smem = (size_t)64 * N1 * 2 * sizeof(T);
+  const dim3 grid(g.ntiles);
+  if (f.do_fwd && f.do_inv) hipLaunchKernelGGL((k_cline<T, N1, N2, 2>), grid, dim3(64), smem, s, g, f);
+  else if (f.do_fwd) hipLaunchKernelGGL((k_cline<T, N1, N2, 0>), grid, dim3(64), smem, s, g, f);
+  else hipLaunchKernelGGL((k_cline<T, N1, N2, 1>), grid, dim3(64), smem, s, g, f);
+}
+
+// One pass of the general FFT pair: the register kernels for the line
+// lengths that have a split, the LDS Stockham kernel otherwise (and for
+// every length under PSX_FORCE_STOCKHAM=1).
 template <typename T>
 psx_status launch_cplx(const Axis& a, int prec, const CplxIO& f, hipStream_t s) {
+  if (g_force_stockham < 0) {
+    const char* v = getenv("PSX_FORCE_STOCKHAM");
+    g_force_stockham = (v && v[0] == '1') ? 1 : 0;
+  }
+  if (g_force_stockham != 1 && (f.do_fwd || f.do_inv)) {
+    bool done = true;
+    switch (a.n) {
+      case 16: launch_cline<T, 4, 4>(a, prec, f, s); break;
+      case 32: launch_cline<T, 8, 4>(a, prec, f, s); break;
+      case 48: launch_cline<T, 12, 4>(a, prec, f, s); break;
+      case 64: launch_cline<T, 8, 8>(a, prec, f, s); break;
+      case 96: launch_cline<T, 12, 8>(a, prec, f, s); break;
+      case 128: launch_cline<T, 8, 16>(a, prec, f, s); break;
+      case 256: launch_cline<T, 16, 16>(a, prec, f, s); break;
+      default: done = false;
+    }
+    if (done) {
+      PSX_HIP(hipGetLastError());
+      return PSX_OK;
+    }
+  }
   const int P = cplx_tile(a);
   LineGeom g = geom(a, prec, P);
   const size_t smem = 2 * (size_t)P * g.ls * 2 * sizeof(T);

@@ -21,7 +21,8 @@ def rel(a, b):
 
 
 @pytest.mark.parametrize("shape", [(8, 12, 10), (16, 16, 16), (4, 6, 8), (3, 5, 1),
-                                   (64, 64, 1), (7, 1, 1), (1, 1, 1), (2, 9, 10)])
+                                   (64, 64, 1), (7, 1, 1), (1, 1, 1), (2, 9, 10),
+                                   (128, 6, 5), (5, 96, 48), (32, 3, 128), (256, 2, 3)])
 def test_fft_pair_fp64(shape):
     print(f"fft pair fp64 {shape}", flush=True)
     rng = np.random.default_rng(11)

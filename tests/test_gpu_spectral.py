@@ -25,7 +25,8 @@ def rel(a, b):
 
 
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-5)])
-@pytest.mark.parametrize("shape", [(16, 12, 10), (32, 32, 32), (16, 16, 1), (12, 1, 1), (6, 5, 3)])
+@pytest.mark.parametrize("shape", [(16, 12, 10), (32, 32, 32), (16, 16, 1), (12, 1, 1), (6, 5, 3),
+                                   (128, 4, 6), (4, 96, 8), (64, 48, 2), (256, 1, 1)])
 def test_fft_family(shape, prec, tol):
     print(f"fft family {shape} {prec}", flush=True)
     rng = np.random.default_rng(11)
