@@ -146,13 +146,13 @@ def batch_leg(n_inst=64, steps=5, warmup=2, seed=4, sequential=True):
                 speedup=seq / dt, autotune_choice=choice)
 
 
-def cpu_baseline(w, seconds_target=12.0):
-    """Faithful single-thread oracle (full-grid complex FFT pair, separate
-    element-wise passes) timed on a bounded sample of contour steps of the
-    same grid, scaled to one iteration (2 Ns contour steps; the O(N) work of
-    an iteration is < 1% of it and is left out)."""
+def cpu_baseline(w, seconds_target=12.0, threads=1):
+    """Faithful oracle (full-grid complex FFT pair, separate element-wise
+    passes) on `threads` host threads, timed on a bounded sample of contour
+    steps of the same grid, scaled to one iteration (2 Ns contour steps; the
+    O(N) work of an iteration is < 1% of it and is left out)."""
     import oracle as O
-    O.set_threads(1)
+    O.set_threads(threads)
     shape = w["shape"]
     Ns = int(round(1.0 / w["ds"]))
     t1 = O.time_steps(shape, 1)
@@ -160,7 +160,8 @@ def cpu_baseline(w, seconds_target=12.0):
     t = O.time_steps(shape, steps)
     per_step = t / steps
     it_s = 1.0 / (2 * Ns * per_step)
-    return {"value": it_s, "unit": "SCFT iter/s", "cores": 1, "kind": "port",
+    O.set_threads(1)
+    return {"value": it_s, "unit": "SCFT iter/s", "cores": threads, "kind": "port",
             "sample": f"{steps} faithful FP64 contour steps on {shape[0]}x{shape[1]}x{shape[2]} "
                       f"({per_step:.3f} s/step), scaled to 2*Ns={2 * Ns} steps per iteration"}
 
@@ -244,10 +245,14 @@ def main():
         batch = batch_leg()
         print(f"[bench] batch: {batch}", file=sys.stderr, flush=True)
 
-    cpu = None
+    cpu = cpu_mt = None
     if not args.no_cpu:
         print("[bench] cpu baseline (oracle, 1 thread)", file=sys.stderr, flush=True)
         cpu = cpu_baseline(w)
+        nthr = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        nthr = max(1, min(nthr, 16))
+        print(f"[bench] cpu baseline (oracle, {nthr} threads)", file=sys.stderr, flush=True)
+        cpu_mt = cpu_baseline(w, seconds_target=6.0, threads=nthr)
 
     traffic = counter_traffic()
     m = legs["fp32"]
@@ -291,6 +296,7 @@ def main():
         "accuracy": acc,
         "batch": batch,
         "cpu_baseline": cpu,
+        "cpu_baseline_threads": cpu_mt,
     }
     print(json.dumps(out))
 
