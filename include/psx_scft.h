@@ -329,14 +329,18 @@ psx_status psx_scft_param_grad(psx_scft* e, double* h_dFdchiN,
 
 /* Average device time (ms) of one contour step of the dual sweep and of
  * each of its line passes (z, y, x), timed with hipEvents on the engine's
- * stream over `reps` repetitions. */
+ * stream over `reps` repetitions.  In plane mode (psx_scft_set_sweep_mode)
+ * the first figure is the plane launches (z and y together), the second 0. */
 psx_status psx_scft_time_step(psx_scft* e, int reps, double* h_ms_step,
                               double* h_ms_pass3);
 
 /* Time the candidate line kernels of each pass (register splits, LDS
  * Stockham) on this engine's data and keep the fastest; h_choice (3 ints,
  * may be NULL) receives the chosen variant per pass (see
- * psx_scft_set_variant).  No reference counterpart. */
+ * psx_scft_set_variant).  Where plane launches are available (and the
+ * engine is not a batch's), whole sweeps in both sweep modes are timed
+ * with the tuned line kernels and the faster mode is kept.  No reference
+ * counterpart. */
 psx_status psx_scft_autotune(psx_scft* e, int reps, int* h_choice);
 /* Choose the line kernel of pass `pass` (grid axes with n > 1 from the
  * last, z, to the first, x) for the current precision: -1 the default for
@@ -348,6 +352,22 @@ psx_status psx_scft_autotune(psx_scft* e, int reps, int* h_choice);
  * that does not apply to a pass uses the default.  No reference
  * counterpart. */
 psx_status psx_scft_set_variant(psx_scft* e, int pass, int variant);
+/* How a sweep is launched (no reference counterpart; results agree to
+ * rounding).  mode 0: line launches, two per contour step (a strided pass
+ * alone, then a fused pass that ends one step and starts the next).  mode 1:
+ * plane launches, one per contour step -- each x-plane's z and y round
+ * trips of two consecutive steps run in one launch that stages the plane in
+ * LDS, alternating with the fused x launches, so a step moves 4 fields
+ * through HBM instead of 6.  Mode 1 needs a 3D grid with ny == nz in
+ * {32, 64, 96, 128} (PSX_ERR_INVALID otherwise) and is the default there.
+ * Switching rebuilds the sweep buffers: fields are kept, propagate again.
+ * Engines of a batch switch through psx_batch_set_sweep_mode
+ * (PSX_ERR_STATE here).  The environment variable PSX_PLANE=0 makes line
+ * launches the default of new engines. */
+psx_status psx_scft_set_sweep_mode(psx_scft* e, int mode);
+/* Current sweep mode and whether plane launches are available (either
+ * pointer may be NULL). */
+psx_status psx_scft_sweep_mode(const psx_scft* e, int* h_mode, int* h_available);
 
 /* ===================================================================== */
 /* Batched parameter sweeps: many independent diblock instances (grid,     */
@@ -371,8 +391,16 @@ psx_status psx_batch_set_precision(psx_batch* b, psx_prec prec);
 /* Per grid size and pass, time the candidate line kernels on the batch's
  * own fields and keep the fastest; h_choice (3 per size class, may be
  * NULL) receives the chosen variants (-1 register default, 1 Stockham,
- * 2 alternative register split). */
+ * 2 alternative register split).  Classes where plane launches are
+ * available then time whole class sweeps in both sweep modes and keep the
+ * faster. */
 psx_status psx_batch_autotune(psx_batch* b, int reps, int* h_choice);
+/* Sweep mode of size class cls (see psx_scft_set_sweep_mode; class order
+ * is that of first appearance among the instances).  Planes are the
+ * default for classes of n < 96 and for classes of at most four
+ * instances. */
+psx_status psx_batch_set_sweep_mode(psx_batch* b, int cls, int mode);
+psx_status psx_batch_sweep_mode(const psx_batch* b, int cls, int* h_mode);
 /* n_iters iterations of every instance; histories are n * n_iters, row-major
  * by instance (may be NULL). */
 psx_status psx_batch_iterate(psx_batch* b, const psx_mixing* mix, int n_iters,

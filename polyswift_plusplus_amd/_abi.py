@@ -110,6 +110,8 @@ EXPORTS = {
     "psx_scft_param_grad": (ctypes.c_int, [_vp, _dp, _dp]),
     "psx_scft_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
     "psx_scft_set_variant": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "psx_scft_set_sweep_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "psx_scft_sweep_mode": (ctypes.c_int, [_vp, _ip, _ip]),
     "psx_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Grid), ctypes.POINTER(Diblock),
                                         ctypes.c_int, _vp, ctypes.POINTER(_vp)]),
     "psx_batch_destroy": (ctypes.c_int, [_vp]),
@@ -117,6 +119,8 @@ EXPORTS = {
     "psx_batch_instance": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp)]),
     "psx_batch_set_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
     "psx_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
+    "psx_batch_set_sweep_mode": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "psx_batch_sweep_mode": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
     "psx_batch_iterate": (ctypes.c_int, [_vp, ctypes.POINTER(Mixing), ctypes.c_int, ctypes.c_int,
                                          _dp, _dp, _ip]),
     "psx_scft_time_step": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),

@@ -18,7 +18,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpsx_scft.so")
 SOURCES = [os.path.join(CSRC, "psx_scft.hip")]
 DEPS = SOURCES + [os.path.join(CSRC, f) for f in ("psx_kernels.hpp", "psx_fft_lds.hpp", "psx_fft_reg.hpp",
-                                                  "psx_fft_cline.hpp", "psx_filter.hpp", "psx_rng.hpp")] + [
+                                                  "psx_fft_cline.hpp", "psx_fft_plane.hpp", "psx_filter.hpp",
+                                                  "psx_rng.hpp")] + [
     os.path.join(ROOT, "include", "psx_scft.h")]
 
 HIP_FLAGS = [

@@ -1,0 +1,164 @@
+// psx_fft_plane.hpp -- both in-plane axes of a contour step in one launch.
+//
+// A 3D grid (n_x, n, n) stores each x-plane as one contiguous n x n block
+// (z fastest).  The diffusion operator of a step is the product of three
+// commuting 1D round trips (z, y, x), so the z and y round trips of one
+// plane need no data from any other plane: a block stages its plane in LDS,
+// runs the z round trips (two rows per complex line) and the y round trips
+// (two columns per complex line) with the register line FFT of
+// psx_fft_reg.hpp, and writes the plane back.  Like the fused line launches,
+// one launch may join two contour steps: the plane's result of the first
+// step is multiplied by post, stored (mid), multiplied by pre2 and carried on
+// in LDS through the second step's y and z round trips.
+//
+// Against the line launches this moves 4 fields per contour step instead of
+// 6: the step's x round trip stays a (fused) strided line launch, and the y
+// and z round trips of a step pair become one plane launch.
+#pragma once
+#include "psx_fft_reg.hpp"
+
+namespace psx {
+
+// Spectral factors of a plane launch: emul (and emul2) point at 2n values,
+// the z factors followed by the y factors (each includes 1/n).
+//
+// NW waves of 64 / N2 complex lines each cover the plane's n / 2 complex
+// lines (2 NW 64 / N2 == n).  REUSE: the waves' transpose buffers alias the
+// plane (block barriers around each pass) so the block needs only the plane's
+// LDS; otherwise they sit behind it and a pass needs no barrier of its own
+// (every wave reads and writes only its own lines of the plane).
+template <typename T, int N1, int N2, int NW, bool REUSE>
+struct PlaneShape {
+  static constexpr int n = N1 * N2;
+  static constexpr int CL = 64 / N2;                       // complex lines per wave
+  static constexpr int RS = n + (sizeof(T) == 4 ? 4 : 2);  // LDS row stride (elements)
+  static constexpr int PB = n * RS * (int)sizeof(T);       // plane bytes
+  static constexpr int TB = NW * 64 * N1 * 2 * (int)sizeof(T);   // transpose buffers
+  static constexpr int BUF = ((REUSE ? (PB > TB ? PB : TB) : PB + TB) + 15) / 16 * 16;
+  // + twiddles (n complex) + up to four n-point spectral factor rows
+  static constexpr int SMEM = BUF + n * 2 * (int)sizeof(T) + 4 * n * (int)sizeof(T);
+  static_assert(2 * NW * CL == n, "the waves must cover the plane's complex lines");
+};
+
+// Round trips along z, then y (second step: y, then z) of one x-plane per
+// block.  g.ntiles = planes per field (n_x), g.nfields = fields; PairIO as
+// for the line launches with the plane spectral tables above.
+template <typename T, int N1, int N2, int NW, bool REUSE>
+__global__ void __launch_bounds__(64 * NW)
+k_plane(LineGeom g, const PairIO* __restrict__ fio) {
+  typedef typename Cx<T>::type C;
+  typedef typename Vec16<T>::type V;
+  typedef PlaneShape<T, N1, N2, NW, REUSE> S;
+  constexpr int n = S::n, CL = S::CL, RS = S::RS;
+  constexpr int VW = 16 / (int)sizeof(T);
+  constexpr int CPR = n / VW;          // 16-byte chunks per plane row
+  constexpr int NCH = n * CPR;
+  static_assert(n % VW == 0 && RS % VW == 0, "plane rows must be whole 16-byte chunks");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* P = (T*)smem_raw;
+  const int f = blockIdx.x / g.ntiles, x = blockIdx.x - f * g.ntiles;
+  const PairIO fd = fio[f];
+  const bool fused = fd.emul2 != nullptr;
+  const long long base = (long long)x * n * n;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int p = lane / N2, t = lane % N2;
+  const int cl = wave * CL + p;        // this lane's complex line: rows / columns 2cl, 2cl+1
+  C* L = (C*)(smem_raw + (REUSE ? 0 : S::PB)) + wave * (64 * N1) + p * n;
+  const C* __restrict__ tw = (const C*)g.tw;
+  C* twl = (C*)(smem_raw + S::BUF);
+  T* em = (T*)(twl + n);               // e_z, e_y of the first step, then of the second
+  for (int i = tid; i < n; i += 64 * NW) twl[i] = tw[i];
+  for (int i = tid; i < 2 * n; i += 64 * NW) {
+    em[i] = ((const T*)fd.emul)[i];
+    if (fused) em[2 * n + i] = ((const T*)fd.emul2)[i];
+  }
+  {
+    const T* __restrict__ src = (const T*)fd.in;
+    const T* __restrict__ pre = (const T*)fd.pre;
+#pragma unroll 4
+    for (int c = tid; c < NCH; c += 64 * NW) {
+      const int row = c / CPR, col = (c - row * CPR) * VW;
+      const long long ga = base + (long long)row * n + col;
+      V v = *(const V*)(src + ga);
+      if (pre) { const V w = *(const V*)(pre + ga); vmul(v, w); }
+      vstore_lds(P + row * RS + col, v);
+    }
+  }
+  __syncthreads();
+
+  auto zpass = [&](const T* ez) {
+    C v[N1];
+#pragma unroll
+    for (int r = 0; r < N1; ++r) {
+      const int k = t + N2 * r;
+      v[r].x = P[(2 * cl) * RS + k];
+      v[r].y = P[(2 * cl + 1) * RS + k];
+    }
+    if (REUSE) __syncthreads();
+    line_roundtrip<T, N1, N2, true>(v, L, tw, twl, ez, t);
+    if (REUSE) __syncthreads();
+    else lds_sync<true>();
+#pragma unroll
+    for (int r = 0; r < N1; ++r) {
+      const int k = t + N2 * r;
+      P[(2 * cl) * RS + k] = v[r].x;
+      P[(2 * cl + 1) * RS + k] = v[r].y;
+    }
+  };
+  auto ypass = [&](const T* ey) {
+    C v[N1];
+#pragma unroll
+    for (int r = 0; r < N1; ++r) v[r] = *(const C*)(P + (t + N2 * r) * RS + 2 * cl);
+    if (REUSE) __syncthreads();
+    line_roundtrip<T, N1, N2, true>(v, L, tw, twl, ey, t);
+    if (REUSE) __syncthreads();
+    else lds_sync<true>();
+#pragma unroll
+    for (int r = 0; r < N1; ++r) *(C*)(P + (t + N2 * r) * RS + 2 * cl) = v[r];
+  };
+
+  zpass(em);
+  __syncthreads();
+  ypass(em + n);
+  __syncthreads();
+  {
+    T* __restrict__ dst = (T*)(fused ? fd.mid : fd.out);
+    const T* __restrict__ post = (const T*)fd.post;
+    const T* __restrict__ pre2 = fused ? (const T*)fd.pre2 : nullptr;
+#pragma unroll 4
+    for (int c = tid; c < NCH; c += 64 * NW) {
+      const int row = c / CPR, col = (c - row * CPR) * VW;
+      const long long ga = base + (long long)row * n + col;
+      V v = vload_lds<T>(P + row * RS + col);
+      V w;
+      if (post) { w = *(const V*)(post + ga); vmul(v, w); }
+      *(V*)(dst + ga) = v;
+      if (fused) {
+        if (pre2) {
+          if (pre2 != post) w = *(const V*)(pre2 + ga);
+          vmul(v, w);
+        }
+        vstore_lds(P + row * RS + col, v);
+      }
+    }
+  }
+  if (!fused) return;
+  __syncthreads();
+  ypass(em + 3 * n);
+  __syncthreads();
+  zpass(em + 2 * n);
+  __syncthreads();
+  {
+    const T* __restrict__ post2 = (const T*)fd.post2;
+#pragma unroll 4
+    for (int c = tid; c < NCH; c += 64 * NW) {
+      const int row = c / CPR, col = (c - row * CPR) * VW;
+      const long long ga = base + (long long)row * n + col;
+      V v = vload_lds<T>(P + row * RS + col);
+      if (post2) { const V w = *(const V*)(post2 + ga); vmul(v, w); }
+      *(V*)((T*)fd.out + ga) = v;
+    }
+  }
+}
+
+}  // namespace psx

@@ -24,6 +24,7 @@
 #include "psx_kernels.hpp"
 #include "psx_fft_reg.hpp"
 #include "psx_fft_cline.hpp"
+#include "psx_fft_plane.hpp"
 #include "psx_filter.hpp"
 #include "psx_rng.hpp"
 
@@ -333,6 +334,47 @@ psx_status launch_pair(const Axis& a, int prec, const PairIO* fio,
       case 256: launch_reg<T, 16, 16>(a, prec, fio, nfields, s); break;
       default: launch_stockham<T>(a, prec, fio, nfields, s); break;
     }
+  }
+  PSX_HIP(hipGetLastError());
+  return PSX_OK;
+}
+
+// Plane launches (k_plane): the z and y round trips of whole x-planes, for
+// 3D grids with n_y = n_z = n of a register split below.
+bool plane_length(int n) { return n == 32 || n == 64 || n == 96 || n == 128; }
+
+template <typename T, int N1, int N2, int NW, bool REUSE>
+psx_status plane_kernel(const Axis& az, int prec, const PairIO* fio, int nfields, int nplanes,
+                        hipStream_t s, bool prepare) {
+  typedef PlaneShape<T, N1, N2, NW, REUSE> S;
+  static_assert(S::SMEM <= 160 * 1024, "plane kernel exceeds the LDS of a CU");
+  if (prepare) {
+    PSX_HIP(hipFuncSetAttribute((const void*)k_plane<T, N1, N2, NW, REUSE>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, S::SMEM));
+    return PSX_OK;
+  }
+  LineGeom g = geom(az, prec, 0);
+  g.ntiles = nplanes;
+  g.nfields = nfields;
+  hipLaunchKernelGGL((k_plane<T, N1, N2, NW, REUSE>), dim3(nplanes * nfields), dim3(64 * NW), S::SMEM, s, g, fio);
+  return PSX_OK;
+}
+
+// One plane launch over nfields fields of nplanes planes; az is the z axis
+// (its twiddles serve y too).  prepare: only raise the kernel's LDS limit
+// (once per length and type, outside any stream capture).  Transpose
+// buffers behind the plane (REUSE false) where two blocks per CU still fit,
+// else aliasing it (measured at 128^3 and 96^3 x 38 fields, both orders).
+template <typename T>
+psx_status launch_plane(const Axis& az, int prec, const PairIO* fio, int nfields, int nplanes,
+                        hipStream_t s, bool prepare = false) {
+  constexpr bool F32 = sizeof(T) == 4;
+  switch (az.n) {
+    case 32: PSX_TRY((plane_kernel<T, 8, 4, 1, false>(az, prec, fio, nfields, nplanes, s, prepare))); break;
+    case 64: PSX_TRY((plane_kernel<T, 8, 8, 4, F32>(az, prec, fio, nfields, nplanes, s, prepare))); break;
+    case 96: PSX_TRY((plane_kernel<T, 12, 8, 6, F32>(az, prec, fio, nfields, nplanes, s, prepare))); break;
+    case 128: PSX_TRY((plane_kernel<T, 8, 16, 16, !F32>(az, prec, fio, nfields, nplanes, s, prepare))); break;
+    default: return fail(PSX_ERR_INVALID, "no plane kernel for n = " + std::to_string(az.n));
   }
   PSX_HIP(hipGetLastError());
   return PSX_OK;
@@ -841,6 +883,7 @@ struct SweepBufs {
   std::vector<PathBufs> path;
   void* wf[2] = {nullptr, nullptr};          // exp(-ds w / 2) of species A, B (T)
   std::vector<std::array<void*, 3>> emul;    // [b class][axis idx] spectral factors (T)
+  std::vector<void*> epl;          // [b class] plane launches: z factors, then y factors (T)
   std::vector<PairIO*> rtab;       // per round: device table [launch][path slot]
   bool wfac_valid = false;
   // contour checkpointing (linear chains; see CkptSeg)
@@ -914,6 +957,9 @@ struct psx_scft {
   int nck = 0;                          // checkpoint slices
   std::vector<CkptSeg> segs;
   std::vector<int> btab_axis;
+  // z and y passes as plane launches (k_plane; 3D grids with n_y = n_z of a
+  // plane length); on by default where available
+  bool plane = false;
   bool in_batch = false;                // owned by a psx_batch (its tables point at our slices)
   psx_fft* fplan = nullptr;             // FP64 plan of the spectral filters (first use)
   long long graph_gen = 0;              // bumped whenever captured graphs go stale
@@ -981,18 +1027,36 @@ std::vector<double> axis_emul(const Axis& a, double cell, double ds, double b) {
 // commute), so the last pass of step s and the first of step s+1 run along
 // the same axis and are fused into one launch (two round trips, the step's
 // result stored in between).  Both sweeps share the list.
+//
+// Plane mode (3D, see k_plane): every run of consecutive z and y passes is
+// one launch with axis code kPlaneAxis.  Such a run is (0 first, 1) at the
+// start of a sweep, (1, 0 last) at its end, and (1, 0 last | 0 first, 1)
+// across each step boundary in between, so the launches alternate between
+// planes and fused x passes: one launch per contour step.
 struct SweepOp { int axis, step; bool first, last; };
+constexpr int kPlaneAxis = 3;
 
-std::vector<std::pair<SweepOp, int>> sweep_launches(int na, int Ns) {
+std::vector<std::pair<SweepOp, int>> sweep_launches(int na, int Ns, bool plane = false) {
   std::vector<SweepOp> ops;
   for (int st = 0; st < Ns; ++st)
     for (int p = 0; p < na; ++p) {
       const int a = (st % 2 == 0) ? p : na - 1 - p;
       ops.push_back(SweepOp{a, st, p == 0, p == na - 1});
     }
-  std::vector<std::pair<SweepOp, int>> out;   // (first op, number of ops: 1 or 2)
+  // (first op, number of ops: 1 or 2; plane launches 2 or 4).  A plane
+  // launch's op has first = its first op starts a step, last = (2 ops) its
+  // second op ends the step.
+  std::vector<std::pair<SweepOp, int>> out;
   for (size_t i = 0; i < ops.size();) {
-    if (ops[i].last && i + 1 < ops.size() && ops[i + 1].axis == ops[i].axis) {
+    if (plane && na == 3 && ops[i].axis <= 1) {
+      size_t j = i;
+      while (j < ops.size() && ops[j].axis <= 1) ++j;
+      SweepOp o = ops[i];
+      o.axis = kPlaneAxis;
+      o.last = (j - i == 2) && ops[j - 1].last;
+      out.push_back({o, (int)(j - i)});
+      i = j;
+    } else if (ops[i].last && i + 1 < ops.size() && ops[i + 1].axis == ops[i].axis) {
       out.push_back({ops[i], 2});
       i += 2;
     } else {
@@ -1012,18 +1076,33 @@ std::vector<PairIO> range_fio(const psx_scft* e, int prec, int p, int s0, int ns
   const SweepBufs& s = e->sw[prec];
   const PathInfo& pi = e->paths[p];
   const int na = (int)e->axes.size();
-  const auto launches = sweep_launches(na, nsteps);
+  const auto launches = sweep_launches(na, nsteps, e->plane);
   std::vector<PairIO> tab(launches.size());
   void* tmp = s.path[p].tmp;
   auto blk = [&](int st) -> const BlockInfo& { return e->blocks[pi.step_block[s0 + st]]; };
   auto W = [&](int st) { return (const void*)s.wf[blk(st).species]; };
   auto E = [&](int a, int st) { return (const void*)s.emul[blk(st).bclass][a]; };
+  auto EP = [&](int st) { return (const void*)s.epl[blk(st).bclass]; };
   for (size_t j = 0; j < launches.size(); ++j) {
     const SweepOp& o = launches[j].first;
     PairIO& f = tab[j];
     std::memset(&f, 0, sizeof(f));
     f.in = o.first ? slice(o.step) : tmp;
     f.pre = o.first ? W(o.step) : nullptr;
+    if (o.axis == kPlaneAxis) {
+      f.emul = EP(o.step);
+      if (launches[j].second == 2) {
+        f.post = o.last ? W(o.step) : nullptr;
+        f.out = o.last ? slice(o.step + 1) : tmp;
+      } else {   // (1, 0 last | 0 first, 1): the step's result is stored between
+        f.post = W(o.step);
+        f.mid = slice(o.step + 1);
+        f.pre2 = W(o.step + 1);
+        f.emul2 = EP(o.step + 1);
+        f.out = tmp;
+      }
+      continue;
+    }
     f.emul = E(o.axis, o.step);
     if (launches[j].second == 1) {
       f.post = o.last ? W(o.step) : nullptr;
@@ -1085,7 +1164,7 @@ void ckpt_plan(psx_scft* e) {
   e->nck = Ns / K + 1 + (Ns % K ? 1 : 0);
   const int na = (int)e->axes.size();
   e->btab_axis.clear();
-  for (const auto& l : sweep_launches(na, Ns)) e->btab_axis.push_back(l.first.axis);
+  for (const auto& l : sweep_launches(na, Ns, e->plane)) e->btab_axis.push_back(l.first.axis);
   e->segs.clear();
   const int nseg = (Ns + K - 1) / K;
   for (int j = nseg - 1; j >= 0; --j) {
@@ -1094,7 +1173,7 @@ void ckpt_plan(psx_scft* e) {
     c.u1 = std::min((j + 1) * K, Ns);
     c.t0 = Ns - c.u1;
     c.t1 = Ns - c.u0;
-    for (const auto& l : sweep_launches(na, c.u1 - c.u0)) {
+    for (const auto& l : sweep_launches(na, c.u1 - c.u0, e->plane)) {
       c.g_axis.push_back(l.first.axis);
       c.f_axis.push_back(l.first.axis);
     }
@@ -1143,6 +1222,15 @@ psx_status ensure_ckpt(psx_scft* e, int prec) {
   return PSX_OK;
 }
 
+// Spectral factors of a plane launch for b class c: the z factors followed
+// by the y factors (both include 1/n).
+std::vector<double> plane_factors(const psx_scft* e, size_t c) {
+  std::vector<double> em = axis_emul(e->axes[0], e->cell[e->axes[0].axis], e->ds, e->bval[c]);
+  const std::vector<double> ey = axis_emul(e->axes[1], e->cell[e->axes[1].axis], e->ds, e->bval[c]);
+  em.insert(em.end(), ey.begin(), ey.end());
+  return em;
+}
+
 psx_status ensure_sweep(psx_scft* e, int prec) {
   SweepBufs& s = e->sw[prec];
   if (s.ready) return PSX_OK;
@@ -1167,6 +1255,19 @@ psx_status ensure_sweep(psx_scft* e, int prec) {
         PSX_TRY(upload(&s.emul[c][i], f.data(), f.size() * 4));
       }
     }
+  }
+  if (e->plane) {
+    s.epl.assign(e->bval.size(), nullptr);
+    for (size_t c = 0; c < e->bval.size(); ++c) {
+      const std::vector<double> em = plane_factors(e, c);
+      if (prec == PSX_FP64) PSX_TRY(upload(&s.epl[c], em.data(), em.size() * 8));
+      else {
+        std::vector<float> f = cast_vec<float>(em);
+        PSX_TRY(upload(&s.epl[c], f.data(), f.size() * 4));
+      }
+    }
+    if (prec == PSX_FP64) PSX_TRY(launch_plane<double>(e->axes[0], prec, nullptr, 0, 0, e->stream, true));
+    else PSX_TRY(launch_plane<float>(e->axes[0], prec, nullptr, 0, 0, e->stream, true));
   }
   const int g = grid_1d(e->N);
   if (e->ckpt_K > 0) {
@@ -1206,6 +1307,8 @@ void free_sweep(SweepBufs& s) {
   s.rtab.clear();
   for (auto& b : s.emul) for (auto& p : b) dfree(p);
   s.emul.clear();
+  for (auto& p : s.epl) dfree(p);
+  s.epl.clear();
   s.ready = false;
 }
 
@@ -1248,6 +1351,15 @@ psx_status launch_junctions(psx_scft* e, int r) {
   return PSX_OK;
 }
 
+// One launch of a sweep list: a line pass along axis a with its variant, or
+// a plane launch (a == kPlaneAxis) over the engine's x-planes.
+template <typename T>
+psx_status launch_sweep_op(const psx_scft* e, int a, int prec, const PairIO* fio, int nfields,
+                           hipStream_t s, const int* variant) {
+  if (a == kPlaneAxis) return launch_plane<T>(e->axes[0], prec, fio, nfields, e->dims[0], s);
+  return launch_pair<T>(e->axes[a], prec, fio, nfields, s, variant[a]);
+}
+
 // All propagators, round by round.  Within a round every path advances on
 // its own stream slot (forked from and joined back into the engine stream;
 // inside a capture these become parallel graph branches), or, with
@@ -1271,7 +1383,7 @@ psx_status sweep_range(psx_scft* e, int only_axis = -1, bool quad_chunks = false
       for (size_t j = 0; j < p0.launch_axis.size(); ++j) {
         const int a = p0.launch_axis[j];
         if (only_axis >= 0 && a != only_axis) continue;
-        PSX_TRY(launch_pair<T>(e->axes[a], e->prec, tab + j * np, np, e->stream, e->variant[e->prec][a]));
+        PSX_TRY(launch_sweep_op<T>(e, a, e->prec, tab + j * np, np, e->stream, e->variant[e->prec]));
       }
       continue;
     }
@@ -1291,7 +1403,7 @@ psx_status sweep_range(psx_scft* e, int only_axis = -1, bool quad_chunks = false
         const int a = pi.launch_axis[j];
         if (only_axis >= 0 && a != only_axis) continue;
         hipStream_t st = (fork && k > 0) ? e->pstream[k] : e->stream;
-        PSX_TRY(launch_pair<T>(e->axes[a], e->prec, tab + j * np + k, 1, st, e->variant[e->prec][a]));
+        PSX_TRY(launch_sweep_op<T>(e, a, e->prec, tab + j * np + k, 1, st, e->variant[e->prec]));
         if (quad_chunks)
           for (size_t c = 0; c + 1 < e->qchunks.size(); ++c)
             if (e->qchunks[c].launch == (int)j) PSX_HIP(hipEventRecord(e->qchunks[c].ev[k], st));
@@ -1325,7 +1437,7 @@ psx_status sweep_ckpt(psx_scft* e) {
   const PairIO* tab = e->sw[e->prec].btab;
   for (size_t j = 0; j < e->btab_axis.size(); ++j) {
     const int a = e->btab_axis[j];
-    PSX_TRY(launch_pair<T>(e->axes[a], e->prec, tab + j, 1, e->stream, e->variant[e->prec][a]));
+    PSX_TRY(launch_sweep_op<T>(e, a, e->prec, tab + j, 1, e->stream, e->variant[e->prec]));
   }
   return PSX_OK;
 }
@@ -1354,7 +1466,7 @@ psx_status ckpt_chunk_sweeps(psx_scft* e, size_t ci) {
                      (const T*)((const char*)s.ck + (size_t)j * sb), e->N, (T*)s.seg);
   for (size_t k = 0; k < c.g_axis.size(); ++k) {
     const int a = c.g_axis[k];
-    PSX_TRY(launch_pair<T>(e->axes[a], e->prec, s.gtab[ci] + k, 1, s2, e->variant[e->prec][a]));
+    PSX_TRY(launch_sweep_op<T>(e, a, e->prec, s.gtab[ci] + k, 1, s2, e->variant[e->prec]));
   }
   T* cur = (T*)s.fch[ci % 2];
   if (ci == 0) {
@@ -1366,7 +1478,7 @@ psx_status ckpt_chunk_sweeps(psx_scft* e, size_t ci) {
   }
   for (size_t k = 0; k < c.f_axis.size(); ++k) {
     const int a = c.f_axis[k];
-    PSX_TRY(launch_pair<T>(e->axes[a], e->prec, s.ftab[ci] + k, 1, e->stream, e->variant[e->prec][a]));
+    PSX_TRY(launch_sweep_op<T>(e, a, e->prec, s.ftab[ci] + k, 1, e->stream, e->variant[e->prec]));
   }
   if (s2 != e->stream) {
     PSX_HIP(hipEventRecord(e->ev_join[1], s2));
@@ -1744,6 +1856,25 @@ static std::vector<double> all_weights(const psx_scft* e, int mode) {
 }
 
 // Blocks, junctions and propagation paths of the graph d[0..nb).
+// Launch lists of every path for the engine's sweep mode.
+static void plan_launches(psx_scft* e) {
+  const int na = (int)e->axes.size();
+  for (PathInfo& pi : e->paths) {
+    pi.launch_axis.clear();
+    pi.launch_nops.clear();
+    for (const auto& l : sweep_launches(na, pi.len, e->plane)) {
+      pi.launch_axis.push_back(l.first.axis);
+      pi.launch_nops.push_back(l.second);
+    }
+  }
+}
+
+// Plane launches need a 3D grid whose y and z lengths are equal and one of
+// the plane kernel's lengths.
+static bool plane_available(const psx_scft* e) {
+  return e->axes.size() == 3 && e->axes[0].n == e->axes[1].n && plane_length(e->axes[0].n);
+}
+
 static psx_status build_graph(psx_scft* e, int nb, const psx_block* d) {
   if (nb < 1 || nb > PSX_MAX_BLOCKS)
     return fail(PSX_ERR_INVALID, "block count must be in [1, " + std::to_string(PSX_MAX_BLOCKS) + "]");
@@ -1844,11 +1975,9 @@ static psx_status build_graph(psx_scft* e, int nb, const psx_block* d) {
     PathInfo& pi = e->paths[p];
     pi.slot = (int)e->rounds[pi.round].size();
     e->rounds[pi.round].push_back((int)p);
-    for (const auto& l : sweep_launches(na, pi.len)) {
-      pi.launch_axis.push_back(l.first.axis);
-      pi.launch_nops.push_back(l.second);
-    }
   }
+  (void)na;
+  plan_launches(e);
   // species totals, contour steps
   e->fA = e->fB = 0.0;
   e->Ns = 0;
@@ -1865,6 +1994,19 @@ static psx_status build_graph(psx_scft* e, int nb, const psx_block* d) {
 // Chunks of contour points for the quadrature under the sweeps (see
 // psx_scft::qchunks): oriented linear chains, two paths in one round.
 static void plan_quad_overlap(psx_scft* e) {
+  // a re-plan (new launch lists) keeps the chunks' events
+  std::vector<std::array<hipEvent_t, 2>> evs;
+  for (const auto& c : e->qchunks) evs.push_back({c.ev[0], c.ev[1]});
+  struct Restore {
+    psx_scft* e;
+    std::vector<std::array<hipEvent_t, 2>>& evs;
+    ~Restore() {
+      for (size_t c = 0; c < evs.size() && c < e->qchunks.size(); ++c) {
+        e->qchunks[c].ev[0] = evs[c][0];
+        e->qchunks[c].ev[1] = evs[c][1];
+      }
+    }
+  } restore{e, evs};
   e->qchunks.clear();
   if (!e->overlap_quad || e->rounds.size() != 1 || e->paths.size() != 2) return;
   if (!e->blocks[0].cnt[0].empty() || 2 * e->blocks.size() > (size_t)kMaxBlocks) return;
@@ -1876,10 +2018,15 @@ static void plan_quad_overlap(psx_scft* e) {
   const int na = (int)e->axes.size();
   std::vector<int> written;
   int w = 0;
-  for (const auto& l : sweep_launches(na, Ns)) {
+  for (const auto& l : sweep_launches(na, Ns, e->plane)) {
     const SweepOp& o = l.first;
-    if (l.second == 2) w = std::max(w, o.step + 1 + (na == 1 ? 1 : 0));
-    else if (o.last) w = std::max(w, o.step + 1);
+    if (o.axis == kPlaneAxis) {
+      if (l.second == 4 || o.last) w = std::max(w, o.step + 1);
+    } else if (l.second == 2) {
+      w = std::max(w, o.step + 1 + (na == 1 ? 1 : 0));
+    } else if (o.last) {
+      w = std::max(w, o.step + 1);
+    }
     written.push_back(w);
   }
   // point t is ready at step r(t) = max(t, Ns - t) in [rmin, Ns]
@@ -1920,6 +2067,8 @@ static psx_status create_engine(const psx_grid* g, int nb, const psx_block* bloc
   e->chiN = chiN;
   e->prec = prec;
   psx_status st = make_axes(e->dims, &e->axes);
+  e->plane = plane_available(e);
+  if (const char* v = getenv("PSX_PLANE")) e->plane = e->plane && v[0] != '0';
   if (st == PSX_OK) st = build_graph(e, nb, blocks);
   if (st != PSX_OK) { delete e; return st; }
 #define E_TRY(x) do { if (st == PSX_OK) st = (x); } while (0)
@@ -2440,6 +2589,15 @@ static psx_status write_emul(psx_scft* e, int prec) {
       }
     }
   }
+  for (size_t c = 0; c < s.epl.size(); ++c) {   // the plane launches' copies
+    const std::vector<double> em = plane_factors(e, c);
+    if (prec == PSX_FP64) {
+      PSX_HIP(hipMemcpy(s.epl[c], em.data(), em.size() * 8, hipMemcpyHostToDevice));
+    } else {
+      std::vector<float> f = cast_vec<float>(em);
+      PSX_HIP(hipMemcpy(s.epl[c], f.data(), f.size() * 4, hipMemcpyHostToDevice));
+    }
+  }
   return PSX_OK;
 }
 
@@ -2654,9 +2812,10 @@ extern "C" psx_status psx_scft_time_step(psx_scft* e, int reps, double* ms_step,
   if (ms_step) *ms_step = ms * per;
   for (int i = 0; i < 3; ++i) {
     double v = 0.0;
-    if (i < na) {
+    // plane mode: pass 0 times the plane launches (z and y), pass 1 is 0
+    if (i < na && !(e->plane && i == 1)) {
       PSX_HIP(hipEventRecord(a, e->stream));
-      PSX_TRY(run(i));
+      PSX_TRY(run(e->plane && i == 0 ? kPlaneAxis : i));
       PSX_HIP(hipEventRecord(b, e->stream));
       PSX_HIP(hipEventSynchronize(b));
       PSX_HIP(hipEventElapsedTime(&ms, a, b));
@@ -2735,34 +2894,86 @@ extern "C" psx_status psx_scft_set_variant(psx_scft* e, int pass, int variant) {
   return PSX_OK;
 }
 
+// Switch between line launches and plane launches: the launch lists and
+// field tables are rebuilt (sweep buffers reallocated, captured graphs
+// dropped); fields are kept, the propagators must be recomputed.
+static psx_status apply_plane(psx_scft* e, bool on) {
+  if (on == e->plane) return PSX_OK;
+  if (on && !plane_available(e))
+    return fail(PSX_ERR_INVALID, "plane launches need a 3D grid with ny == nz in {32, 64, 96, 128}");
+  PSX_HIP(hipStreamSynchronize(e->stream));
+  drop_graphs(e);
+  for (auto& sw : e->sw) free_sweep(sw);
+  e->plane = on;
+  plan_launches(e);
+  if (e->ckpt_K > 0) ckpt_plan(e);
+  plan_quad_overlap(e);
+  PSX_TRY(ensure_sweep(e, e->prec));
+  e->swept = false;
+  e->dens_valid = false;
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_scft_set_sweep_mode(psx_scft* e, int mode) {
+  if (!e) return fail(PSX_ERR_INVALID, "null engine");
+  if (mode != 0 && mode != 1) return fail(PSX_ERR_INVALID, "sweep mode must be 0 (lines) or 1 (planes)");
+  if (e->in_batch && (mode == 1) != e->plane)
+    return fail(PSX_ERR_STATE, "engine belongs to a batch (use psx_batch_set_sweep_mode)");
+  return apply_plane(e, mode == 1);
+}
+
+extern "C" psx_status psx_scft_sweep_mode(const psx_scft* e, int* mode, int* available) {
+  if (!e) return fail(PSX_ERR_INVALID, "null engine");
+  if (mode) *mode = e->plane ? 1 : 0;
+  if (available) *available = plane_available(e) ? 1 : 0;
+  return PSX_OK;
+}
+
 // Per pass, time whole sweeps (both sweeps on their streams, as an iteration
 // runs them) for each candidate line kernel and keep the fastest; passes are
-// tuned one after another, the others holding their current choice.
+// tuned one after another, the others holding their current choice.  Where
+// plane launches are available, whole sweeps with them and with the tuned
+// line launches are then timed and the faster mode is kept.
 extern "C" psx_status psx_scft_autotune(psx_scft* e, int reps, int* h_choice) {
   if (!e || reps < 1) return fail(PSX_ERR_INVALID, "bad argument");
   if (e->ckpt_K > 0) return fail(PSX_ERR_STATE, "autotune runs on the stored-slice sweep (checkpointing off)");
+  const bool modes = plane_available(e) && !e->in_batch;
+  if (modes) PSX_TRY(apply_plane(e, false));     // line variants are tuned with line launches
   if (!e->sw[e->prec].wfac_valid) PSX_TRY(compute_wfac(e));
   const int na = (int)e->axes.size();
   hipEvent_t a, b;
   PSX_HIP(hipEventCreate(&a));
   PSX_HIP(hipEventCreate(&b));
   psx_status st = PSX_OK;
+  auto time_sweeps = [&](float* ms) -> psx_status {
+    PSX_TRY(sweep(e));                               // warm
+    PSX_HIP(hipEventRecord(a, e->stream));
+    for (int r = 0; r < reps; ++r) PSX_TRY(sweep(e));
+    PSX_HIP(hipEventRecord(b, e->stream));
+    PSX_HIP(hipEventSynchronize(b));
+    PSX_HIP(hipEventElapsedTime(ms, a, b));
+    return PSX_OK;
+  };
   for (int i = 0; i < na && st == PSX_OK; ++i) {
+    if (e->plane && i <= 1) continue;                // inside the plane launches
     int best = e->variant[e->prec][i];
     float best_ms = 1e30f;
     for (int v : variant_candidates(e->axes[i].n)) {
       e->variant[e->prec][i] = v;
-      st = sweep(e);                                 // warm
-      if (st != PSX_OK) break;
-      if (hipEventRecord(a, e->stream) != hipSuccess) { st = fail(PSX_ERR_HIP, "event record"); break; }
-      for (int r = 0; r < reps && st == PSX_OK; ++r) st = sweep(e);
-      if (st != PSX_OK) break;
       float ms = 0.f;
-      if (hipEventRecord(b, e->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
-          hipEventElapsedTime(&ms, a, b) != hipSuccess) { st = fail(PSX_ERR_HIP, "event timing"); break; }
+      st = time_sweeps(&ms);
+      if (st != PSX_OK) break;
       if (ms < best_ms) { best_ms = ms; best = v; }
     }
     e->variant[e->prec][i] = best;
+  }
+  if (st == PSX_OK && modes) {
+    float ms_lines = 0.f, ms_plane = 0.f;
+    st = time_sweeps(&ms_lines);
+    if (st == PSX_OK) st = apply_plane(e, true);
+    if (st == PSX_OK) st = compute_wfac(e);
+    if (st == PSX_OK) st = time_sweeps(&ms_plane);
+    if (st == PSX_OK && ms_lines < ms_plane) st = apply_plane(e, false);
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
@@ -2877,6 +3088,25 @@ static psx_status batch_tables(psx_batch* b, int prec) {
   return PSX_OK;
 }
 
+// Plane launches by default where they measured faster than the line
+// launches: lengths below 96, or few fields (one engine's two sweeps).  At
+// 96^3 with 38 fields (19 instances) the line launches move 5.2 TB/s and a
+// plane launch, three blocks of six waves per CU, 2.0 TB/s.
+static bool plane_default(int n, int nfields) { return n < 96 || nfields <= 8; }
+
+// Line or plane launches for every member of class ci; the class tables
+// are rebuilt for the batch's precision (the other one on its next switch).
+static psx_status batch_class_mode(psx_batch* b, size_t ci, bool on) {
+  BatchClass& c = b->cls[ci];
+  for (auto s : b->ss) if (s) PSX_HIP(hipStreamSynchronize(s));
+  for (int m : c.members) PSX_TRY(apply_plane(b->eng[m], on));
+  dfree((void*&)c.fio[0]);
+  dfree((void*&)c.fio[1]);
+  batch_drop_graphs(b);
+  PSX_TRY(batch_tables(b, b->prec));
+  return PSX_OK;
+}
+
 extern "C" psx_status psx_batch_destroy(psx_batch* b) {
   if (!b) return PSX_OK;
   for (auto s : b->ss) if (s) (void)hipStreamSynchronize(s);
@@ -2922,7 +3152,13 @@ extern "C" psx_status psx_batch_create(int n, const psx_grid* grids, const psx_d
       }
       found->members.push_back(i);
     }
-    st = batch_tables(b, prec);
+    for (auto& c : b->cls) {
+      psx_scft* e0 = b->eng[c.members[0]];
+      const bool on = plane_available(e0) && plane_default(e0->axes[0].n, 2 * (int)c.members.size());
+      for (int m : c.members)
+        if (st == PSX_OK) st = apply_plane(b->eng[m], on);
+    }
+    if (st == PSX_OK) st = batch_tables(b, prec);
   }
   if (st == PSX_OK) {
     b->ss[0] = b->stream;
@@ -2966,13 +3202,35 @@ extern "C" psx_status psx_batch_set_precision(psx_batch* b, psx_prec prec) {
   return PSX_OK;
 }
 
+extern "C" psx_status psx_batch_set_sweep_mode(psx_batch* b, int cls, int mode) {
+  if (!b || cls < 0 || cls >= (int)b->cls.size()) return fail(PSX_ERR_INVALID, "bad batch or class");
+  if (mode != 0 && mode != 1) return fail(PSX_ERR_INVALID, "sweep mode must be 0 (lines) or 1 (planes)");
+  psx_scft* e0 = b->eng[b->cls[cls].members[0]];
+  if ((mode == 1) == e0->plane) return PSX_OK;
+  if (mode == 1 && !plane_available(e0))
+    return fail(PSX_ERR_INVALID, "plane launches need a 3D grid with ny == nz in {32, 64, 96, 128}");
+  PSX_TRY(batch_class_mode(b, (size_t)cls, mode == 1));
+  batch_schedule(b);
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_batch_sweep_mode(const psx_batch* b, int cls, int* mode) {
+  if (!b || cls < 0 || cls >= (int)b->cls.size() || !mode) return fail(PSX_ERR_INVALID, "bad argument");
+  *mode = b->eng[b->cls[cls].members[0]]->plane ? 1 : 0;
+  return PSX_OK;
+}
+
+static psx_status batch_time_class(psx_batch* b, size_t ci, int reps, float* ms);
+
 extern "C" psx_status psx_batch_autotune(psx_batch* b, int reps, int* h_choice) {
   if (!b || reps < 1) return fail(PSX_ERR_INVALID, "bad argument");
-  for (auto* e : b->eng)
-    if (!e->sw[b->prec].wfac_valid) PSX_TRY(compute_wfac(e));
   for (size_t ci = 0; ci < b->cls.size(); ++ci) {
     BatchClass& c = b->cls[ci];
     psx_scft* e0 = b->eng[c.members[0]];
+    const bool modes = plane_available(e0);
+    if (modes && e0->plane) PSX_TRY(batch_class_mode(b, ci, false));
+    for (int m : c.members)
+      if (!b->eng[m]->sw[b->prec].wfac_valid) PSX_TRY(compute_wfac(b->eng[m]));
     const int na = (int)e0->axes.size();
     const int K = (int)c.members.size();
     for (int i = 0; i < na; ++i) {
@@ -2982,7 +3240,20 @@ extern "C" psx_status psx_batch_autotune(psx_batch* b, int reps, int* h_choice) 
       else PSX_TRY(tune_pass<float>(e0->axes[i], b->prec, fio, 2 * K, b->stream, reps, v));
       if (h_choice) h_choice[ci * 3 + i] = *v;
     }
+    if (modes) {
+      // whole class sweeps, line launches against plane launches
+      float ms[2] = {0.f, 0.f};
+      for (int on = 0; on < 2; ++on) {
+        if (on) {
+          PSX_TRY(batch_class_mode(b, ci, true));
+          for (int m : c.members) PSX_TRY(compute_wfac(b->eng[m]));
+        }
+        PSX_TRY(batch_time_class(b, ci, reps, &ms[on]));
+      }
+      if (ms[0] < ms[1]) PSX_TRY(batch_class_mode(b, ci, false));
+    }
   }
+  batch_schedule(b);
   batch_drop_graphs(b);
   for (auto* e : b->eng) e->swept = false;
   return PSX_OK;
@@ -2997,10 +3268,29 @@ static psx_status batch_sweep(psx_batch* b, size_t ci) {
   hipStream_t s = b->ss[b->cls_stream[ci]];
   for (size_t j = 0; j < pi.launch_axis.size(); ++j) {
     const int a = pi.launch_axis[j];
-    PSX_TRY(launch_pair<T>(e0->axes[a], b->prec, c.fio[b->prec] + j * 2 * K, 2 * K, s, c.variant[b->prec][a]));
+    PSX_TRY(launch_sweep_op<T>(e0, a, b->prec, c.fio[b->prec] + j * 2 * K, 2 * K, s, c.variant[b->prec]));
   }
   PSX_HIP(hipEventRecord(b->ev_cls[ci], s));
   return PSX_OK;
+}
+
+// Device time (ms) of reps sweeps of class ci on its stream.
+static psx_status batch_time_class(psx_batch* b, size_t ci, int reps, float* ms) {
+  hipStream_t s = b->ss[b->cls_stream[ci]];
+  hipEvent_t e0, e1;
+  PSX_HIP(hipEventCreate(&e0));
+  PSX_HIP(hipEventCreate(&e1));
+  psx_status st = PSX_OK;
+  for (int r = 0; r <= reps && st == PSX_OK; ++r) {   // r = 0 warms
+    if (r == 1 && hipEventRecord(e0, s) != hipSuccess) st = fail(PSX_ERR_HIP, "event record");
+    if (st == PSX_OK) st = b->prec == PSX_FP64 ? batch_sweep<double>(b, ci) : batch_sweep<float>(b, ci);
+  }
+  if (st == PSX_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                       hipEventElapsedTime(ms, e0, e1) != hipSuccess))
+    st = fail(PSX_ERR_HIP, "event timing");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return st;
 }
 
 // One batch iteration on the batch streams, forked from and joined back to

@@ -377,9 +377,23 @@ class DiblockSCFT:
         for p in (range(na) if passes is None else passes):
             check(self._L.psx_scft_set_variant(self._h, int(p), int(variant)))
 
+    def setSweepMode(self, mode: str) -> None:
+        """"lines": two line launches per contour step; "planes": one launch
+        per step, the z and y round trips of each x-plane staged in LDS
+        (3D grids with ny == nz in {32, 64, 96, 128}; the default there).
+        Fields are kept; propagate again after switching."""
+        check(self._L.psx_scft_set_sweep_mode(self._h, {"lines": 0, "planes": 1}[mode]))
+
+    def sweepMode(self) -> tuple:
+        """(current mode, whether plane launches are available)."""
+        m, a = ctypes.c_int(), ctypes.c_int()
+        check(self._L.psx_scft_sweep_mode(self._h, ctypes.byref(m), ctypes.byref(a)))
+        return ("planes" if m.value else "lines"), bool(a.value)
+
     def autotune(self, reps: int = 10) -> list:
-        """Pick the fastest line kernel per pass on this engine's data;
-        returns the choice per pass (the variants of setVariant)."""
+        """Pick the fastest line kernel per pass on this engine's data, then
+        the faster sweep mode where planes are available; returns the
+        choice per pass (the variants of setVariant)."""
         ch = (ctypes.c_int * 3)()
         check(self._L.psx_scft_autotune(self._h, int(reps), ch))
         return list(ch)
@@ -495,9 +509,19 @@ class BatchSCFT:
     def setPrecision(self, prec) -> None:
         check(self._L.psx_batch_set_precision(self._h, _prec(prec)))
 
+    def setSweepMode(self, cls: int, mode: str) -> None:
+        """Sweep mode ("lines" / "planes") of size class cls."""
+        check(self._L.psx_batch_set_sweep_mode(self._h, int(cls), {"lines": 0, "planes": 1}[mode]))
+
+    def sweepMode(self, cls: int) -> str:
+        m = ctypes.c_int()
+        check(self._L.psx_batch_sweep_mode(self._h, int(cls), ctypes.byref(m)))
+        return "planes" if m.value else "lines"
+
     def autotune(self, reps: int = 10):
         """Per grid size and pass: fastest line kernel (-1 register default,
-        1 Stockham, 2 alternative register split)."""
+        1 Stockham, 2 alternative register split), then per class the
+        faster sweep mode where planes are available (see sweepMode)."""
         ch = (ctypes.c_int * (3 * self.nclasses))()
         check(self._L.psx_batch_autotune(self._h, int(reps), ch))
         return [list(ch[3 * c:3 * c + 3]) for c in range(self.nclasses)]
