@@ -39,16 +39,30 @@ def algorithmic_bytes_iter(N: int, Ns: int, bp: int, m: int) -> int:
     return N * (6 * Ns * bp + (Ns + 1) * bp + 80 + ((6 * m + 4) * 8 if m > 0 else 0))
 
 
-def counter_traffic():
-    """HBM bytes per dual contour step from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (calibrated on a copy of known size), or None."""
+COUNTER_FILES = {"lines": "k_line_reg_128cubed_hbm_counters.md",
+                 "planes": "k_plane_128cubed_hbm_counters.md"}
+
+
+def counter_traffic(mode):
+    """HBM bytes per dual contour step of sweep mode `mode` at 128^3 from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated on a copy
+    of known size): ({"fp64": B, "fp32": B}, file) or (None, file)."""
     import re
-    path = os.path.join(ROOT, "profiles", "k_line_reg_128cubed_hbm_counters.md")
+    name = COUNTER_FILES[mode]
     try:
-        m = re.search(r"traffic_per_dual_step_bytes: fp64=(\d+) fp32=(\d+)", open(path).read())
+        m = re.search(r"traffic_per_dual_step_bytes: fp64=(\d+) fp32=(\d+)",
+                      open(os.path.join(ROOT, "profiles", name)).read())
     except OSError:
-        return None
-    return None if not m else {"fp64": int(m.group(1)), "fp32": int(m.group(2))}
+        return None, name
+    return (None if not m else {"fp64": int(m.group(1)), "fp32": int(m.group(2))}), name
+
+
+ROOFLINE_KERNEL = {
+    "planes": "k_plane + fused x line launch: one dual contour step (both sweeps; one launch per "
+              "sweep step, plane launches of the z and y round trips of two steps alternating with "
+              "fused x launches)",
+    "lines": "k_line_reg / k_line_tile: one dual contour step (both sweeps; two line launches per "
+             "sweep step: strided + fused boundary pair)"}
 
 
 def make_engine(prec, w):
@@ -122,6 +136,11 @@ def batch_leg(n_inst=64, steps=5, warmup=2, seed=4, sequential=True):
         b.instances[i].setFields(*fields.seeded_fields(sp["shape"], sp["cell"], sigma=0.1, seed=404,
                                                        instance=i))
     choice = b.autotune(5)
+    order = []
+    for sp in specs:
+        if sp["shape"][0] not in order:
+            order.append(sp["shape"][0])
+    modes = {n: b.sweepMode(c) for c, n in enumerate(order)}
     b.iterate(warmup, mix)
     t0 = time.perf_counter()
     b.iterate(steps, mix)
@@ -130,7 +149,7 @@ def batch_leg(n_inst=64, steps=5, warmup=2, seed=4, sequential=True):
     sizes = {n: sum(1 for sp in specs if sp["shape"][0] == n) for n in (32, 48, 64, 96)}
     if not sequential:
         return dict(instances=n_inst, sizes=sizes, steps=steps, aggregate_iter_per_s=n_inst * steps / dt,
-                    autotune_choice=choice)
+                    autotune_choice=choice, sweep_modes=modes)
     seq = 0.0
     for i, sp in enumerate(specs):
         e = DiblockSCFT(sp["shape"], sp["cell"], sp["fA"], sp["ds"], sp["chiN"], prec="fp64")
@@ -143,7 +162,7 @@ def batch_leg(n_inst=64, steps=5, warmup=2, seed=4, sequential=True):
     return dict(instances=n_inst, sizes=sizes, steps=steps,
                 aggregate_iter_per_s=n_inst * steps / dt,
                 sequential_fp64_iter_per_s=n_inst * steps / seq,
-                speedup=seq / dt, autotune_choice=choice)
+                speedup=seq / dt, autotune_choice=choice, sweep_modes=modes)
 
 
 def cpu_baseline(w, seconds_target=12.0, threads=1):
@@ -207,7 +226,8 @@ def main():
             step_ms, pass_ms = e.timeStep(2)
             if prec == "fp64":
                 e.slice(False, 1)
-            out[prec] = dict(step_us=step_ms * 1e3, pass_us=[p * 1e3 for p in pass_ms])
+            out[prec] = dict(step_us=step_ms * 1e3, pass_us=[p * 1e3 for p in pass_ms],
+                             mode=e.sweepMode()[0])
             e.close()
         out["calibration_copy_bytes_each_way"] = 8 * N
         print(json.dumps(out))
@@ -219,14 +239,15 @@ def main():
     for prec in ("fp64", "fp32"):
         e = make_engine(prec, w)
         choice = None if args.no_tune else e.autotune(5)
-        print(f"[bench] {prec}: line kernels {choice}, warmup {args.warmup}, timing {args.steps} "
+        mode = e.sweepMode()[0]
+        print(f"[bench] {prec}: {mode}, line kernels {choice}, warmup {args.warmup}, timing {args.steps} "
               "iterations", file=sys.stderr, flush=True)
         dt, h = time_iters(e, mix, args.warmup, args.steps)
         step_ms, pass_ms = e.timeStep(10)
         Ns = e.Ns
         legs[prec] = dict(it_s=args.steps / dt, ms_per_iter=1e3 * dt / args.steps,
                           step_ms=step_ms, pass_ms=pass_ms, F_last=float(h["F"][-1]),
-                          err_last=float(h["err"][-1]), line_kernels=choice)
+                          err_last=float(h["err"][-1]), line_kernels=choice, mode=mode)
         e.close()
         print(f"[bench] {prec}: {legs[prec]['it_s']:.2f} it/s, step {step_ms * 1e3:.1f} us "
               f"(passes {[round(p * 1e3, 1) for p in pass_ms]} us)", file=sys.stderr, flush=True)
@@ -254,9 +275,10 @@ def main():
         print(f"[bench] cpu baseline (oracle, {nthr} threads)", file=sys.stderr, flush=True)
         cpu_mt = cpu_baseline(w, seconds_target=6.0, threads=nthr)
 
-    traffic = counter_traffic()
     m = legs["fp32"]
     d = legs["fp64"]
+    traffic_m, src_m = counter_traffic(m["mode"])
+    traffic_d, src_d = counter_traffic(d["mode"])
     bp_m, bp_d = 4, 8
     ach_m = algorithmic_bytes_step(N, bp_m) / (m["step_ms"] * 1e-3) / 1e9
     ach_d = algorithmic_bytes_step(N, bp_d) / (d["step_ms"] * 1e-3) / 1e9
@@ -282,15 +304,18 @@ def main():
         "algorithmic_hbm_frac": {
             "fp64": algorithmic_bytes_iter(N, 100, 8, args.depth) * d["it_s"] / (HBM_PEAK_GBS * 1e9),
             "mixed": algorithmic_bytes_iter(N, 100, 4, args.depth) * m["it_s"] / (HBM_PEAK_GBS * 1e9)},
-        "roofline": {"kernel": "k_line_reg: one dual contour step (both sweeps; two line launches per sweep step: strided + fused boundary pair)",
+        "roofline": {"kernel": ROOFLINE_KERNEL[m["mode"]],
                      "bound": "hbm", "achieved": ach_m, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach_m / HBM_PEAK_GBS,
-                     "traffic": (traffic or {}).get("fp32") if args.grid == 128 else None,
-                     "traffic_source": "profiles/k_line_reg_128cubed_hbm_counters.md (bytes per dual step)",
+                     "traffic": (traffic_m or {}).get("fp32") if args.grid == 128 else None,
+                     "traffic_source": f"profiles/{src_m} (bytes per dual step)",
+                     "sweep_mode": m["mode"],
                      "step_us": m["step_ms"] * 1e3, "pass_us": [p * 1e3 for p in m["pass_ms"]],
                      "line_kernels": m["line_kernels"],
                      "fp64": {"achieved": ach_d, "frac": ach_d / HBM_PEAK_GBS,
-                              "traffic": (traffic or {}).get("fp64") if args.grid == 128 else None,
+                              "traffic": (traffic_d or {}).get("fp64") if args.grid == 128 else None,
+                              "traffic_source": f"profiles/{src_d} (bytes per dual step)",
+                              "sweep_mode": d["mode"],
                               "step_us": d["step_ms"] * 1e3, "pass_us": [p * 1e3 for p in d["pass_ms"]],
                               "line_kernels": d["line_kernels"]}},
         "accuracy": acc,

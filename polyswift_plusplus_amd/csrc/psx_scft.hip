@@ -363,8 +363,11 @@ psx_status plane_kernel(const Axis& az, int prec, const PairIO* fio, int nfields
 // One plane launch over nfields fields of nplanes planes; az is the z axis
 // (its twiddles serve y too).  prepare: only raise the kernel's LDS limit
 // (once per length and type, outside any stream capture).  Transpose
-// buffers behind the plane (REUSE false) where two blocks per CU still fit,
-// else aliasing it (measured at 128^3 and 96^3 x 38 fields, both orders).
+// buffers behind the plane (REUSE false) where that fits, else aliasing it
+// (measured at 128^3 and 96^3 x 38 fields, both orders).  FP64 at 128 uses
+// the (16, 8) split on 8 waves: the lane pairs on 16 waves are held to 128
+// VGPRs and spilled 136 bytes per lane (57.6 -> 48.1 us per plane launch of
+// both sweeps).
 template <typename T>
 psx_status launch_plane(const Axis& az, int prec, const PairIO* fio, int nfields, int nplanes,
                         hipStream_t s, bool prepare = false) {
@@ -373,7 +376,10 @@ psx_status launch_plane(const Axis& az, int prec, const PairIO* fio, int nfields
     case 32: PSX_TRY((plane_kernel<T, 8, 4, 1, false>(az, prec, fio, nfields, nplanes, s, prepare))); break;
     case 64: PSX_TRY((plane_kernel<T, 8, 8, 4, F32>(az, prec, fio, nfields, nplanes, s, prepare))); break;
     case 96: PSX_TRY((plane_kernel<T, 12, 8, 6, F32>(az, prec, fio, nfields, nplanes, s, prepare))); break;
-    case 128: PSX_TRY((plane_kernel<T, 8, 16, 16, !F32>(az, prec, fio, nfields, nplanes, s, prepare))); break;
+    case 128:
+      if constexpr (F32) PSX_TRY((plane_kernel<T, 8, 16, 16, false>(az, prec, fio, nfields, nplanes, s, prepare)));
+      else PSX_TRY((plane_kernel<T, 16, 8, 8, true>(az, prec, fio, nfields, nplanes, s, prepare)));
+      break;
     default: return fail(PSX_ERR_INVALID, "no plane kernel for n = " + std::to_string(az.n));
   }
   PSX_HIP(hipGetLastError());
