@@ -53,7 +53,9 @@ k_plane(LineGeom g, const PairIO* __restrict__ fio) {
   constexpr int VW = 16 / (int)sizeof(T);
   constexpr int CPR = n / VW;          // 16-byte chunks per plane row
   constexpr int NCH = n * CPR;
+  constexpr int IT = NCH / (64 * NW);  // chunks per thread, all in flight at once
   static_assert(n % VW == 0 && RS % VW == 0, "plane rows must be whole 16-byte chunks");
+  static_assert(IT * 64 * NW == NCH, "the block's threads must cover the plane's chunks evenly");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* P = (T*)smem_raw;
   const int f = blockIdx.x / g.ntiles, x = blockIdx.x - f * g.ntiles;
@@ -72,16 +74,36 @@ k_plane(LineGeom g, const PairIO* __restrict__ fio) {
     em[i] = ((const T*)fd.emul)[i];
     if (fused) em[2 * n + i] = ((const T*)fd.emul2)[i];
   }
+  // chunk i of this thread: row = c / CPR, column (c % CPR) * VW, c = tid + i 64 NW
+  auto chunk = [&](int i, int* row, int* col) {
+    const int c = tid + i * 64 * NW;
+    *row = c / CPR;
+    *col = (c - *row * CPR) * VW;
+  };
   {
     const T* __restrict__ src = (const T*)fd.in;
     const T* __restrict__ pre = (const T*)fd.pre;
-#pragma unroll 4
-    for (int c = tid; c < NCH; c += 64 * NW) {
-      const int row = c / CPR, col = (c - row * CPR) * VW;
-      const long long ga = base + (long long)row * n + col;
-      V v = *(const V*)(src + ga);
-      if (pre) { const V w = *(const V*)(pre + ga); vmul(v, w); }
-      vstore_lds(P + row * RS + col, v);
+    V v[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+      int row, col;
+      chunk(i, &row, &col);
+      v[i] = *(const V*)(src + base + (long long)row * n + col);
+    }
+    if (pre) {
+#pragma unroll
+      for (int i = 0; i < IT; ++i) {
+        int row, col;
+        chunk(i, &row, &col);
+        const V w = *(const V*)(pre + base + (long long)row * n + col);
+        vmul(v[i], w);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+      int row, col;
+      chunk(i, &row, &col);
+      vstore_lds(P + row * RS + col, v[i]);
     }
   }
   __syncthreads();
@@ -125,19 +147,42 @@ k_plane(LineGeom g, const PairIO* __restrict__ fio) {
     T* __restrict__ dst = (T*)(fused ? fd.mid : fd.out);
     const T* __restrict__ post = (const T*)fd.post;
     const T* __restrict__ pre2 = fused ? (const T*)fd.pre2 : nullptr;
-#pragma unroll 4
-    for (int c = tid; c < NCH; c += 64 * NW) {
-      const int row = c / CPR, col = (c - row * CPR) * VW;
-      const long long ga = base + (long long)row * n + col;
+    V w[IT];
+    if (post) {
+#pragma unroll
+      for (int i = 0; i < IT; ++i) {
+        int row, col;
+        chunk(i, &row, &col);
+        w[i] = *(const V*)(post + base + (long long)row * n + col);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+      int row, col;
+      chunk(i, &row, &col);
       V v = vload_lds<T>(P + row * RS + col);
-      V w;
-      if (post) { w = *(const V*)(post + ga); vmul(v, w); }
-      *(V*)(dst + ga) = v;
-      if (fused) {
-        if (pre2) {
-          if (pre2 != post) w = *(const V*)(pre2 + ga);
-          vmul(v, w);
-        }
+      if (post) vmul(v, w[i]);
+      *(V*)(dst + base + (long long)row * n + col) = v;
+      if (fused && pre2 && pre2 == post) {
+        vmul(v, w[i]);
+        vstore_lds(P + row * RS + col, v);
+      } else if (fused) {
+        vstore_lds(P + row * RS + col, v);
+      }
+    }
+    if (fused && pre2 && pre2 != post) {   // a block change: the next step's own w-factor
+#pragma unroll
+      for (int i = 0; i < IT; ++i) {
+        int row, col;
+        chunk(i, &row, &col);
+        w[i] = *(const V*)(pre2 + base + (long long)row * n + col);
+      }
+#pragma unroll
+      for (int i = 0; i < IT; ++i) {
+        int row, col;
+        chunk(i, &row, &col);
+        V v = vload_lds<T>(P + row * RS + col);
+        vmul(v, w[i]);
         vstore_lds(P + row * RS + col, v);
       }
     }
@@ -150,9 +195,10 @@ k_plane(LineGeom g, const PairIO* __restrict__ fio) {
   __syncthreads();
   {
     const T* __restrict__ post2 = (const T*)fd.post2;
-#pragma unroll 4
-    for (int c = tid; c < NCH; c += 64 * NW) {
-      const int row = c / CPR, col = (c - row * CPR) * VW;
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+      int row, col;
+      chunk(i, &row, &col);
       const long long ga = base + (long long)row * n + col;
       V v = vload_lds<T>(P + row * RS + col);
       if (post2) { const V w = *(const V*)(post2 + ga); vmul(v, w); }
