@@ -40,7 +40,7 @@ struct PlaneShape {
   static_assert(2 * NW * CL == n, "the waves must cover the plane's complex lines");
 };
 
-// Round trips along z, then y (second step: y, then z) of one x-plane per
+// Round trips along y, then z (second step: z, then y) of one x-plane per
 // block.  g.ntiles = planes per field (n_x), g.nfields = fields; PairIO as
 // for the line launches with the plane spectral tables above.
 template <typename T, int N1, int N2, int NW, bool REUSE>
@@ -108,25 +108,6 @@ k_plane(LineGeom g, const PairIO* __restrict__ fio) {
   }
   __syncthreads();
 
-  auto zpass = [&](const T* ez) {
-    C v[N1];
-#pragma unroll
-    for (int r = 0; r < N1; ++r) {
-      const int k = t + N2 * r;
-      v[r].x = P[(2 * cl) * RS + k];
-      v[r].y = P[(2 * cl + 1) * RS + k];
-    }
-    if (REUSE) __syncthreads();
-    line_roundtrip<T, N1, N2, true>(v, L, tw, twl, ez, t);
-    if (REUSE) __syncthreads();
-    else lds_sync<true>();
-#pragma unroll
-    for (int r = 0; r < N1; ++r) {
-      const int k = t + N2 * r;
-      P[(2 * cl) * RS + k] = v[r].x;
-      P[(2 * cl + 1) * RS + k] = v[r].y;
-    }
-  };
   auto ypass = [&](const T* ey) {
     C v[N1];
 #pragma unroll
@@ -139,59 +120,66 @@ k_plane(LineGeom g, const PairIO* __restrict__ fio) {
     for (int r = 0; r < N1; ++r) *(C*)(P + (t + N2 * r) * RS + 2 * cl) = v[r];
   };
 
-  zpass(em);
-  __syncthreads();
+  // The step pair runs as y, z | z, y (the launch list's own op order): the
+  // step boundary sits between the two z round trips of a wave's own rows,
+  // so post, the store of mid and pre2 happen in registers, and each wave's
+  // w-factor loads and stores overlap the other waves' arithmetic.
   ypass(em + n);
   __syncthreads();
   {
-    T* __restrict__ dst = (T*)(fused ? fd.mid : fd.out);
+    C v[N1];
+#pragma unroll
+    for (int r = 0; r < N1; ++r) {
+      const int k = t + N2 * r;
+      v[r].x = P[(2 * cl) * RS + k];
+      v[r].y = P[(2 * cl + 1) * RS + k];
+    }
+    if (REUSE) __syncthreads();
+    line_roundtrip<T, N1, N2, true>(v, L, tw, twl, em, t);
+    // register r of this lane: points t + N2 r of rows 2cl (x) and 2cl + 1 (y)
+    const long long ra = base + (long long)(2 * cl) * n + t;
     const T* __restrict__ post = (const T*)fd.post;
-    const T* __restrict__ pre2 = fused ? (const T*)fd.pre2 : nullptr;
-    V w[IT];
+    T* __restrict__ dst = (T*)(fused ? fd.mid : fd.out);
+    T wx[N1], wy[N1];
     if (post) {
 #pragma unroll
-      for (int i = 0; i < IT; ++i) {
-        int row, col;
-        chunk(i, &row, &col);
-        w[i] = *(const V*)(post + base + (long long)row * n + col);
+      for (int r = 0; r < N1; ++r) {
+        wx[r] = post[ra + N2 * r];
+        wy[r] = post[ra + n + N2 * r];
       }
+#pragma unroll
+      for (int r = 0; r < N1; ++r) { v[r].x *= wx[r]; v[r].y *= wy[r]; }
     }
 #pragma unroll
-    for (int i = 0; i < IT; ++i) {
-      int row, col;
-      chunk(i, &row, &col);
-      V v = vload_lds<T>(P + row * RS + col);
-      if (post) vmul(v, w[i]);
-      *(V*)(dst + base + (long long)row * n + col) = v;
-      if (fused && pre2 && pre2 == post) {
-        vmul(v, w[i]);
-        vstore_lds(P + row * RS + col, v);
-      } else if (fused) {
-        vstore_lds(P + row * RS + col, v);
-      }
+    for (int r = 0; r < N1; ++r) {
+      dst[ra + N2 * r] = v[r].x;
+      dst[ra + n + N2 * r] = v[r].y;
     }
-    if (fused && pre2 && pre2 != post) {   // a block change: the next step's own w-factor
+    if (!fused) return;    // uniform over the block
+    const T* __restrict__ pre2 = (const T*)fd.pre2;
+    if (pre2) {
+      if (pre2 != post) {
 #pragma unroll
-      for (int i = 0; i < IT; ++i) {
-        int row, col;
-        chunk(i, &row, &col);
-        w[i] = *(const V*)(pre2 + base + (long long)row * n + col);
+        for (int r = 0; r < N1; ++r) {
+          wx[r] = pre2[ra + N2 * r];
+          wy[r] = pre2[ra + n + N2 * r];
+        }
       }
 #pragma unroll
-      for (int i = 0; i < IT; ++i) {
-        int row, col;
-        chunk(i, &row, &col);
-        V v = vload_lds<T>(P + row * RS + col);
-        vmul(v, w[i]);
-        vstore_lds(P + row * RS + col, v);
-      }
+      for (int r = 0; r < N1; ++r) { v[r].x *= wx[r]; v[r].y *= wy[r]; }
+    }
+    line_roundtrip<T, N1, N2, true>(v, L, tw, twl, em + 2 * n, t);
+    if (REUSE) __syncthreads();
+    else lds_sync<true>();
+#pragma unroll
+    for (int r = 0; r < N1; ++r) {
+      const int k = t + N2 * r;
+      P[(2 * cl) * RS + k] = v[r].x;
+      P[(2 * cl + 1) * RS + k] = v[r].y;
     }
   }
-  if (!fused) return;
   __syncthreads();
   ypass(em + 3 * n);
-  __syncthreads();
-  zpass(em + 2 * n);
   __syncthreads();
   {
     const T* __restrict__ post2 = (const T*)fd.post2;
