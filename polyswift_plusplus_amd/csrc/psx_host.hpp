@@ -171,6 +171,14 @@ class DiblockSolver {
   void specFilter(int fields, double cutoffFactor, double filterStrength, const int* specCells = nullptr) {
     check(psx_scft_spec_filter(e_, fields, cutoffFactor, filterStrength, specCells));
   }
+  // sweep launches: 0 two line launches per contour step, 1 plane launches
+  // (one per step; 3D grids with ny == nz in {32, 64, 96, 128}, the default there)
+  void setSweepMode(int mode) { check(psx_scft_set_sweep_mode(e_, mode)); }
+  int sweepMode() const {
+    int m = 0;
+    check(psx_scft_sweep_mode(e_, &m, nullptr));
+    return m;
+  }
   double steepestDescent(double lambda0, double lambda1) {
     double F;
     check(psx_scft_update_steepest(e_, lambda0, lambda1, &F));
