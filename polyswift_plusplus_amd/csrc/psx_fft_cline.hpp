@@ -77,7 +77,7 @@ __device__ __forceinline__ void fwd_half(C* v, C* L, const C* __restrict__ tw, i
       o.y = pair_swap(y.y);
       const C e0 = h ? o : y;
       const C e1 = h ? y : o;
-      const C w1 = cmul(e1, twiddle<-1>(tw, j * N1));
+      const C w1 = tw_mul<-1, T>(e1, j, N2, tw, j * N1);
       if (h) { y.x = e0.x - w1.x; y.y = e0.y - w1.y; }
       else { y.x = e0.x + w1.x; y.y = e0.y + w1.y; }
     }
@@ -113,7 +113,7 @@ __device__ __forceinline__ void inv_half(C* v, C* L, const C* __restrict__ tw, i
       const C hi = h ? y : o;
       if (h) {
         C d; d.x = lo.x - hi.x; d.y = lo.y - hi.y;
-        y = cmul(d, twiddle<+1>(tw, j * N1));
+        y = tw_mul<+1, T>(d, j, N2, tw, j * N1);
       } else {
         y.x = lo.x + hi.x; y.y = lo.y + hi.y;
       }

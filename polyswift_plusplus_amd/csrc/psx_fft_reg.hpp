@@ -49,6 +49,37 @@ __device__ __forceinline__ C twiddle(const C* __restrict__ tw, int idx) {
   return w;
 }
 
+// u * W_D^e for SIGN = -1 (W_D = exp(-2 pi i / D)), u * conj(W_D^e) for
+// SIGN = +1; tw[idx] is the same twiddle in the size-n table.  e and D are
+// compile-time once the callers' loops are unrolled, so the branches fold:
+// powers of -i take no multiply, odd powers of W_8 a scaled sum and
+// difference (1/sqrt(2)), everything else the table entry.
+template <int SIGN, typename T, typename C>
+__device__ __forceinline__ C tw_mul(const C& u, int e, int D, const C* __restrict__ tw, int idx) {
+  if ((4 * e) % D == 0) {
+    int k = ((4 * e) / D) & 3;          // W_4^k = (-i)^k
+    if (SIGN > 0) k = (4 - k) & 3;
+    C r;
+    if (k == 0) r = u;
+    else if (k == 1) { r.x = u.y; r.y = -u.x; }
+    else if (k == 2) { r.x = -u.x; r.y = -u.y; }
+    else { r.x = -u.y; r.y = u.x; }
+    return r;
+  }
+  if ((8 * e) % D == 0) {
+    int k = ((8 * e) / D) & 7;          // odd: W_8^k = ((+-1) + (+-1) i) / sqrt(2)
+    if (SIGN > 0) k = (8 - k) & 7;
+    const T h = (T)0.70710678118654752440;
+    C r;
+    if (k == 1) { r.x = h * (u.x + u.y); r.y = h * (u.y - u.x); }
+    else if (k == 3) { r.x = h * (u.y - u.x); r.y = -h * (u.x + u.y); }
+    else if (k == 5) { r.x = -h * (u.x + u.y); r.y = h * (u.x - u.y); }
+    else { r.x = h * (u.x - u.y); r.y = h * (u.x + u.y); }
+    return r;
+  }
+  return cmul(u, twiddle<SIGN>(tw, idx));
+}
+
 // In-register natural-order DFT of size N (all indices compile-time).
 // tw is the size-n table exp(-2 pi i j / n); twst = n / N.
 template <int N, int SIGN, typename T, typename C>
@@ -70,8 +101,7 @@ __device__ __forceinline__ void dft_reg(C* v, const C* __restrict__ tw, int twst
       bfly<R, SIGN, T>(u);
 #pragma unroll
       for (int c = 0; c < R; ++c) {
-        if (b * c == 0) U[c * M + b] = u[c];
-        else U[c * M + b] = cmul(u[c], twiddle<SIGN>(tw, b * c * twst));
+        U[c * M + b] = tw_mul<SIGN, T>(u[c], b * c, N, tw, b * c * twst);
       }
     }
 #pragma unroll
@@ -196,7 +226,7 @@ __device__ __forceinline__ void pair_roundtrip(C* v, C* L, const C* __restrict__
     o.y = pair_swap(v[k2].y);
     const C e0 = h ? o : v[k2];
     const C e1 = h ? v[k2] : o;
-    const C w1 = cmul(e1, twiddle<-1>(tw, k2 * N1));
+    const C w1 = tw_mul<-1, T>(e1, k2, 2 * N1, tw, k2 * N1);
     C x;
     if (h) { x.x = e0.x - w1.x; x.y = e0.y - w1.y; }
     else { x.x = e0.x + w1.x; x.y = e0.y + w1.y; }
@@ -215,7 +245,7 @@ __device__ __forceinline__ void pair_roundtrip(C* v, C* L, const C* __restrict__
     C e;
     if (h) {
       C d; d.x = lo.x - hi.x; d.y = lo.y - hi.y;
-      e = cmul(d, twiddle<+1>(tw, k2 * N1));
+      e = tw_mul<+1, T>(d, k2, 2 * N1, tw, k2 * N1);
     } else {
       e.x = lo.x + hi.x; e.y = lo.y + hi.y;
     }
@@ -276,7 +306,7 @@ __device__ __forceinline__ void pairs_roundtrip(C* v, C* L, const C* __restrict_
       o.y = pair_swap(y.y);
       const C e0 = h ? o : y;
       const C e1 = h ? y : o;
-      const C w1 = cmul(e1, twiddle<-1>(tw, j * N1));
+      const C w1 = tw_mul<-1, T>(e1, j, N2, tw, j * N1);
       C x;
       if (h) { x.x = e0.x - w1.x; x.y = e0.y - w1.y; }
       else { x.x = e0.x + w1.x; x.y = e0.y + w1.y; }
@@ -288,7 +318,7 @@ __device__ __forceinline__ void pairs_roundtrip(C* v, C* L, const C* __restrict_
       const C hi = h ? x : o;
       if (h) {
         C d; d.x = lo.x - hi.x; d.y = lo.y - hi.y;
-        y = cmul(d, twiddle<+1>(tw, j * N1));
+        y = tw_mul<+1, T>(d, j, N2, tw, j * N1);
       } else {
         y.x = lo.x + hi.x; y.y = lo.y + hi.y;
       }
