@@ -89,15 +89,24 @@ def time_iters(e, mix, warmup, steps):
 def accuracy_leg(w, mix, tol, max_iters, switch_tol, max_fp32):
     """FP64 run to tol vs the mixed protocol (FP32 sweeps until the residual
     falls below switch_tol, then FP64 iterations to tol with a fresh, deeper
-    Anderson history) from the same seed."""
+    Anderson history) from the same seed.  The FP64 engine also runs the
+    mixed protocol's restart on its own (FP64 to switch_tol, then the fresh
+    deeper history): the like-for-like time to solution."""
     from polyswift_plusplus_amd import Mixing
+    from polyswift_plusplus_amd.relax import tail_mixing
+    tail = tail_mixing(mix)
     e64 = make_engine("fp64", w)
     t0 = time.perf_counter()
     h64 = e64.iterate(max_iters, mix, tol=tol)
     t64 = time.perf_counter() - t0
     e64.close()
-    from polyswift_plusplus_amd.relax import tail_mixing
-    tail = tail_mixing(mix)
+    er = make_engine("fp64", w)
+    t0 = time.perf_counter()
+    hr1 = er.iterate(max_iters, mix, tol=switch_tol)
+    er.resetHistory()
+    hr2 = er.iterate(max_iters, tail, tol=tol)
+    tr = time.perf_counter() - t0
+    er.close()
     em = make_engine("fp32", w)
     t0 = time.perf_counter()
     h32 = em.iterate(max_fp32, mix, tol=switch_tol)
@@ -112,8 +121,11 @@ def accuracy_leg(w, mix, tol, max_iters, switch_tol, max_fp32):
                 fp64_iters=int(h64["iters"]), fp64_err=float(h64["err"][-1]),
                 mixed_fp32_iters=int(h32["iters"]), mixed_fp64_iters=int(hfin["iters"]),
                 mixed_err=float(hfin["err"][-1]), fp64_seconds=t64, mixed_seconds=tm,
+                fp64_restart_iters=[int(hr1["iters"]), int(hr2["iters"])],
+                fp64_restart_seconds=tr, dF_fp64_restart=abs(float(hr2["F"][-1]) - Fm),
+                mixed_vs_fp64_restart_time=tr / tm,
                 anderson_depth=mix.depth, mixed_tail_depth=tail.depth,
-                converged=bool(h64["err"][-1] < tol and hfin["err"][-1] < tol))
+                converged=bool(h64["err"][-1] < tol and hfin["err"][-1] < tol and hr2["err"][-1] < tol))
 
 
 def batch_leg(n_inst=64, steps=5, warmup=2, seed=4, sequential=True):
