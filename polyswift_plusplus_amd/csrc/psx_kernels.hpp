@@ -616,58 +616,81 @@ k_gram_finish(const double* __restrict__ part, int nb, int m1, DevState* st) {
   }
 }
 
-// Anderson: finish the Gram row, then solve U c = V for the mixing
-// coefficients (one block; the solve itself is one thread, m <= 32).
-__global__ void k_anderson_solve(int m, double beta, double lam0, double start,
-                                 DevState* st) {
+// Anderson: solve U c = V for the mixing coefficients from the Gram matrix
+// (Gaussian elimination with partial pivoting, m <= 32).
+__global__ void __launch_bounds__(64)
+k_anderson_solve(int m, double beta, double lam0, double start, DevState* st) {
+  // one wave; the system in LDS.  Lanes build it, swap rows and eliminate
+  // (each row's factor is taken before the row changes); pivot search and
+  // back substitution run on lane 0.  Every entry sees the same operations
+  // in the same order as a serial elimination.
+  __shared__ double U[kMaxDepth * kMaxDepth];
+  __shared__ double V[kMaxDepth];
+  __shared__ double Fr[kMaxDepth];
+  __shared__ int slot[kMaxDepth];
+  __shared__ int s_piv, s_ok;
+  const int lane = threadIdx.x;
   const int m1 = m + 1;
   const long long cur = st->iter % m1;
   long long avail = st->since_reset < m ? st->since_reset : m;
   int mk = (int)avail;
   if (st->err > start) mk = 0;
-  double U[kMaxDepth * kMaxDepth], V[kMaxDepth];
-  int slot[kMaxDepth];
   const double* Gm = st->gram;
   const double gcc = Gm[cur * m1 + cur];
-  for (int i = 0; i < mk; ++i) slot[i] = (int)((cur - 1 - i + m1) % m1);
-  for (int i = 0; i < mk; ++i) {
-    const int si = slot[i];
-    V[i] = gcc - Gm[si * m1 + cur];
-    for (int j = 0; j < mk; ++j) {
-      const int sj = slot[j];
-      U[i * mk + j] = gcc - Gm[cur * m1 + sj] - Gm[si * m1 + cur] + Gm[si * m1 + sj];
-    }
+  for (int i = lane; i < mk; i += 64) slot[i] = (int)((cur - 1 - i + m1) % m1);
+  __syncthreads();
+  for (int idx = lane; idx < mk * mk; idx += 64) {
+    const int i = idx / mk, j = idx - i * mk;
+    const int si = slot[i], sj = slot[j];
+    U[idx] = gcc - Gm[cur * m1 + sj] - Gm[si * m1 + cur] + Gm[si * m1 + sj];
   }
-  bool ok = mk > 0;
-  if (ok) {
+  for (int i = lane; i < mk; i += 64) V[i] = gcc - Gm[slot[i] * m1 + cur];
+  if (lane == 0) s_ok = mk > 0;
+  __syncthreads();
+  if (s_ok) {
     double dmax = 0.0;
     for (int i = 0; i < mk; ++i) dmax = fmax(dmax, fabs(U[i * mk + i]));
     const double thr = 1e-13 * dmax;
-    for (int c = 0; c < mk && ok; ++c) {
-      int piv = c;
-      for (int r = c + 1; r < mk; ++r)
-        if (fabs(U[r * mk + c]) > fabs(U[piv * mk + c])) piv = r;
-      if (!(fabs(U[piv * mk + c]) > thr)) { ok = false; break; }
+    for (int c = 0; c < mk; ++c) {
+      if (lane == 0) {
+        int piv = c;
+        for (int r = c + 1; r < mk; ++r)
+          if (fabs(U[r * mk + c]) > fabs(U[piv * mk + c])) piv = r;
+        s_piv = piv;
+        if (!(fabs(U[piv * mk + c]) > thr)) s_ok = 0;
+      }
+      __syncthreads();
+      if (!s_ok) break;
+      const int piv = s_piv;
       if (piv != c) {
-        for (int k = 0; k < mk; ++k) { double t = U[c * mk + k]; U[c * mk + k] = U[piv * mk + k]; U[piv * mk + k] = t; }
-        double t = V[c]; V[c] = V[piv]; V[piv] = t;
+        for (int k = lane; k < mk; k += 64) {
+          const double t = U[c * mk + k]; U[c * mk + k] = U[piv * mk + k]; U[piv * mk + k] = t;
+        }
+        if (lane == 0) { const double t = V[c]; V[c] = V[piv]; V[piv] = t; }
+        __syncthreads();
       }
-      for (int r = c + 1; r < mk; ++r) {
-        const double f = U[r * mk + c] / U[c * mk + c];
-        for (int k = c; k < mk; ++k) U[r * mk + k] -= f * U[c * mk + k];
-        V[r] -= f * V[c];
+      for (int r = c + 1 + lane; r < mk; r += 64) Fr[r] = U[r * mk + c] / U[c * mk + c];
+      __syncthreads();
+      const int w = mk - c;   // columns c .. mk-1 of rows c+1 .. mk-1
+      for (int idx = lane; idx < (mk - c - 1) * w; idx += 64) {
+        const int r = c + 1 + idx / w, k = c + idx % w;
+        U[r * mk + k] -= Fr[r] * U[c * mk + k];
       }
+      for (int r = c + 1 + lane; r < mk; r += 64) V[r] -= Fr[r] * V[c];
+      __syncthreads();
     }
-    if (ok) {
+    if (s_ok && lane == 0) {
       for (int r = mk - 1; r >= 0; --r) {
-        double s = V[r];
-        for (int k = r + 1; k < mk; ++k) s -= U[r * mk + k] * st->coef[k];
-        st->coef[r] = s / U[r * mk + r];
+        double sacc = V[r];
+        for (int k = r + 1; k < mk; ++k) sacc -= U[r * mk + k] * st->coef[k];
+        st->coef[r] = sacc / U[r * mk + r];
       }
     }
   }
-  if (ok) { st->mk = mk; st->beta_used = beta; }
-  else { st->mk = 0; st->beta_used = lam0; }
+  if (lane == 0) {
+    if (s_ok) { st->mk = mk; st->beta_used = beta; }
+    else { st->mk = 0; st->beta_used = lam0; }
+  }
 }
 
 // Anderson: x_new = x + sum c_i (x_i - x) + beta (f + sum c_i (f_i - f)),

@@ -1758,7 +1758,7 @@ psx_status enqueue_post_sweep(psx_scft* e, const psx_mixing* mix) {
   else launch_gram<float>(e, m1);
   hipLaunchKernelGGL(k_gram_finish, dim3(m1), dim3(kThreads), 0, e->stream, (const double*)e->part, e->nb, m1,
                      e->st);
-  hipLaunchKernelGGL(k_anderson_solve, dim3(1), dim3(1), 0, e->stream, m, mix->beta, mix->lambda0,
+  hipLaunchKernelGGL(k_anderson_solve, dim3(1), dim3(64), 0, e->stream, m, mix->beta, mix->lambda0,
                      mix->anderson_start, e->st);
   QuadArgs a = quad_args(e, QM_SD, mix);
   if (e->prec == PSX_FP64)
