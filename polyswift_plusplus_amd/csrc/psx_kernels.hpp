@@ -568,21 +568,28 @@ __global__ void k_finish(const double* __restrict__ part, int nb, double invN,
 // (MAXM1 >= m1 is a compile-time bound so the accumulators stay in VGPRs).
 // The ring holds the sweep's type R (FP32 sweeps keep an FP32 history);
 // the dots accumulate in FP64.
-template <int MAXM1, typename R>
+// V consecutive elements per load (16-byte vectors when 2N % V == 0).
+template <int MAXM1, typename R, int V>
 __global__ void __launch_bounds__(kThreads)
 k_gram(const R* __restrict__ ringf, long long N, int m1, const DevState* st,
        double* __restrict__ part) {
+  typedef CellVec<R, V> VT;
   const long long cur = st->iter % m1;
-  const R* fc = ringf + cur * 2 * N;
+  const long long NV = 2 * N / V;
+  const VT* fc = (const VT*)(ringf + cur * 2 * N);
   double acc[MAXM1];
 #pragma unroll
   for (int j = 0; j < MAXM1; ++j) acc[j] = 0.0;
-  for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < 2 * N;
-       r += (long long)gridDim.x * blockDim.x) {
-    const double x = (double)fc[r];
+  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < NV;
+       g += (long long)gridDim.x * blockDim.x) {
+    const VT x = fc[g];
 #pragma unroll
     for (int j = 0; j < MAXM1; ++j)
-      if (j < m1) acc[j] += x * (double)ringf[(long long)j * 2 * N + r];
+      if (j < m1) {
+        const VT y = ((const VT*)(ringf + (long long)j * 2 * N))[g];
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[j] += (double)x.v[v] * (double)y.v[v];
+      }
   }
   __shared__ double sh[MAXM1][kThreads / 64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;

@@ -1726,20 +1726,28 @@ psx_status enqueue_iteration(psx_scft* e, const psx_mixing* mix) {
   return st;
 }
 
-// The Anderson ring holds the sweep's type R.
-template <typename R>
-void launch_gram(psx_scft* e, int m1) {
+// The Anderson ring holds the sweep's type R; its slots are read as 16-byte
+// vectors when 2N allows (every slot then starts 16-byte aligned).
+template <typename R, int V>
+void launch_gram_v(psx_scft* e, int m1) {
   const R* rf = (const R*)e->ringf;
   const DevState* st = e->st;
   if (m1 <= 4)
-    hipLaunchKernelGGL((k_gram<4, R>), dim3(e->nb), dim3(kThreads), 0, e->stream, rf, e->N, m1, st, e->part);
+    hipLaunchKernelGGL((k_gram<4, R, V>), dim3(e->nb), dim3(kThreads), 0, e->stream, rf, e->N, m1, st, e->part);
   else if (m1 <= 11)
-    hipLaunchKernelGGL((k_gram<11, R>), dim3(e->nb), dim3(kThreads), 0, e->stream, rf, e->N, m1, st, e->part);
+    hipLaunchKernelGGL((k_gram<11, R, V>), dim3(e->nb), dim3(kThreads), 0, e->stream, rf, e->N, m1, st, e->part);
   else if (m1 <= 21)
-    hipLaunchKernelGGL((k_gram<21, R>), dim3(e->nb), dim3(kThreads), 0, e->stream, rf, e->N, m1, st, e->part);
+    hipLaunchKernelGGL((k_gram<21, R, V>), dim3(e->nb), dim3(kThreads), 0, e->stream, rf, e->N, m1, st, e->part);
   else
-    hipLaunchKernelGGL((k_gram<kMaxDepth + 1, R>), dim3(e->nb), dim3(kThreads), 0, e->stream, rf, e->N, m1, st,
-                       e->part);
+    hipLaunchKernelGGL((k_gram<kMaxDepth + 1, R, V>), dim3(e->nb), dim3(kThreads), 0, e->stream, rf, e->N, m1,
+                       st, e->part);
+}
+
+template <typename R>
+void launch_gram(psx_scft* e, int m1) {
+  constexpr int V = 16 / (int)sizeof(R);
+  if ((2 * e->N) % V == 0) launch_gram_v<R, V>(e, m1);
+  else launch_gram_v<R, 1>(e, m1);
 }
 
 // Q, quadrature and the field update after both sweeps (capturable).
