@@ -437,9 +437,14 @@ psx_status launch_cplx(const Axis& a, int prec, const CplxIO& f, hipStream_t s) 
   return PSX_OK;
 }
 
+// Blocks of the grid-stride cell kernels (and of their partial sums).  At
+// most 1024 blocks of 256 threads: the quadrature kernel holds 5 waves per
+// SIMD, so 1024 blocks are all resident at once, where 2048 ran as one full
+// and one partial round (128^3 FP32 iteration 2.47 -> 2.44 ms; 512 blocks:
+// 2.54 ms).
 int grid_1d(long long N) {
   long long b = (N + kThreads - 1) / kThreads;
-  if (b > 2048) b = 2048;
+  if (b > 1024) b = 1024;
   if (b < 1) b = 1;
   return (int)b;
 }
