@@ -177,6 +177,47 @@ def batch_leg(n_inst=64, steps=5, warmup=2, seed=4, sequential=True):
                 speedup=seq / dt, autotune_choice=choice, sweep_modes=modes)
 
 
+def config5_leg(iters=20, warmup=3):
+    """The differentiation workload at full size: 96^3, box (7.2, 7.6, 8.0) Rg,
+    fA = 0.32, chiN = 22, Ns = 200 (ds = 0.005), FP64, seeded lamellar
+    fields.  Per contour mode (every slice stored; backward slices
+    checkpointed every 20 steps): SCFT iterations/s (Anderson m = 10, one
+    hipGraph each) and the time of one propagate + densities + adjoint box
+    stress, against the slice memory the mode holds."""
+    from polyswift_plusplus_amd import DiblockSCFT, Mixing, fields
+    shape, box = (96, 96, 96), (7.2, 7.6, 8.0)
+    cell = tuple(b / n for b, n in zip(box, shape))
+    mix = Mixing("anderson", lambda0=0.1, depth=10, beta=0.1, anderson_start=1e300)
+    out = {"workload": "AB diblock 96^3, box (7.2, 7.6, 8.0) Rg, fA=0.32, chiN=22, Ns=200 (ds=0.005), FP64"}
+    for K in (0, 20):
+        e = DiblockSCFT(shape, cell, 0.32, 0.005, 22.0, prec="fp64")
+        e.setFields(*fields.seeded_fields(shape, cell, sigma=0.1, seed=5, seed_kind="lamellar", amplitude=1.0))
+        if K:
+            e.setCheckpointing(K)
+        e.iterate(warmup, mix)
+        t0 = time.perf_counter()
+        e.iterate(iters, mix)
+        dt = time.perf_counter() - t0
+        e.solveQ()
+        e.densities()
+        e.boxStress()   # warm
+        t0 = time.perf_counter()
+        e.solveQ()
+        e.densities()
+        g = e.boxStress()
+        ts = time.perf_counter() - t0
+        Ns = e.Ns
+        # stored: both propagators' Ns + 1 slices; checkpointed: the backward
+        # checkpoints, K + 1 recomputed backward slices, two forward chunks of
+        # K + 1 and three scratch slices
+        slices = 2 * (Ns + 1) if not K else (Ns // K + 1 + (1 if Ns % K else 0)) + 3 * (K + 1) + 3
+        out["stored" if not K else f"checkpoint_every_{K}"] = dict(
+            iter_per_s=iters / dt, stress_seconds=ts, dFdL=[float(x) for x in g],
+            slice_gib=slices * 8 * int(np.prod(shape)) / 2 ** 30, sweep_mode=e.sweepMode()[0])
+        e.close()
+    return out
+
+
 def cpu_baseline(w, seconds_target=12.0, threads=1):
     """Faithful oracle (full-grid complex FFT pair, separate element-wise
     passes) on `threads` host threads, timed on a bounded sample of contour
@@ -211,6 +252,9 @@ def main():
                     help="keep the default line kernels (no per-pass autotuning)")
     ap.add_argument("--batch-profile", action="store_true",
                     help="only the batched half of the batch leg (for rocprofv3)")
+    ap.add_argument("--config5", action="store_true",
+                    help="only the differentiation workload (96^3, Ns = 200, FP64): iterations/s and "
+                         "adjoint box stress time, with and without contour checkpointing")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3 counters: the dual contour step of "
                          "both precisions (5 reps each) and one FP64 copy of known size")
@@ -228,6 +272,9 @@ def main():
     N = int(np.prod(w["shape"]))
     if args.batch_profile:
         print(json.dumps(batch_leg(sequential=False)))
+        return
+    if args.config5:
+        print(json.dumps(config5_leg()))
         return
     if args.profile:
         # Counter calibration: one k_to_f64<double> over N cells moves exactly
