@@ -359,7 +359,7 @@ psx_status psx_scft_set_variant(psx_scft* e, int pass, int variant);
  * trips of two consecutive steps run in one launch that stages the plane in
  * LDS, alternating with the fused x launches, so a step moves 4 fields
  * through HBM instead of 6.  Mode 1 needs a 3D grid with ny == nz in
- * {32, 64, 96, 128} (PSX_ERR_INVALID otherwise) and is the default there.
+ * {32, 48, 64, 96, 128} (PSX_ERR_INVALID otherwise) and is the default there.
  * Switching rebuilds the sweep buffers: fields are kept, propagate again.
  * Engines of a batch switch through psx_batch_set_sweep_mode
  * (PSX_ERR_STATE here).  The environment variable PSX_PLANE=0 makes line

@@ -23,7 +23,9 @@ namespace psx {
 // the z factors followed by the y factors (each includes 1/n).
 //
 // NW waves of 64 / N2 complex lines each cover the plane's n / 2 complex
-// lines (2 NW 64 / N2 == n).  REUSE: the waves' transpose buffers alias the
+// lines; where they do not divide evenly (n = 48: 2 waves of 16 for 24
+// lines) the last wave's extra lanes join its transposes but load and store
+// nothing.  REUSE: the waves' transpose buffers alias the
 // plane (block barriers around each pass) so the block needs only the plane's
 // LDS; otherwise they sit behind it and a pass needs no barrier of its own
 // (every wave reads and writes only its own lines of the plane).
@@ -37,7 +39,7 @@ struct PlaneShape {
   static constexpr int BUF = ((REUSE ? (PB > TB ? PB : TB) : PB + TB) + 15) / 16 * 16;
   // + twiddles (n complex) + up to four n-point spectral factor rows
   static constexpr int SMEM = BUF + n * 2 * (int)sizeof(T) + 4 * n * (int)sizeof(T);
-  static_assert(2 * NW * CL == n, "the waves must cover the plane's complex lines");
+  static_assert(2 * NW * CL >= n && 2 * (NW - 1) * CL < n, "the waves must cover the plane's complex lines");
 };
 
 // Round trips along y, then z (second step: z, then y) of one x-plane per
@@ -53,9 +55,10 @@ k_plane(LineGeom g, const PairIO* __restrict__ fio) {
   constexpr int VW = 16 / (int)sizeof(T);
   constexpr int CPR = n / VW;          // 16-byte chunks per plane row
   constexpr int NCH = n * CPR;
-  constexpr int IT = NCH / (64 * NW);  // chunks per thread, all in flight at once
+  constexpr int IT = (NCH + 64 * NW - 1) / (64 * NW);   // chunks per thread, all in flight at once
+  constexpr bool CPART = IT * 64 * NW != NCH;             // the last round of chunks is partial
+  constexpr bool PART = 2 * NW * CL != n;                 // lanes past the plane's lines
   static_assert(n % VW == 0 && RS % VW == 0, "plane rows must be whole 16-byte chunks");
-  static_assert(IT * 64 * NW == NCH, "the block's threads must cover the plane's chunks evenly");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* P = (T*)smem_raw;
   const int f = blockIdx.x / g.ntiles, x = blockIdx.x - f * g.ntiles;
@@ -65,6 +68,7 @@ k_plane(LineGeom g, const PairIO* __restrict__ fio) {
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int p = lane / N2, t = lane % N2;
   const int cl = wave * CL + p;        // this lane's complex line: rows / columns 2cl, 2cl+1
+  const bool lv = !PART || 2 * cl < n;
   C* L = (C*)(smem_raw + (REUSE ? 0 : S::PB)) + wave * (64 * N1) + p * n;
   const C* __restrict__ tw = (const C*)g.tw;
   C* twl = (C*)(smem_raw + S::BUF);
@@ -74,11 +78,13 @@ k_plane(LineGeom g, const PairIO* __restrict__ fio) {
     em[i] = ((const T*)fd.emul)[i];
     if (fused) em[2 * n + i] = ((const T*)fd.emul2)[i];
   }
-  // chunk i of this thread: row = c / CPR, column (c % CPR) * VW, c = tid + i 64 NW
-  auto chunk = [&](int i, int* row, int* col) {
+  // chunk i of this thread: row = c / CPR, column (c % CPR) * VW, c = tid + i 64 NW;
+  // false past the plane's last chunk
+  auto chunk = [&](int i, int* row, int* col) -> bool {
     const int c = tid + i * 64 * NW;
     *row = c / CPR;
     *col = (c - *row * CPR) * VW;
+    return !CPART || c < NCH;
   };
   {
     const T* __restrict__ src = (const T*)fd.in;
@@ -87,14 +93,13 @@ k_plane(LineGeom g, const PairIO* __restrict__ fio) {
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
       int row, col;
-      chunk(i, &row, &col);
-      v[i] = *(const V*)(src + base + (long long)row * n + col);
+      if (chunk(i, &row, &col)) v[i] = *(const V*)(src + base + (long long)row * n + col);
     }
     if (pre) {
 #pragma unroll
       for (int i = 0; i < IT; ++i) {
         int row, col;
-        chunk(i, &row, &col);
+        if (!chunk(i, &row, &col)) continue;
         const V w = *(const V*)(pre + base + (long long)row * n + col);
         vmul(v[i], w);
       }
@@ -102,8 +107,7 @@ k_plane(LineGeom g, const PairIO* __restrict__ fio) {
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
       int row, col;
-      chunk(i, &row, &col);
-      vstore_lds(P + row * RS + col, v[i]);
+      if (chunk(i, &row, &col)) vstore_lds(P + row * RS + col, v[i]);
     }
   }
   __syncthreads();
@@ -111,13 +115,18 @@ k_plane(LineGeom g, const PairIO* __restrict__ fio) {
   auto ypass = [&](const T* ey) {
     C v[N1];
 #pragma unroll
-    for (int r = 0; r < N1; ++r) v[r] = *(const C*)(P + (t + N2 * r) * RS + 2 * cl);
+    for (int r = 0; r < N1; ++r) {
+      if (lv) v[r] = *(const C*)(P + (t + N2 * r) * RS + 2 * cl);
+      else { v[r].x = (T)0; v[r].y = (T)0; }
+    }
     if (REUSE) __syncthreads();
     line_roundtrip<T, N1, N2, true>(v, L, tw, twl, ey, t);
     if (REUSE) __syncthreads();
     else lds_sync<true>();
+    if (lv) {
 #pragma unroll
-    for (int r = 0; r < N1; ++r) *(C*)(P + (t + N2 * r) * RS + 2 * cl) = v[r];
+      for (int r = 0; r < N1; ++r) *(C*)(P + (t + N2 * r) * RS + 2 * cl) = v[r];
+    }
   };
 
   // The step pair runs as y, z | z, y (the launch list's own op order): the
@@ -131,8 +140,8 @@ k_plane(LineGeom g, const PairIO* __restrict__ fio) {
 #pragma unroll
     for (int r = 0; r < N1; ++r) {
       const int k = t + N2 * r;
-      v[r].x = P[(2 * cl) * RS + k];
-      v[r].y = P[(2 * cl + 1) * RS + k];
+      v[r].x = lv ? P[(2 * cl) * RS + k] : (T)0;
+      v[r].y = lv ? P[(2 * cl + 1) * RS + k] : (T)0;
     }
     if (REUSE) __syncthreads();
     line_roundtrip<T, N1, N2, true>(v, L, tw, twl, em, t);
@@ -141,7 +150,7 @@ k_plane(LineGeom g, const PairIO* __restrict__ fio) {
     const T* __restrict__ post = (const T*)fd.post;
     T* __restrict__ dst = (T*)(fused ? fd.mid : fd.out);
     T wx[N1], wy[N1];
-    if (post) {
+    if (post && lv) {
 #pragma unroll
       for (int r = 0; r < N1; ++r) {
         wx[r] = post[ra + N2 * r];
@@ -150,14 +159,16 @@ k_plane(LineGeom g, const PairIO* __restrict__ fio) {
 #pragma unroll
       for (int r = 0; r < N1; ++r) { v[r].x *= wx[r]; v[r].y *= wy[r]; }
     }
+    if (lv) {
 #pragma unroll
-    for (int r = 0; r < N1; ++r) {
-      dst[ra + N2 * r] = v[r].x;
-      dst[ra + n + N2 * r] = v[r].y;
+      for (int r = 0; r < N1; ++r) {
+        dst[ra + N2 * r] = v[r].x;
+        dst[ra + n + N2 * r] = v[r].y;
+      }
     }
     if (!fused) return;    // uniform over the block
     const T* __restrict__ pre2 = (const T*)fd.pre2;
-    if (pre2) {
+    if (pre2 && lv) {
       if (pre2 != post) {
 #pragma unroll
         for (int r = 0; r < N1; ++r) {
@@ -171,11 +182,13 @@ k_plane(LineGeom g, const PairIO* __restrict__ fio) {
     line_roundtrip<T, N1, N2, true>(v, L, tw, twl, em + 2 * n, t);
     if (REUSE) __syncthreads();
     else lds_sync<true>();
+    if (lv) {
 #pragma unroll
-    for (int r = 0; r < N1; ++r) {
-      const int k = t + N2 * r;
-      P[(2 * cl) * RS + k] = v[r].x;
-      P[(2 * cl + 1) * RS + k] = v[r].y;
+      for (int r = 0; r < N1; ++r) {
+        const int k = t + N2 * r;
+        P[(2 * cl) * RS + k] = v[r].x;
+        P[(2 * cl + 1) * RS + k] = v[r].y;
+      }
     }
   }
   __syncthreads();
@@ -186,7 +199,7 @@ k_plane(LineGeom g, const PairIO* __restrict__ fio) {
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
       int row, col;
-      chunk(i, &row, &col);
+      if (!chunk(i, &row, &col)) continue;
       const long long ga = base + (long long)row * n + col;
       V v = vload_lds<T>(P + row * RS + col);
       if (post2) { const V w = *(const V*)(post2 + ga); vmul(v, w); }

@@ -172,7 +172,7 @@ class DiblockSolver {
     check(psx_scft_spec_filter(e_, fields, cutoffFactor, filterStrength, specCells));
   }
   // sweep launches: 0 two line launches per contour step, 1 plane launches
-  // (one per step; 3D grids with ny == nz in {32, 64, 96, 128}, the default there)
+  // (one per step; 3D grids with ny == nz in {32, 48, 64, 96, 128}, the default there)
   void setSweepMode(int mode) { check(psx_scft_set_sweep_mode(e_, mode)); }
   int sweepMode() const {
     int m = 0;

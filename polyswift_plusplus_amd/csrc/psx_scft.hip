@@ -341,7 +341,7 @@ psx_status launch_pair(const Axis& a, int prec, const PairIO* fio,
 
 // Plane launches (k_plane): the z and y round trips of whole x-planes, for
 // 3D grids with n_y = n_z = n of a register split below.
-bool plane_length(int n) { return n == 32 || n == 64 || n == 96 || n == 128; }
+bool plane_length(int n) { return n == 32 || n == 48 || n == 64 || n == 96 || n == 128; }
 
 template <typename T, int N1, int N2, int NW, bool REUSE>
 psx_status plane_kernel(const Axis& az, int prec, const PairIO* fio, int nfields, int nplanes,
@@ -374,6 +374,8 @@ psx_status launch_plane(const Axis& az, int prec, const PairIO* fio, int nfields
   constexpr bool F32 = sizeof(T) == 4;
   switch (az.n) {
     case 32: PSX_TRY((plane_kernel<T, 8, 4, 1, false>(az, prec, fio, nfields, nplanes, s, prepare))); break;
+    // 2 waves of 16 lines for 24: the second wave's last 8 lines are idle
+    case 48: PSX_TRY((plane_kernel<T, 12, 4, 2, false>(az, prec, fio, nfields, nplanes, s, prepare))); break;
     case 64: PSX_TRY((plane_kernel<T, 8, 8, 4, F32>(az, prec, fio, nfields, nplanes, s, prepare))); break;
     case 96: PSX_TRY((plane_kernel<T, 12, 8, 6, F32>(az, prec, fio, nfields, nplanes, s, prepare))); break;
     case 128:
@@ -2919,7 +2921,7 @@ extern "C" psx_status psx_scft_set_variant(psx_scft* e, int pass, int variant) {
 static psx_status apply_plane(psx_scft* e, bool on) {
   if (on == e->plane) return PSX_OK;
   if (on && !plane_available(e))
-    return fail(PSX_ERR_INVALID, "plane launches need a 3D grid with ny == nz in {32, 64, 96, 128}");
+    return fail(PSX_ERR_INVALID, "plane launches need a 3D grid with ny == nz in {32, 48, 64, 96, 128}");
   PSX_HIP(hipStreamSynchronize(e->stream));
   drop_graphs(e);
   for (auto& sw : e->sw) free_sweep(sw);
@@ -3227,7 +3229,7 @@ extern "C" psx_status psx_batch_set_sweep_mode(psx_batch* b, int cls, int mode) 
   psx_scft* e0 = b->eng[b->cls[cls].members[0]];
   if ((mode == 1) == e0->plane) return PSX_OK;
   if (mode == 1 && !plane_available(e0))
-    return fail(PSX_ERR_INVALID, "plane launches need a 3D grid with ny == nz in {32, 64, 96, 128}");
+    return fail(PSX_ERR_INVALID, "plane launches need a 3D grid with ny == nz in {32, 48, 64, 96, 128}");
   PSX_TRY(batch_class_mode(b, (size_t)cls, mode == 1));
   batch_schedule(b);
   return PSX_OK;

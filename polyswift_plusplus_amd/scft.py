@@ -380,7 +380,7 @@ class DiblockSCFT:
     def setSweepMode(self, mode: str) -> None:
         """"lines": two line launches per contour step; "planes": one launch
         per step, the z and y round trips of each x-plane staged in LDS
-        (3D grids with ny == nz in {32, 64, 96, 128}; the default there).
+        (3D grids with ny == nz in {32, 48, 64, 96, 128}; the default there).
         Fields are kept; propagate again after switching."""
         check(self._L.psx_scft_set_sweep_mode(self._h, {"lines": 0, "planes": 1}[mode]))
 

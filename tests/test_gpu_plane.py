@@ -1,7 +1,7 @@
 """Plane launches (k_plane: the z and y round trips of whole x-planes, one
 launch per contour step) against the oracle and against the line launches,
 through the C ABI.  FP64 to 1e-12 in slices and 1e-10 in densities and
-trajectories; FP32 to 2e-5.  Covers every plane length (32, 64, 96, 128),
+trajectories; FP32 to 2e-5.  Covers every plane length (32, 48, 64, 96, 128),
 odd and even contour step counts (the launch list's start and end groups),
 block graphs with two b classes, checkpointing, the quadrature under the
 sweeps, batches in both modes, the autotuner and the error paths."""
@@ -32,10 +32,10 @@ def _anderson(P):
     P.mix, P.depth, P.beta, P.lambda0, P.anderson_start = 1, 4, 0.5, 0.1, 1e300
 
 
-# (shape, fA, ds): Ns = 20, 20, 20, 20, 9 (odd), 10, 11 (odd)
+# (shape, fA, ds): Ns = 20, 20, 20, 20, 9 (odd), 10, 11 (odd), 20, 9 (odd)
 PLANE_CASES = [((20, 32, 32), 0.3, 0.05), ((6, 64, 64), 0.35, 0.05), ((4, 96, 96), 0.4, 0.05),
                ((3, 128, 128), 0.3, 0.05), ((32, 32, 32), 4 / 9, 1 / 9), ((5, 64, 64), 0.5, 0.1),
-               ((7, 96, 96), 5 / 11, 1 / 11)]
+               ((7, 96, 96), 5 / 11, 1 / 11), ((6, 48, 48), 0.3, 0.05), ((5, 48, 48), 4 / 9, 1 / 9)]
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
@@ -112,7 +112,7 @@ def test_switching_modes_keeps_fields_and_agrees():
         e.close()
 
 
-@pytest.mark.parametrize("shape", [(8, 16, 16), (8, 32, 64), (32, 32, 1), (8, 48, 48)])
+@pytest.mark.parametrize("shape", [(8, 16, 16), (8, 32, 64), (32, 32, 1), (8, 40, 40)])
 def test_planes_unavailable(shape):
     e = DiblockSCFT(shape, CELL, 0.3, 0.05, 15.0)
     try:
