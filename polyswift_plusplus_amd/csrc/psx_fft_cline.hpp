@@ -153,7 +153,7 @@ k_cline(LineGeom g, CplxIO f) {
   const int p = lane / N2, t = lane % N2;
   const int l = blockIdx.x * CL + p;
   const bool valid = l < g.nlines;
-  C* L = (C*)smem_raw + p * n;
+  C* L = (C*)smem_raw + p * TBuf<N1, N2>::line;
   const int inner = g.inner;
   const long long lb = valid ? (long long)(l / inner) * n * inner + (l - (l / inner) * inner) : 0;
   C v[N1];

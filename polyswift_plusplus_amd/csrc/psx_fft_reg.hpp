@@ -16,6 +16,18 @@
 
 namespace psx {
 
+// Transpose buffers.  A wave's 64/N2 complex lines sit TBuf::line complex
+// words apart.  For one register index the N2 lanes of a line touch N2
+// consecutive words of one row (XOR-swizzled inside it), so with lines n
+// words apart (a multiple of 32 words) every line of the wave fell on the
+// same LDS banks: 2-4 times the conflict-free access time in FP32.  N2 more
+// words per line shift consecutive lines onto disjoint banks.
+template <int N1, int N2>
+struct TBuf {
+  static constexpr int line = N1 * N2 + N2;       // complex words per line
+  static constexpr int wave = (64 / N2) * line;   // per wave: 64 (N1 + 1)
+};
+
 // 16-byte vectors for staged contiguous rows.
 template <typename T> struct Vec16;
 template <> struct Vec16<double> { typedef double2 type; };
@@ -395,14 +407,15 @@ k_line_reg(LineGeom g, const PairIO* __restrict__ fio) {
     bb = ob * n * inner + (la + 1 - ob * inner);
     stride = inner;
   }
-  C* L = lds + p * n;
+  C* L = lds + p * TBuf<N1, N2>::line;
   // FP32: LDS copies of the per-lane tables after the transpose/staging
   // buffer (shorter latency than L2 in the middle of the round trip).  FP64
   // keeps them in global memory: the extra 4 KB would drop the wave below
   // 8 waves per CU of LDS, and a 128^3 pass needs exactly 8.
   constexpr bool TLDS = sizeof(T) == 4;
-  constexpr int BUF = (64 * N1 * 2 * (int)sizeof(T) > 2 * 64 / N2 * (n + N2 / 2) * (int)sizeof(T))
-                          ? 64 * N1 * 2 * (int)sizeof(T)
+  constexpr int TBB = TBuf<N1, N2>::wave * 2 * (int)sizeof(T);
+  constexpr int BUF = (TBB > 2 * 64 / N2 * (n + N2 / 2) * (int)sizeof(T))
+                          ? TBB
                           : 2 * 64 / N2 * (n + N2 / 2) * (int)sizeof(T);
   const C* twl = tw;
   const T* e1 = (const T*)fd.emul;
@@ -572,7 +585,7 @@ k_line_tile(LineGeom g, const PairIO* __restrict__ fio) {
   constexpr int CPR = LT / VW;                    // 16-byte chunks per row
   constexpr int NCH = n * CPR;
   constexpr int SB = n * LS * (int)sizeof(T);
-  constexpr int TB = W * 64 * N1 * 2 * (int)sizeof(T);
+  constexpr int TB = W * TBuf<N1, N2>::wave * 2 * (int)sizeof(T);
   constexpr int BUF = ((SB > TB ? SB : TB) + 15) / 16 * 16;
   static_assert(LT % VW == 0, "tile rows must be whole 16-byte chunks");
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -587,7 +600,7 @@ k_line_tile(LineGeom g, const PairIO* __restrict__ fio) {
   const int inner = g.inner;
   const int l0 = tile * LT;
   const long long base = (long long)(l0 / inner) * n * inner + (l0 - (l0 / inner) * inner);
-  C* L = (C*)smem_raw + wave * (64 * N1) + p * n;
+  C* L = (C*)smem_raw + wave * TBuf<N1, N2>::wave + p * TBuf<N1, N2>::line;
   const int j0 = wave * RL + 2 * p;               // this lane's real-line pair in the tile
 
   constexpr bool TLDS = sizeof(T) == 4;

@@ -197,8 +197,8 @@ void launch_reg(const Axis& a, int prec, const PairIO* fio, int nfields,
   LineGeom g = geom(a, prec, RL);
   g.ntiles = (a.nlines + RL - 1) / RL;
   g.nfields = nfields;
-  // transpose buffer (64 N1 complex) or the staged rows (RL (n + N2/2) reals)
-  const size_t tbuf = (size_t)64 * N1 * 2 * sizeof(T);
+  // transpose buffer (64 (N1 + 1) complex) or the staged rows (RL (n + N2/2) reals)
+  const size_t tbuf = (size_t)TBuf<N1, N2>::wave * 2 * sizeof(T);
   const size_t sbuf = (size_t)RL * (N1 * N2 + N2 / 2) * sizeof(T);
   // FP32: + LDS copies of the twiddles (n complex) and two spectral multipliers
   const size_t smem = (tbuf > sbuf ? tbuf : sbuf) + (sizeof(T) == 4 ? (size_t)N1 * N2 * 4 * sizeof(T) : 0);
@@ -226,7 +226,7 @@ void launch_tile(const Axis& a, int prec, const PairIO* fio, int nfields, hipStr
   g.ntiles = a.nlines / LT;
   g.nfields = nfields;
   const size_t sb = (size_t)n * (LT + 2) * sizeof(T);
-  const size_t tb = (size_t)W * 64 * N1 * 2 * sizeof(T);
+  const size_t tb = (size_t)W * TBuf<N1, N2>::wave * 2 * sizeof(T);
   const size_t buf = ((sb > tb ? sb : tb) + 15) / 16 * 16;
   const size_t smem = buf + (sizeof(T) == 4 ? (size_t)n * 4 * sizeof(T) : 0);
   hipLaunchKernelGGL((k_line_tile<T, N1, N2, W>), dim3(g.ntiles * nfields), dim3(64 * W), smem, s, g, fio);
@@ -394,7 +394,7 @@ void launch_cline(const Axis& a, int prec, const CplxIO& f, hipStream_t s) {
   constexpr int CL = 64 / N2;
   LineGeom g = geom(a, prec, CL);
   g.ntiles = (a.nlines + CL - 1) / CL;
-  const size_t smem = (size_t)64 * N1 * 2 * sizeof(T);
+  const size_t smem = (size_t)TBuf<N1, N2>::wave * 2 * sizeof(T);
   const dim3 grid(g.ntiles);
   if (f.do_fwd && f.do_inv) hipLaunchKernelGGL((k_cline<T, N1, N2, 2>), grid, dim3(64), smem, s, g, f);
   else if (f.do_fwd) hipLaunchKernelGGL((k_cline<T, N1, N2, 0>), grid, dim3(64), smem, s, g, f);
