@@ -33,12 +33,16 @@ Mapping to the engine (``configure``):
   pseffhamil/PsFloryInteraction.cpp:160-161);
 * the ``steepestDescent`` Updater's ``relaxlambdas`` and ``noise``
   (pseffhamil/PsSteepDUpdater.cpp:56-63); ``nsteps``, ``randomSeed``;
-* ``simpleSpecFilter`` / ``multiSpecFilter`` Updaters after it in the
-  EffHamil's ``updaterSequence`` (pseffhamil/PsEffHamil.cpp:185-203): their
+* ``simpleSpecFilter`` / ``multiSpecFilter`` Updaters in the EffHamil's
+  ``updaterSequence`` (pseffhamil/PsEffHamil.cpp:185-203): their
   ``updatefields``, ``cutoffFactor``, ``filterStrength``, ``specCellSizes``
   and the ``applyStart`` / ``applyFrequency`` / ``applyEnd`` gate
   (psbase/PsUpdater.cpp:60-118), applied to the engine's conjugate fields
-  after the field update of each step they are due at.
+  at each step they are due at.  A filter after the steepest-descent
+  Updater acts on the updated fields; one before it acts between the
+  step's densities (computed first, pstrol/PsDomain.cpp:354-365) and the
+  field update, which then starts from the filtered fields
+  (pseffhamil/PsSteepDUpdater.cpp:170-184).
 
 Anything else that changes the physics (solvents, several polymers,
 charged or polydisperse blocks, walls, chi functions) raises ``DeckError``.
@@ -453,6 +457,8 @@ def configure(root: Node) -> MeltConfig:
     noise = float(sd.get("noise", 0.0))
     seq = ham.get("updaterSequence", [u.name for u in ups])
     seq = seq if isinstance(seq, list) else [seq]
+    if sd.name not in seq:
+        raise DeckError(f"updaterSequence does not name the field update {sd.name!r}")
     by_name = {u.name: u for u in ups}
     filters = []
     for pos, nm in enumerate(seq):
@@ -466,9 +472,6 @@ def configure(root: Node) -> MeltConfig:
             raise DeckError("the Poisson updater needs charged species, outside the melt subset")
         if kind not in ("simpleSpecFilter", "multiSpecFilter"):
             raise DeckError(f"Updater kind {kind!r} unsupported")
-        if pos < seq.index(sd.name):
-            raise DeckError(f"spectral filter {nm!r} before the field update in updaterSequence "
-                            "is unsupported")
         if u.get("fftKind") is None:
             raise DeckError(f"Updater {nm!r}: fftKind not set")
         ufields = u.get("updatefields")
@@ -487,7 +490,8 @@ def configure(root: Node) -> MeltConfig:
         sched = UpdaterSchedule(int(u.get("applyStart", 0)), int(u.get("applyFrequency", 1)),
                                 None if u.get("applyEnd") is None else int(u.get("applyEnd")))
         filters.append(SpecFilterUpdater(which, float(u.get("cutoffFactor", 0.0)),
-                                         float(u.get("filterStrength", 0.0)), cells, sched, nm))
+                                         float(u.get("filterStrength", 0.0)), cells, sched, nm,
+                                         before=pos < seq.index(sd.name)))
 
     bnodes = poly.all("Block")
     if not bnodes:
@@ -614,27 +618,51 @@ def run(deck, iters: int | None = None, prec="fp64", sigma: float = 0.1, fields_
     period = cfg.dump_period if prefix else None
     filters = [dataclasses.replace(f, schedule=f.schedule.resolve(cfg.nsteps, step0))
                for f in cfg.filters]
+    pre = [f for f in filters if f.before]
+    post = [f for f in filters if not f.before]
     hist = {"F": [], "Q": [], "err": []}
     t, done = step0, 0
     while done < n:
-        chunk = n - done
-        if period:
-            chunk = min(chunk, period - t % period)
-        for f in filters:
-            nd = f.schedule.next_due(t)
-            if nd is not None:
-                chunk = min(chunk, nd - t)
-        h = e.iterate(chunk, mix, tol=tol, use_graph=use_graph)
-        for k in hist:
-            hist[k].extend(h[k])
-        t += h["iters"]
-        done += h["iters"]
-        if h["iters"] == chunk:
-            for f in filters:          # after the field update, in updaterSequence order
+        if any(f.schedule.due(t + 1) for f in pre):
+            # A step with filters before its field update runs split: the
+            # densities of the unfiltered fields, the filters, then the
+            # steepest-descent update from the filtered fields.  Its residual
+            # is not read back (NaN in the history).
+            e.update()
+            Q = e.calcBigQ()
+            for f in pre:
+                f.update(e, t + 1)
+            F = e.steepestDescent(*cfg.lambdas)
+            hist["F"].append(F)
+            hist["Q"].append(Q)
+            hist["err"].append(float("nan"))
+            t += 1
+            done += 1
+            chunk = h_iters = 1
+        else:
+            chunk = n - done
+            if period:
+                chunk = min(chunk, period - t % period)
+            for f in post:
+                nd = f.schedule.next_due(t)
+                if nd is not None:
+                    chunk = min(chunk, nd - t)
+            for f in pre:              # stop before the step it is due at
+                nd = f.schedule.next_due(t)
+                if nd is not None:
+                    chunk = min(chunk, nd - 1 - t)
+            h = e.iterate(chunk, mix, tol=tol, use_graph=use_graph)
+            for k in hist:
+                hist[k].extend(h[k])
+            h_iters = h["iters"]
+            t += h_iters
+            done += h_iters
+        if h_iters == chunk:
+            for f in post:             # after the field update, in updaterSequence order
                 f.update(e, t)
         if period and t % period == 0:
             physfield.dump(e, prefix, t, names)
-        if h["iters"] < chunk:     # converged
+        if h_iters < chunk:        # converged
             break
     out = {k: np.asarray(v) for k, v in hist.items()}
     out.update(iters=done, step=t, config=cfg, engine=e, rng=rng,

@@ -70,6 +70,9 @@ class SpecFilterUpdater:
     specCellSizes: tuple | None = None
     schedule: UpdaterSchedule = field(default_factory=UpdaterSchedule)
     name: str = "specFilter"
+    # placed before the field update in updaterSequence: it then filters the
+    # fields between the step's densities and its field update
+    before: bool = False
 
     def apply(self, engine) -> None:
         if self.cutoffFactor <= 0.0:      # the reference returns before anything

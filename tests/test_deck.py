@@ -151,3 +151,24 @@ def test_reference_spectral_filter_decks():
     path = os.path.join(REF, "psexamples", "PSimBase", "spectralFilter", "spectralFilter.pre")
     ex = deck.configure(deck.load(path))
     assert len(ex.filters) == 1 and ex.filters[0].specCellSizes is not None
+
+
+def test_filter_before_the_field_update():
+    """A filter listed before the steepest-descent Updater in updaterSequence
+    runs between the step's densities and its update (flagged ``before``);
+    after it, on the updated fields."""
+    macros = [os.path.join(DECKS, "macros")]
+    after = deck.configure(deck.load(os.path.join(DECKS, "melt_filter.pre"), search_path=macros))
+    before = deck.configure(deck.load(os.path.join(DECKS, "melt_filter_before.pre"), search_path=macros))
+    (fa,), (fb,) = after.filters, before.filters
+    assert not fa.before and fb.before
+    s = fb.schedule.resolve(before.nsteps)
+    assert [t for t in range(1, before.nsteps + 1) if s.due(t)] == [3, 7, 11]
+
+
+def test_update_sequence_without_the_field_update():
+    text = open(os.path.join(DECKS, "melt_filter.pre")).read().replace(
+        "updaterSequence = [wAwB filt]", "updaterSequence = [filt]")
+    root = deck.parse(text, search_path=[os.path.join(DECKS, "macros")], here=DECKS)
+    with pytest.raises(DeckError, match="does not name the field update"):
+        deck.configure(root)

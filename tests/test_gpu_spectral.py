@@ -165,3 +165,43 @@ def test_filter_deck_matches_oracle():
     gA, gB, _ = h["engine"].getFields()
     assert rel(gA, wA) < 1e-10 and rel(gB, wB) < 1e-10
     h["engine"].close()
+
+
+def test_filter_before_update_deck_matches_oracle():
+    """melt_filter_before.pre: the multiSpecFilter listed before the
+    steepest-descent Updater, due at steps 3, 7, 11.  Such a step takes the
+    densities of the unfiltered fields, filters the fields, then updates from
+    the filtered fields (pstrol/PsDomain.cpp:354-365,
+    pseffhamil/PsSteepDUpdater.cpp:170-184); the oracle runs that order."""
+    root = deck.load(os.path.join(DECKS, "melt_filter_before.pre"),
+                     search_path=[os.path.join(DECKS, "macros")])
+    cfg = deck.configure(root)
+    print("filter-before deck", cfg.shape, flush=True)
+    h = deck.run(root, init="gaussian")
+    (f,) = cfg.filters
+    due = f.schedule.resolve(cfg.nsteps)
+    P = O.Params(*cfg.shape, *cfg.cell, cfg.ds, 0.3, cfg.chiN, lambda0=cfg.lambdas[0],
+                 lambda1=cfg.lambdas[1])
+    wA, wB = fields.seeded_fields(cfg.shape, cfg.cell, sigma=0.1, seed=cfg.seed)
+    p = np.zeros(cfg.shape)
+    F, Q = [], []
+    for t in range(1, cfg.nsteps + 1):
+        if due.due(t):
+            phiA, phiB, q = O.diblock_densities(P, wA, wB)
+            wA = O.spec_filter(wA, f.cutoffFactor, f.filterStrength, f.specCellSizes)
+            wB = O.spec_filter(wB, f.cutoffFactor, f.filterStrength, f.specCellSizes)
+            F.append(O.sd_update(P, wA, wB, p, phiA, phiB, q))
+            Q.append(q)
+        else:
+            r = O.scft_run(P, wA, wB, p, n_iters=1)
+            wA, wB, p = r["wA"], r["wB"], r["p"]
+            F.extend(r["F"])
+            Q.extend(r["Q"])
+    assert len(h["F"]) == cfg.nsteps
+    assert rel(h["F"], F) < 1e-10 and rel(h["Q"], Q) < 1e-10
+    split = [t - 1 for t in range(1, cfg.nsteps + 1) if due.due(t)]
+    assert split == [2, 6, 10] and np.isnan(h["err"][split]).all()
+    assert np.isfinite(np.delete(h["err"], split)).all()
+    gA, gB, _ = h["engine"].getFields()
+    assert rel(gA, wA) < 1e-10 and rel(gB, wB) < 1e-10
+    h["engine"].close()
