@@ -262,7 +262,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(json.dumps({"error": "this engine runs one grid on one GPU; "
-                          "multi-GPU runs are not part of this build (use --gpus 1)"}))
+                          "there is no multi-GPU path (use --gpus 1)"}))
         sys.exit(2)
 
     from polyswift_plusplus_amd import Mixing
