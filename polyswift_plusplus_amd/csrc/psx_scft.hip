@@ -360,11 +360,45 @@ psx_status plane_kernel(const Axis& az, int prec, const PairIO* fio, int nfields
   return PSX_OK;
 }
 
+// CUs of the current device (queried once).
+static int g_device_cus = 0;
+static int device_cus() {
+  if (g_device_cus == 0) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+      g_device_cus = v;
+    else
+      g_device_cus = 256;
+  }
+  return g_device_cus;
+}
+
+// FP32 planes at n = 64 and 96 fit either buffer layout.  Behind the plane
+// the transpose buffers need no block barriers but twice the LDS; aliasing
+// the plane (REUSE) fits twice the blocks per CU.  A launch with no more
+// blocks than the CUs hold in the barrier-free layout takes it (single 96^3
+// FP32 engine: 18.4 -> 17.5 us per dual step); a larger one aliases (the
+// 96^3 x 19-instance batch class: 20.0 -> 19.5 ms per iteration).
+template <typename T, int N1, int N2, int NW>
+psx_status plane_kernel_fit(const Axis& az, int prec, const PairIO* fio, int nfields, int nplanes,
+                            hipStream_t s, bool prepare) {
+  if (prepare) {
+    PSX_TRY((plane_kernel<T, N1, N2, NW, false>(az, prec, fio, nfields, nplanes, s, true)));
+    return plane_kernel<T, N1, N2, NW, true>(az, prec, fio, nfields, nplanes, s, true);
+  }
+  constexpr int per_cu = 160 * 1024 / PlaneShape<T, N1, N2, NW, false>::SMEM;
+  if ((long long)nfields * nplanes <= (long long)per_cu * device_cus())
+    return plane_kernel<T, N1, N2, NW, false>(az, prec, fio, nfields, nplanes, s, false);
+  return plane_kernel<T, N1, N2, NW, true>(az, prec, fio, nfields, nplanes, s, false);
+}
+
 // One plane launch over nfields fields of nplanes planes; az is the z axis
 // (its twiddles serve y too).  prepare: only raise the kernel's LDS limit
-// (once per length and type, outside any stream capture).  Transpose
-// buffers behind the plane (REUSE false) where that fits, else aliasing it
-// (measured at 128^3 and 96^3 x 38 fields, both orders).  FP64 at 128 uses
+// (once per length and type, outside any stream capture).  FP32 at n = 64
+// and 96 picks the buffer layout by launch size (plane_kernel_fit); the
+// other kernels keep the transpose buffers behind the plane where both fit
+// in LDS, else alias it.  FP64 at 128 uses
 // the (16, 8) split on 8 waves: the lane pairs on 16 waves are held to 128
 // VGPRs and spilled 136 bytes per lane (57.6 -> 48.1 us per plane launch of
 // both sweeps).
@@ -376,8 +410,14 @@ psx_status launch_plane(const Axis& az, int prec, const PairIO* fio, int nfields
     case 32: PSX_TRY((plane_kernel<T, 8, 4, 1, false>(az, prec, fio, nfields, nplanes, s, prepare))); break;
     // 2 waves of 16 lines for 24: the second wave's last 8 lines are idle
     case 48: PSX_TRY((plane_kernel<T, 12, 4, 2, false>(az, prec, fio, nfields, nplanes, s, prepare))); break;
-    case 64: PSX_TRY((plane_kernel<T, 8, 8, 4, F32>(az, prec, fio, nfields, nplanes, s, prepare))); break;
-    case 96: PSX_TRY((plane_kernel<T, 12, 8, 6, F32>(az, prec, fio, nfields, nplanes, s, prepare))); break;
+    case 64:
+      if constexpr (F32) PSX_TRY((plane_kernel_fit<T, 8, 8, 4>(az, prec, fio, nfields, nplanes, s, prepare)));
+      else PSX_TRY((plane_kernel<T, 8, 8, 4, false>(az, prec, fio, nfields, nplanes, s, prepare)));
+      break;
+    case 96:
+      if constexpr (F32) PSX_TRY((plane_kernel_fit<T, 12, 8, 6>(az, prec, fio, nfields, nplanes, s, prepare)));
+      else PSX_TRY((plane_kernel<T, 12, 8, 6, false>(az, prec, fio, nfields, nplanes, s, prepare)));
+      break;
     case 128:
       if constexpr (F32) PSX_TRY((plane_kernel<T, 8, 16, 16, false>(az, prec, fio, nfields, nplanes, s, prepare)));
       else PSX_TRY((plane_kernel<T, 16, 8, 8, true>(az, prec, fio, nfields, nplanes, s, prepare)));

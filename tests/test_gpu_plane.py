@@ -289,3 +289,36 @@ def test_autotune_chooses_a_mode_and_keeps_results():
                 e.close()
     finally:
         b.close()
+
+
+@pytest.mark.parametrize("n,nx,count", [(96, 48, 6), (64, 60, 9)])
+def test_fp32_plane_layouts_agree(n, nx, count, monkeypatch):
+    """FP32 plane launches at n = 64 and 96 alias the transpose buffers with
+    the staged plane only when the launch has more blocks than the CUs hold
+    in the barrier-free layout.  A batch class of `count` (nx, n, n)
+    instances (2 * count * nx plane blocks per launch) takes the aliased
+    layout, each single engine the barrier-free one: every instance must
+    equal its single engine bit for bit."""
+    print(f"fp32 plane layouts n={n} nx={nx} count={count}", flush=True)
+    monkeypatch.setenv("PSX_OVERLAP_QUAD", "0")
+    specs = [dict(shape=(nx, n, n), cell=(0.3, 0.08, 0.08), fA=0.3 + 0.05 * (i % 3), ds=0.05,
+                  chiN=15.0 + i) for i in range(count)]
+    b = BatchSCFT(specs, prec="fp32")
+    try:
+        b.setSweepMode(0, "planes")
+        for i, sp in enumerate(specs):
+            b.instances[i].setFields(*_bfields(i, sp))
+        h = b.iterate(2, BMIX)
+        for i, sp in enumerate(specs):
+            e = DiblockSCFT(sp["shape"], sp["cell"], sp["fA"], sp["ds"], sp["chiN"], prec="fp32")
+            try:
+                e.setFields(*_bfields(i, sp))
+                e.setSweepMode("planes")
+                r = e.iterate(2, BMIX)
+                assert np.array_equal(h["F"][i], r["F"]), i
+                for got, want in zip(b.instances[i].getFields(), e.getFields()):
+                    assert np.array_equal(got, want), i
+            finally:
+                e.close()
+    finally:
+        b.close()
