@@ -537,6 +537,95 @@ k_quad_update(QuadArgs a) {
   }
 }
 
+// k_quad_update for small grids: each cell's contour sum split over the
+// block's S waves (wave w takes slices t = w, w + S, ...), combined through
+// LDS in wave order; wave 0 then finishes its 64 cell vectors.  A block
+// covers 64 vectors per round, so where the one-thread-per-vector kernel
+// leaves most of its threads idle (FP32 vectors of 4 cells at 32^3-64^3),
+// S times the loads are in flight.  Contour modes only (not QM_SD_PHI /
+// QM_FE); the same launch count of blocks, so k_finish reduces as many
+// partials.
+template <typename T, int V, int S>
+__global__ void __launch_bounds__(64 * S)
+k_quad_split(QuadArgs a) {
+  typedef CellVec<T, V> VT;
+  static_assert(64 * S == kThreads, "block_sum and the partials assume kThreads threads");
+  const long long N = a.N, NV = N / V;
+  const double* __restrict__ cw = a.cw;
+  const double invQ = 1.0 / a.st->Q;
+  const double pbar = a.st->pbar;
+  const long long slot = (a.mode == QM_RESID) ? (a.st->iter % a.m1) : 0;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __shared__ double red[2][S][64 * V];
+  double acc[kNPart] = {0.0, 0.0, 0.0, 0.0};
+  for (long long g0 = (long long)blockIdx.x * 64; g0 < NV; g0 += (long long)gridDim.x * 64) {
+    const long long g = g0 + lane;
+    const bool live = g < NV;
+    double qa[V], qb[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) { qa[v] = 0.0; qb[v] = 0.0; }
+    if (live) {
+      for (int b = 0; b < a.nblk; ++b) {
+        const VT* __restrict__ H = (const VT*)a.blk[b].H + g;
+        const VT* __restrict__ Tq = (const VT*)a.blk[b].T + g;
+        const double* __restrict__ c = cw + a.blk[b].woff;
+        const int M = a.blk[b].M;
+        double sm[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) sm[v] = 0.0;
+#pragma unroll 4
+        for (int t = w; t < M; t += S) {
+          const VT h = H[(long long)t * NV];
+          const VT q = Tq[(long long)(M - 1 - t) * NV];
+          const double ct = c[t];
+#pragma unroll
+          for (int v = 0; v < V; ++v) sm[v] += ct * ((double)h.v[v] * (double)q.v[v]);
+        }
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          if (a.blk[b].species == 0) qa[v] += sm[v];
+          else qb[v] += sm[v];
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      red[0][w][lane * V + v] = qa[v];
+      red[1][w][lane * V + v] = qb[v];
+    }
+    __syncthreads();
+    if (w == 0 && live) {
+      const long long r0 = g * V;
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        double sa = a.acc_in ? a.phiA[r0 + v] : 0.0;
+        double sb = a.acc_in ? a.phiB[r0 + v] : 0.0;
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+          sa += red[0][k][lane * V + v];
+          sb += red[1][k][lane * V + v];
+        }
+        if (a.mode == QM_ACC) {
+          a.phiA[r0 + v] = sa;
+          a.phiB[r0 + v] = sb;
+          continue;
+        }
+        const double phA = sa * invQ * a.dens, phB = sb * invQ * a.dens;
+        a.phiA[r0 + v] = phA;
+        a.phiB[r0 + v] = phB;
+        if (a.mode != QM_DENS) quad_tail<T>(a, r0 + v, phA, phB, acc, slot, pbar);
+      }
+    }
+    __syncthreads();   // red is rewritten by the next round
+  }
+  if (a.mode == QM_ACC) return;
+  block_sum<kNPart>(acc);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < kNPart; ++i) a.part[blockIdx.x * kNPart + i] = acc[i];
+  }
+}
+
 // Reduce the partials of k_quad_update / k_mix and close the iteration.
 enum FinishMode { FIN_ITER_SD = 0, FIN_ERR = 1, FIN_FE = 2, FIN_ITER_MIX = 3 };
 __global__ void k_finish(const double* __restrict__ part, int nb, double invN,

@@ -1633,11 +1633,17 @@ psx_status stage_noise(psx_scft* e) {
 // allows (every slice then starts 16-byte aligned; measured 516 -> 437 us at
 // 128^3), FP64 cell by cell (double2 measured slower, 780 -> 815 us).
 // Always e->nb blocks: k_finish reduces that many partials.
+// Contour modes over FP32 vectors on small grids (all vectors in one round
+// of 64 per block) split each cell's sum over the block's four waves
+// (k_quad_split).
 template <typename T>
 psx_status launch_quad_kernel(psx_scft* e, const QuadArgs& a, hipStream_t stream = nullptr) {
   constexpr int V = sizeof(T) == 4 ? 4 : 1;
   if (!stream) stream = e->stream;
-  if (V > 1 && e->N % V == 0) {
+  const bool contour = a.mode != QM_SD_PHI && a.mode != QM_FE;
+  if (V > 1 && e->N % V == 0 && contour && e->N / V <= (long long)e->nb * 64) {
+    hipLaunchKernelGGL((k_quad_split<T, V, kThreads / 64>), dim3(e->nb), dim3(kThreads), 0, stream, a);
+  } else if (V > 1 && e->N % V == 0) {
     hipLaunchKernelGGL((k_quad_update<T, V>), dim3(e->nb), dim3(kThreads), 0, stream, a);
   } else {
     hipLaunchKernelGGL((k_quad_update<T, 1>), dim3(e->nb), dim3(kThreads), 0, stream, a);
