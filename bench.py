@@ -86,19 +86,42 @@ def time_iters(e, mix, warmup, steps):
     return dt, h
 
 
+def solution_state(e):
+    """(phiA, phiB, Q, F) of one evaluation at the engine's fields, in its
+    current precision."""
+    e.solveQ()
+    Q = e.calcBigQ()
+    phiA, phiB = e.densities()
+    return phiA, phiB, Q, e.getFreeE()
+
+
+def state_error(a, b):
+    """|dF|, max|dphi| and |dQ|/Q of state a against state b."""
+    return dict(dF=abs(a[3] - b[3]),
+                max_dphi=float(max(np.max(np.abs(a[0] - b[0])), np.max(np.abs(a[1] - b[1])))),
+                dQ_rel=abs(a[2] / b[2] - 1.0))
+
+
 def accuracy_leg(w, mix, tol, max_iters, switch_tol, max_fp32):
     """FP64 run to tol vs the mixed protocol (FP32 sweeps until the residual
     falls below switch_tol, then FP64 iterations to tol with a fresh, deeper
     Anderson history) from the same seed.  The FP64 engine also runs the
     mixed protocol's restart on its own (FP64 to switch_tol, then the fresh
-    deeper history): the like-for-like time to solution."""
-    from polyswift_plusplus_amd import Mixing
+    deeper history): the like-for-like time to solution.
+
+    Each solution is evaluated once more in FP64 at its final fields; the
+    mixed result states |dF|, max|dphi| and |dQ|/Q against the FP64 solution
+    (whose evaluation the oracle reproduces to 1e-10 in the tests), and one
+    FP32 evaluation at the FP64 solution gives the FP32 sweep's own error."""
     from polyswift_plusplus_amd.relax import tail_mixing
     tail = tail_mixing(mix)
     e64 = make_engine("fp64", w)
     t0 = time.perf_counter()
     h64 = e64.iterate(max_iters, mix, tol=tol)
     t64 = time.perf_counter() - t0
+    s64 = solution_state(e64)
+    e64.setPrecision("fp32")
+    s32 = solution_state(e64)
     e64.close()
     er = make_engine("fp64", w)
     t0 = time.perf_counter()
@@ -106,6 +129,7 @@ def accuracy_leg(w, mix, tol, max_iters, switch_tol, max_fp32):
     er.resetHistory()
     hr2 = er.iterate(max_iters, tail, tol=tol)
     tr = time.perf_counter() - t0
+    sr = solution_state(er)
     er.close()
     em = make_engine("fp32", w)
     t0 = time.perf_counter()
@@ -114,17 +138,23 @@ def accuracy_leg(w, mix, tol, max_iters, switch_tol, max_fp32):
     em.resetHistory()
     hfin = em.iterate(max_iters, tail, tol=tol)
     tm = time.perf_counter() - t0
+    sm = solution_state(em)
     em.close()
     F64 = float(h64["F"][-1])
     Fm = float(hfin["F"][-1])
-    return dict(F_fp64=F64, F_mixed=Fm, dF=abs(F64 - Fm),
+    err_m = state_error(sm, s64)
+    return dict(F_fp64=F64, F_mixed=Fm, dF=err_m["dF"], max_dphi=err_m["max_dphi"], dQ_rel=err_m["dQ_rel"],
+                error_vs="FP64 solution evaluated at its fields (the oracle reproduces that evaluation to "
+                         "1e-10 at 128^3 in tests/test_gpu_configs.py)",
+                mixed_vs_fp64_restart=state_error(sm, sr),
+                fp32_sweep_at_fp64_solution=state_error(s32, s64),
                 fp64_iters=int(h64["iters"]), fp64_err=float(h64["err"][-1]),
                 mixed_fp32_iters=int(h32["iters"]), mixed_fp64_iters=int(hfin["iters"]),
                 mixed_err=float(hfin["err"][-1]), fp64_seconds=t64, mixed_seconds=tm,
                 fp64_restart_iters=[int(hr1["iters"]), int(hr2["iters"])],
-                fp64_restart_seconds=tr, dF_fp64_restart=abs(float(hr2["F"][-1]) - Fm),
+                fp64_restart_seconds=tr, mixed_vs_fp64_time=t64 / tm,
                 mixed_vs_fp64_restart_time=tr / tm,
-                anderson_depth=mix.depth, mixed_tail_depth=tail.depth,
+                anderson_depth=mix.depth, mixed_tail_depth=tail.depth, switch_tol=switch_tol,
                 converged=bool(h64["err"][-1] < tol and hfin["err"][-1] < tol and hr2["err"][-1] < tol))
 
 
@@ -218,11 +248,55 @@ def config5_leg(iters=20, warmup=3):
     return out
 
 
-def cpu_baseline(w, seconds_target=12.0, threads=1):
-    """Faithful oracle (full-grid complex FFT pair, separate element-wise
-    passes) on `threads` host threads, timed on a bounded sample of contour
-    steps of the same grid, scaled to one iteration (2 Ns contour steps; the
-    O(N) work of an iteration is < 1% of it and is left out)."""
+def host_info():
+    """Host CPU model, cores visible to this process, and the oracle's build."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"cpu_model": model, "cpu_count": os.cpu_count(), "cores_available": avail,
+            "compiler": "g++ -O2 -std=c++17 -fopenmp (oracle/Makefile)"}
+
+
+def cpu_baseline_iterations(w, mix, threads, warmup=1, runs=5):
+    """Full faithful oracle iterations (full-grid complex FFT pair and the
+    separate element-wise passes of the reference, propagation, Q,
+    quadrature, Anderson update and F) on the same grid and seed, each timed
+    on the host's steady clock; the median of `runs` one-iteration runs after
+    `warmup` untimed ones."""
+    import oracle as O
+    from polyswift_plusplus_amd import fields
+    O.set_threads(threads)
+    shape = w["shape"]
+    cell = tuple(w["box"] / n for n in shape)
+    wA, wB = fields.seeded_fields(shape, cell, sigma=w["sigma"], seed=w["seed"],
+                                  seed_kind=w["seed_kind"], amplitude=w["amplitude"])
+    P = O.Params(*shape, *cell, ds=w["ds"], fA=w["fA"], chiN=w["chiN"], mix=1, lambda0=mix.lambda0,
+                 depth=mix.depth, beta=mix.beta, anderson_start=mix.anderson_start)
+    r = O.scft_run_timed(P, wA, wB, n_iters=warmup + runs)
+    O.set_threads(1)
+    t = float(np.median(r["seconds"][warmup:]))
+    return {"value": 1.0 / t, "unit": "SCFT iter/s", "cores": threads, "kind": "port",
+            "sample": f"median of {runs} full faithful FP64 SCFT iterations of the bench workload "
+                      f"({shape[0]}^3, Ns={int(round(1 / w['ds']))}, Anderson m={mix.depth}) after "
+                      f"{warmup} untimed, {threads} OpenMP threads: {t:.2f} s per iteration "
+                      f"(runs {[round(float(x), 2) for x in r['seconds'][warmup:]]})"}
+
+
+def cpu_baseline_steps(w, seconds_target=12.0, threads=1):
+    """Faithful oracle contour steps (full-grid complex FFT pair, separate
+    element-wise passes) on `threads` host threads, timed on a bounded
+    sample of the same grid and scaled to one iteration's 2 Ns steps (a
+    full single-thread iteration takes minutes at 128^3; the O(N) work of an
+    iteration is about 1% of it)."""
     import oracle as O
     O.set_threads(threads)
     shape = w["shape"]
@@ -325,14 +399,17 @@ def main():
         batch = batch_leg()
         print(f"[bench] batch: {batch}", file=sys.stderr, flush=True)
 
-    cpu = cpu_mt = None
+    cpu = cpu_1 = host = None
     if not args.no_cpu:
-        print("[bench] cpu baseline (oracle, 1 thread)", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(w)
-        nthr = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        host = host_info()
+        nthr = int(os.environ.get("OMP_NUM_THREADS", "0")) or (host["cores_available"] or 1)
         nthr = max(1, min(nthr, 16))
-        print(f"[bench] cpu baseline (oracle, {nthr} threads)", file=sys.stderr, flush=True)
-        cpu_mt = cpu_baseline(w, seconds_target=6.0, threads=nthr)
+        print(f"[bench] cpu baseline (oracle, full iterations, {nthr} threads)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline_iterations(w, Mixing("anderson", lambda0=0.1, depth=args.depth, beta=0.1,
+                                                anderson_start=1e300), nthr)
+        print(f"[bench] cpu baseline: {cpu['sample']}", file=sys.stderr, flush=True)
+        print("[bench] cpu baseline (oracle contour steps, 1 thread)", file=sys.stderr, flush=True)
+        cpu_1 = cpu_baseline_steps(w)
 
     m = legs["fp32"]
     d = legs["fp64"]
@@ -380,7 +457,8 @@ def main():
         "accuracy": acc,
         "batch": batch,
         "cpu_baseline": cpu,
-        "cpu_baseline_threads": cpu_mt,
+        "cpu_baseline_1thread": cpu_1,
+        "cpu_host": host,
     }
     print(json.dumps(out))
 

@@ -185,7 +185,8 @@ psx_status psx_scft_get_block_slice_host(psx_scft* e, int block,
                                          int from_tail, int s, double* h_out);
 /* Switch the sweep precision (allocates the other slice set on first use).
  * The Anderson history is stored in the sweep's type, so a switch restarts
- * it (as psx_scft_reset_history). */
+ * it (as psx_scft_reset_history).  Engines of a batch switch through
+ * psx_batch_set_precision (PSX_ERR_STATE here). */
 psx_status psx_scft_set_precision(psx_scft* e, psx_prec prec);
 /* Contour steps of the A blocks and of the B blocks (for a diblock, the
  * block steps of A and B). */
@@ -296,6 +297,12 @@ psx_status psx_scft_box_stress(psx_scft* e, double* h_dFdL);
  * stress is its exact total derivative.  Mode 1 has no counterpart in the
  * reference. */
 psx_status psx_scft_set_quadrature(psx_scft* e, int mode);
+/* Which kernel sums the contour quadrature (no reference counterpart):
+ * 0 (default) lets FP32 sweeps on grids of up to 64^3 cells (4-cell vectors
+ * that fit one round of 64 per block) split each cell's contour sum over a
+ * block's four waves; 1 always sums each cell serially in slice order.  The
+ * two agree to rounding. */
+psx_status psx_scft_set_quad_kernel(psx_scft* e, int kind);
 
 /* Contour checkpointing for long chains (no reference counterpart): with
  * every = K > 0 the backward sweep keeps only its slices 0, K, 2K, ... and
@@ -385,7 +392,9 @@ psx_status psx_batch_destroy(psx_batch* b);
 /* Instance count and number of distinct grid sizes. */
 psx_status psx_batch_size(const psx_batch* b, int* n, int* nclasses);
 /* The engine of instance i (owned by the batch): fields, densities, stress
- * and the other psx_scft_* calls work on it directly. */
+ * and the other psx_scft_* calls work on it directly, except the switches
+ * that change which buffers the class launches write (precision, sweep mode,
+ * checkpointing, noise, spectral filters: PSX_ERR_STATE). */
 psx_status psx_batch_instance(psx_batch* b, int i, psx_scft** out);
 psx_status psx_batch_set_precision(psx_batch* b, psx_prec prec);
 /* Per grid size and pass, time the candidate line kernels on the batch's

@@ -145,6 +145,8 @@ def lib():
         L.orc_free_energy.restype = ctypes.c_double
         L.orc_scft_run.argtypes = ([ctypes.POINTER(OrcParams)] + [_dp] * 3 +
                                    [ctypes.c_int, ctypes.c_double] + [_dp] * 5)
+        L.orc_scft_run_timed.argtypes = ([ctypes.POINTER(OrcParams)] + [_dp] * 3 +
+                                         [ctypes.c_int] + [_dp] * 2)
         L.orc_box_stress.argtypes = [ctypes.POINTER(OrcParams), _dp, _dp, _dp]
         L.orc_time_steps.argtypes = [ctypes.c_int] * 4
         L.orc_time_steps.restype = ctypes.c_double
@@ -329,6 +331,19 @@ def scft_run(P: Params, wA, wB, p=None, n_iters=1, tol=0.0):
         raise RuntimeError("oracle scft_run failed")
     return dict(wA=wA, wB=wB, p=p, F=F[:done], Q=Qh[:done], err=err[:done],
                 phiA=phiA, phiB=phiB, iters=done)
+
+
+def scft_run_timed(P: Params, wA, wB, p=None, n_iters=1):
+    """n_iters full iterations from (wA, wB, p), each timed on the host's
+    steady clock: dict(wA, wB, p, F, seconds) with seconds per iteration."""
+    wA, wB = _f64(wA).copy(), _f64(wB).copy()
+    p = np.zeros_like(wA) if p is None else _f64(p).copy()
+    F, t = np.zeros(n_iters), np.zeros(n_iters)
+    cp = P.c()
+    done = lib().orc_scft_run_timed(ctypes.byref(cp), _p(wA), _p(wB), _p(p), n_iters, _p(F), _p(t))
+    if done != n_iters:
+        raise RuntimeError("oracle scft_run_timed failed")
+    return dict(wA=wA, wB=wB, p=p, F=F, seconds=t)
 
 
 def box_stress(P: Params, wA, wB) -> np.ndarray:

@@ -656,10 +656,10 @@ bool solve_dense(int n, std::vector<double> A, std::vector<double> b,
 
 }  // namespace
 
-extern "C" int orc_scft_run(const orc_params* P, double* wA, double* wB,
-                            double* p, int n_iters, double tol,
-                            double* F_hist, double* Q_hist, double* err_hist,
-                            double* phiA_out, double* phiB_out) {
+static int scft_run_impl(const orc_params* P, double* wA, double* wB,
+                         double* p, int n_iters, double tol,
+                         double* F_hist, double* Q_hist, double* err_hist,
+                         double* phiA_out, double* phiB_out, double* t_hist) {
   const long N = ncells(P);
   std::vector<double> phiA(N), phiB(N);
   double fA, fB;
@@ -669,6 +669,7 @@ extern "C" int orc_scft_run(const orc_params* P, double* wA, double* wB,
   std::vector<std::vector<double>> hx, hf;
   int done = 0;
   for (int it = 0; it < n_iters; ++it) {
+    const auto t_it = std::chrono::steady_clock::now();
     double Q;
     // pstrol/PsDomain.cpp:332-372: densities reset, propagate, Q, densities
     int rc = P->nblocks > 0 ? orc_graph_densities(P, wA, wB, phiA.data(), phiB.data(), &Q, nullptr)
@@ -752,12 +753,28 @@ extern "C" int orc_scft_run(const orc_params* P, double* wA, double* wB,
     if (F_hist) F_hist[it] = F;
     if (Q_hist) Q_hist[it] = Q;
     if (err_hist) err_hist[it] = err;
+    if (t_hist)
+      t_hist[it] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_it).count();
     done = it + 1;
     if (tol > 0.0 && err < tol) break;
   }
   if (phiA_out) std::memcpy(phiA_out, phiA.data(), N * sizeof(double));
   if (phiB_out) std::memcpy(phiB_out, phiB.data(), N * sizeof(double));
   return done;
+}
+
+extern "C" int orc_scft_run(const orc_params* P, double* wA, double* wB,
+                            double* p, int n_iters, double tol,
+                            double* F_hist, double* Q_hist, double* err_hist,
+                            double* phiA_out, double* phiB_out) {
+  return scft_run_impl(P, wA, wB, p, n_iters, tol, F_hist, Q_hist, err_hist, phiA_out, phiB_out,
+                       nullptr);
+}
+
+extern "C" int orc_scft_run_timed(const orc_params* P, double* wA, double* wB, double* p,
+                                  int n_iters, double* F_hist, double* t_hist) {
+  return scft_run_impl(P, wA, wB, p, n_iters, 0.0, F_hist, nullptr, nullptr, nullptr, nullptr,
+                       t_hist);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -145,6 +145,13 @@ int orc_scft_run(const orc_params* P, double* wA, double* wB, double* p,
                  int n_iters, double tol, double* F_hist, double* Q_hist,
                  double* err_hist, double* phiA, double* phiB);
 
+/* The same iterations timed: t_hist[it] = wall-clock seconds of iteration it
+ * (std::chrono::steady_clock around propagation, Q, densities, the field
+ * update and F, as the reference's TxuTimer wraps PsDomain::update,
+ * polyswift/polyswift.cxx:571-574).  Returns the iterations done. */
+int orc_scft_run_timed(const orc_params* P, double* wA, double* wB, double* p,
+                       int n_iters, double* F_hist, double* t_hist);
+
 /* Analytic box stress dF/dL_d (d = x,y,z) at fixed w for the diblock: the
  * derivative of -ln Q through k^2 only (fields held fixed in cell units). */
 int orc_box_stress(const orc_params* P, const double* wA, const double* wB,

@@ -105,6 +105,7 @@ EXPORTS = {
     "psx_scft_free_energy": (ctypes.c_int, [_vp, _dp]),
     "psx_scft_box_stress": (ctypes.c_int, [_vp, _dp]),
     "psx_scft_set_quadrature": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "psx_scft_set_quad_kernel": (ctypes.c_int, [_vp, ctypes.c_int]),
     "psx_scft_set_checkpointing": (ctypes.c_int, [_vp, ctypes.c_int]),
     "psx_scft_set_cell": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     "psx_scft_param_grad": (ctypes.c_int, [_vp, _dp, _dp]),

@@ -461,7 +461,9 @@ __device__ __forceinline__ void quad_tail(const QuadArgs& a, long long r, double
 
 // Contour quadrature, densities and the field update in one pass over the
 // cells, V consecutive cells per thread (slices read as aligned V-vectors;
-// N % V == 0).  Every cell's contour sum runs in the same order for any V.
+// N % V == 0).  Every cell's contour sum runs serially in slice order for
+// any V; k_quad_split (below) sums the same terms in a wave-interleaved
+// order, so the two agree to rounding, not bit for bit.
 template <typename T, int V>
 __global__ void __launch_bounds__(kThreads)
 k_quad_update(QuadArgs a) {

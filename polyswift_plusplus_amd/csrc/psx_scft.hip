@@ -1014,6 +1014,7 @@ struct psx_scft {
   // plane length); on by default where available
   bool plane = false;
   bool in_batch = false;                // owned by a psx_batch (its tables point at our slices)
+  int quad_kernel = 0;                  // 0 auto, 1 always k_quad_update (psx_scft_set_quad_kernel)
   psx_fft* fplan = nullptr;             // FP64 plan of the spectral filters (first use)
   long long graph_gen = 0;              // bumped whenever captured graphs go stale
   // steepest-descent noise (PsPolymerUpdater::addRandUpdateConjgFields)
@@ -1635,13 +1636,13 @@ psx_status stage_noise(psx_scft* e) {
 // Always e->nb blocks: k_finish reduces that many partials.
 // Contour modes over FP32 vectors on small grids (all vectors in one round
 // of 64 per block) split each cell's sum over the block's four waves
-// (k_quad_split).
+// (k_quad_split), unless psx_scft_set_quad_kernel asked for the serial sums.
 template <typename T>
 psx_status launch_quad_kernel(psx_scft* e, const QuadArgs& a, hipStream_t stream = nullptr) {
   constexpr int V = sizeof(T) == 4 ? 4 : 1;
   if (!stream) stream = e->stream;
   const bool contour = a.mode != QM_SD_PHI && a.mode != QM_FE;
-  if (V > 1 && e->N % V == 0 && contour && e->N / V <= (long long)e->nb * 64) {
+  if (V > 1 && e->N % V == 0 && contour && e->N / V <= (long long)e->nb * 64 && e->quad_kernel == 0) {
     hipLaunchKernelGGL((k_quad_split<T, V, kThreads / 64>), dim3(e->nb), dim3(kThreads), 0, stream, a);
   } else if (V > 1 && e->N % V == 0) {
     hipLaunchKernelGGL((k_quad_update<T, V>), dim3(e->nb), dim3(kThreads), 0, stream, a);
@@ -2266,9 +2267,7 @@ extern "C" psx_status psx_scft_set_quadrature(psx_scft* e, int mode) {
   return PSX_OK;
 }
 
-extern "C" psx_status psx_scft_set_precision(psx_scft* e, psx_prec prec) {
-  if (!e) return fail(PSX_ERR_INVALID, "null engine");
-  if (prec != PSX_FP64 && prec != PSX_FP32) return fail(PSX_ERR_INVALID, "bad precision");
+static psx_status set_precision_impl(psx_scft* e, psx_prec prec) {
   if (e->prec == (int)prec) return PSX_OK;
   PSX_TRY(ensure_sweep(e, prec));
   e->prec = prec;
@@ -2280,6 +2279,26 @@ extern "C" psx_status psx_scft_set_precision(psx_scft* e, psx_prec prec) {
   }
   e->sw[prec].wfac_valid = false;
   e->swept = false;
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_scft_set_precision(psx_scft* e, psx_prec prec) {
+  if (!e) return fail(PSX_ERR_INVALID, "null engine");
+  if (prec != PSX_FP64 && prec != PSX_FP32) return fail(PSX_ERR_INVALID, "bad precision");
+  if (e->prec == (int)prec) return PSX_OK;
+  // a batch's class launches write the slices of the batch's precision, and
+  // the instance's own quadrature reads those of e->prec: they switch together
+  if (e->in_batch) return fail(PSX_ERR_STATE, "engine belongs to a batch: use psx_batch_set_precision");
+  return set_precision_impl(e, prec);
+}
+
+extern "C" psx_status psx_scft_set_quad_kernel(psx_scft* e, int kind) {
+  if (!e) return fail(PSX_ERR_INVALID, "null engine");
+  if (kind != 0 && kind != 1) return fail(PSX_ERR_INVALID, "quadrature kernel must be 0 or 1");
+  if (kind == e->quad_kernel) return PSX_OK;
+  PSX_HIP(hipStreamSynchronize(e->stream));
+  drop_graphs(e);
+  e->quad_kernel = kind;
   return PSX_OK;
 }
 
@@ -3262,7 +3281,8 @@ extern "C" psx_status psx_batch_instance(psx_batch* b, int i, psx_scft** out) {
 
 extern "C" psx_status psx_batch_set_precision(psx_batch* b, psx_prec prec) {
   if (!b) return fail(PSX_ERR_INVALID, "null batch");
-  for (auto* e : b->eng) PSX_TRY(psx_scft_set_precision(e, prec));
+  if (prec != PSX_FP64 && prec != PSX_FP32) return fail(PSX_ERR_INVALID, "bad precision");
+  for (auto* e : b->eng) PSX_TRY(set_precision_impl(e, prec));
   PSX_TRY(batch_tables(b, prec));
   b->prec = prec;
   batch_schedule(b);

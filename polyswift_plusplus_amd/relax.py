@@ -21,8 +21,11 @@ DEFAULT_MIXING = Mixing("anderson", lambda0=0.2, depth=10, beta=1.0, anderson_st
 def converge(e: DiblockSCFT, tol: float = 1e-10, max_iters: int = 5000,
              mixing: Mixing | None = None) -> dict:
     """Iterate to residual RMS < tol, then sweep once more at the final
-    fields so that slices, Q and densities belong to them."""
+    fields so that slices, Q and densities belong to them.  The histories
+    carry ``converged``: whether the last residual RMS is below tol (False
+    when max_iters ran out first)."""
     h = e.iterate(max_iters, mixing or DEFAULT_MIXING, tol=tol)
+    h["converged"] = bool(h["iters"] > 0 and h["err"][-1] < tol)
     e.update()
     return h
 
@@ -56,25 +59,39 @@ def mixed_precision_solve(e: DiblockSCFT, tol: float = 1e-8, switch_tol: float =
 
 def relax_box(e: DiblockSCFT, tol_stress: float = 1e-7, tol_scft: float = 1e-11,
               max_outer: int = 40, first_step: float = 5.0, axes=None,
-              mixing: Mixing | None = None, log=None) -> dict:
+              mixing: Mixing | None = None, max_iters: int = 5000, log=None) -> dict:
     """Relax box lengths L_d = n_d * cell_d to dF/dL_d = 0 by a secant step
     per axis (the orthorhombic cell keeps its angles).  Each outer step
     converges the fields from the previous ones, then takes the stress from
-    the adjoint of the converged sweep."""
+    the adjoint of the converged sweep.
+
+    The relaxation stops, with ``converged`` False and ``reason`` set, when an
+    inner solve misses tol_scft within max_iters (its stress belongs to
+    unconverged fields, so no step is taken from it) or when max_outer runs
+    out.  ``L`` is always the box of the last evaluated step, and the engine
+    is left at that box."""
     n = np.array(e.shape, dtype=float)
     cell = np.array(e.cell, dtype=float)
     active = np.array([s > 1 for s in e.shape]) if axes is None else np.array(axes)
     hist = []
     L_prev = g_prev = None
+    reason = "max_outer"
     for outer in range(max_outer):
-        h = converge(e, tol=tol_scft, mixing=mixing)
+        h = converge(e, tol=tol_scft, mixing=mixing, max_iters=max_iters)
         g = e.boxStress()
         F = e.getFreeE()
         L = n * cell
-        hist.append(dict(L=L.copy(), dFdL=g.copy(), F=F, iters=int(h["iters"])))
+        hist.append(dict(L=L.copy(), dFdL=g.copy(), F=F, iters=int(h["iters"]),
+                         scft_err=float(h["err"][-1]) if h["iters"] else None,
+                         scft_converged=h["converged"]))
         if log:
-            log(f"outer {outer}: {h['iters']} iterations, L = {L}, dF/dL = {g}, F = {F:.12f}")
+            log(f"outer {outer}: {h['iters']} iterations (err {hist[-1]['scft_err']}), L = {L}, "
+                f"dF/dL = {g}, F = {F:.12f}")
+        if not h["converged"]:
+            reason = "scft_not_converged"
+            break
         if np.max(np.abs(g[active])) < tol_stress:
+            reason = "stress"
             break
         step = np.zeros(3)
         for d in np.where(active)[0]:
@@ -82,7 +99,11 @@ def relax_box(e: DiblockSCFT, tol_stress: float = 1e-7, tol_scft: float = 1e-11,
                 step[d] = -first_step * g[d]
             else:
                 step[d] = -g[d] * (L[d] - L_prev[d]) / (g[d] - g_prev[d])
+        if outer == max_outer - 1:
+            break          # leave the engine at the last evaluated box
         L_prev, g_prev = L, g
         cell = (L + step) / n
         e.setCell(cell)
-    return dict(history=hist, L=n * cell, converged=bool(np.max(np.abs(hist[-1]["dFdL"][active])) < tol_stress))
+    last = hist[-1]
+    return dict(history=hist, L=last["L"].copy(), reason=reason,
+                converged=bool(reason == "stress" and last["scft_converged"]))

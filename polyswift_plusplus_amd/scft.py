@@ -344,6 +344,12 @@ class DiblockSCFT:
         m = {"reference": 0, "simpson": 0, "trapezoid": 1, "consistent": 1}.get(mode, mode)
         check(self._L.psx_scft_set_quadrature(self._h, int(m)))
 
+    def setQuadKernel(self, kind: str) -> None:
+        """'auto' (default: small FP32 grids split each cell's contour sum
+        over a block's waves) or 'serial' (every cell summed in slice order)."""
+        k = {"auto": 0, "serial": 1}.get(kind, kind)
+        check(self._L.psx_scft_set_quad_kernel(self._h, int(k)))
+
     def setCheckpointing(self, every: int) -> None:
         """Keep only every `every`-th backward slice (0: store all slices);
         densities, iterations and the stress recompute the rest chunk by

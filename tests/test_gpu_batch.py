@@ -93,3 +93,26 @@ def test_batch_rejects_mixed_step_counts():
     specs = [dict(SPECS[0]), dict(SPECS[1], ds=0.025)]
     with pytest.raises(PsxError):
         BatchSCFT(specs)
+
+
+def test_batch_instance_precision_switch_refused():
+    """An instance cannot change precision on its own: the class launches
+    write the batch precision's slices, which the instance's quadrature must
+    read.  The batch switches all of them together."""
+    print("batch instance precision switch", flush=True)
+    from polyswift_plusplus_amd import PsxError
+    b = BatchSCFT(SPECS[:2], prec="fp32")
+    try:
+        with pytest.raises(PsxError) as ex:
+            b.instances[0].setPrecision("fp64")
+        assert ex.value.status == 7
+        b.instances[0].setPrecision("fp32")      # no change: accepted
+        b.setPrecision("fp64")
+        for i, sp in enumerate(SPECS[:2]):
+            b.instances[i].setFields(*_fields(i, sp))
+        h = b.iterate(2, MIX)
+        for i, sp in enumerate(SPECS[:2]):
+            ref = _single(sp, i, "fp64", 2)
+            assert np.max(np.abs(h["F"][i] - ref["F"])) <= 1e-14 * np.max(np.abs(ref["F"]))
+    finally:
+        b.close()

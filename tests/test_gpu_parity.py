@@ -1,8 +1,9 @@
 """GPU parity of the HIP engine against the CPU oracle (FP64 to 1e-10 rel).
 
 Every call goes through the C ABI of libpsx_scft.so.  Sizes are ones the
-oracle finishes in seconds; full-size behaviour is covered by invariants in
-test_gpu_invariants.py.
+oracle finishes in seconds; the workloads at their full sizes (64^3, 128^3
+mixed precision with its invariants, the parameter sweep's shapes, 96^3
+differentiation) are in test_gpu_configs.py and test_gpu_config5.py.
 """
 import numpy as np
 import pytest
