@@ -96,10 +96,15 @@ def solution_state(e):
 
 
 def state_error(a, b):
-    """|dF|, max|dphi| and |dQ|/Q of state a against state b."""
+    """|dF|, max|dphi| and |dQ|/Q of state a against state b, and the
+    largest difference of the Fourier moduli |phi^(k)|/N (translation
+    invariant: independently converged runs can settle on slightly shifted
+    copies of the same periodic pattern)."""
+    spec = max(float(np.max(np.abs(np.abs(np.fft.rfftn(x)) - np.abs(np.fft.rfftn(y))))) / x.size
+               for x, y in ((a[0], b[0]), (a[1], b[1])))
     return dict(dF=abs(a[3] - b[3]),
                 max_dphi=float(max(np.max(np.abs(a[0] - b[0])), np.max(np.abs(a[1] - b[1])))),
-                dQ_rel=abs(a[2] / b[2] - 1.0))
+                dQ_rel=abs(a[2] / b[2] - 1.0), max_dphi_fourier_moduli=spec)
 
 
 def accuracy_leg(w, mix, tol, max_iters, switch_tol, max_fp32):
@@ -144,6 +149,7 @@ def accuracy_leg(w, mix, tol, max_iters, switch_tol, max_fp32):
     Fm = float(hfin["F"][-1])
     err_m = state_error(sm, s64)
     return dict(F_fp64=F64, F_mixed=Fm, dF=err_m["dF"], max_dphi=err_m["max_dphi"], dQ_rel=err_m["dQ_rel"],
+                max_dphi_fourier_moduli=err_m["max_dphi_fourier_moduli"],
                 error_vs="FP64 solution evaluated at its fields (the oracle reproduces that evaluation to "
                          "1e-10 at 128^3 in tests/test_gpu_configs.py)",
                 mixed_vs_fp64_restart=state_error(sm, sr),

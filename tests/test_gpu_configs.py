@@ -137,10 +137,15 @@ def test_c3_mixed_precision_error_report():
         assert h["err"][-1] < 1e-8
         dA, dB, Qd, Fd = engine_state(e64)
         dF, dphi, dQ = abs(Fm - Fd), max(np.max(np.abs(mA - dA)), np.max(np.abs(mB - dB))), abs(Qm / Qd - 1)
-        print(f"  mixed vs FP64: |dF| = {dF:.3e}, max|dphi| = {dphi:.3e}, |dQ|/Q = {dQ:.3e} "
-              f"(FP64 {h['iters']} iterations)", flush=True)
-        assert dF < 1e-8
-        assert dphi < 1e-5 and dQ < 1e-6
+        # The saddle point is degenerate under translations of the periodic
+        # pattern: two runs may converge to slightly shifted copies.  The
+        # Fourier moduli of phi are translation invariant.
+        dspec = max(float(np.max(np.abs(np.abs(np.fft.rfftn(x)) - np.abs(np.fft.rfftn(y))))) / x.size
+                    for x, y in ((mA, dA), (mB, dB)))
+        print(f"  mixed vs FP64: |dF| = {dF:.3e}, max|dphi| = {dphi:.3e}, |dQ|/Q = {dQ:.3e}, "
+              f"max||phi^(k)| - |phi^(k)||/N = {dspec:.3e} (FP64 {h['iters']} iterations)", flush=True)
+        assert dF < 1e-8 and dQ < 1e-8
+        assert dphi < 1e-4 and dspec < 1e-7
         # the FP64 reference itself against the oracle, at its fields
         print("  oracle evaluation at the FP64 solution", flush=True)
         w64A, w64B, _ = e64.getFields()
@@ -150,11 +155,18 @@ def test_c3_mixed_precision_error_report():
         assert abs(Qd / oQ - 1) < RTOL64
         assert rel(dA, oA) < RTOL64 and rel(dB, oB) < RTOL64
         assert abs(Fd - oF) < RTOL64 * abs(oF)
-        # the mixed result against the oracle directly
-        print(f"  mixed vs oracle: |dF| = {abs(Fm - oF):.3e}, "
-              f"max|dphi| = {max(np.max(np.abs(mA - oA)), np.max(np.abs(mB - oB))):.3e}, "
+        # the mixed result against the oracle directly: F and Q of the FP64
+        # solution, and one oracle evaluation at the mixed result's own fields
+        print(f"  mixed vs oracle at the FP64 solution: |dF| = {abs(Fm - oF):.3e}, "
               f"|dQ|/Q = {abs(Qm / oQ - 1):.3e}", flush=True)
-        assert abs(Fm - oF) < 1e-8
+        assert abs(Fm - oF) < 1e-8 and abs(Qm / oQ - 1) < 1e-8
+        wmA, wmB, _ = em.getFields()
+        pA, pB, pQ, pF = oracle_state(P, wmA, wmB)
+        print(f"  mixed vs oracle at its own fields: |dF| = {abs(Fm - pF):.3e}, "
+              f"max|dphi| = {max(np.max(np.abs(mA - pA)), np.max(np.abs(mB - pB))):.3e}, "
+              f"|dQ|/Q = {abs(Qm / pQ - 1):.3e}", flush=True)
+        assert abs(Fm - pF) < RTOL64 * abs(pF) and abs(Qm / pQ - 1) < RTOL64
+        assert rel(mA, pA) < RTOL64 and rel(mB, pB) < RTOL64
         # one FP32 evaluation at the FP64 solution: the sweep's own error
         em.setFields(w64A, w64B)
         em.setPrecision("fp32")
@@ -199,7 +211,8 @@ def test_c3_full_size_invariants():
             want = np.exp(-a * e.sA * C3["ds"] - b * e.sB * C3["ds"])
             got = e.calcBigQ()
             print(f"  homogeneous {prec}: Q = {got:.15e}, closed form {want:.15e}", flush=True)
-            assert abs(got / want - 1) < (1e-13 if prec == "fp64" else 2e-6)
+            # FP32: every cell rounds alike, so the per-step errors add up
+            assert abs(got / want - 1) < (1e-13 if prec == "fp64" else 2e-5)
     finally:
         e.close()
 

@@ -1,9 +1,10 @@
 """The two contour-quadrature kernels: on small FP32 grids the default
 splits each cell's contour sum over a block's four waves (k_quad_split); the
 serial kernel (k_quad_update) sums every cell in slice order.  They sum the
-same FP64 terms in different orders, so densities, steepest-descent and
-Anderson trajectories and the checkpointed chunk sums (raw sums carried
-between chunks) agree to rounding."""
+same FP64 terms in different orders, so densities and the checkpointed chunk
+sums (raw sums carried between chunks) agree to FP64 rounding; steepest-
+descent and Anderson trajectories agree until an FP32 w-factor rounds
+differently (far below the FP32 sweep's own error)."""
 import numpy as np
 import pytest
 
@@ -52,11 +53,14 @@ def test_split_and_serial_trajectories(mix):
     print(f"quad kernels: {mix.kind} trajectory", flush=True)
     a, b = pair()
     try:
+        # the first iteration starts from identical fields; later ones from
+        # fields 1e-15 apart, whose FP32 w-factors can round differently
         ha, hb = a.iterate(6, mix), b.iterate(6, mix)
-        assert rel(ha["F"], hb["F"]) < 1e-13
-        assert rel(ha["err"], hb["err"]) < 1e-11
+        assert abs(ha["F"][0] - hb["F"][0]) < 1e-13
+        assert rel(ha["F"], hb["F"]) < 1e-8
+        assert rel(ha["err"], hb["err"]) < 1e-8
         for x, y in zip(a.getFields(), b.getFields()):
-            assert rel(x, y) < 1e-12
+            assert rel(x, y) < 1e-8
     finally:
         a.close()
         b.close()
@@ -72,7 +76,7 @@ def test_split_and_serial_checkpointed_chunks():
         assert rel(pa[0], pb[0]) < 1e-14 and rel(pa[1], pb[1]) < 1e-14
         ha, hb = a.iterate(3, Mixing("anderson", lambda0=0.1, depth=3, beta=0.5)), \
             b.iterate(3, Mixing("anderson", lambda0=0.1, depth=3, beta=0.5))
-        assert rel(ha["F"], hb["F"]) < 1e-13
+        assert rel(ha["F"], hb["F"]) < 1e-8
     finally:
         a.close()
         b.close()
