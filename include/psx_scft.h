@@ -66,6 +66,25 @@ psx_status psx_fft_scaled_pair(psx_fft* plan, const void* d_data,
 psx_status psx_fft_scaled_pair_host(psx_fft* plan, const void* h_data,
                                     const void* h_kdata, void* h_res);
 
+/* The same pair without a plan and without allocating anything.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, hosts that
+ * forbid hipMalloc).  The twiddle tables and the complex values between the
+ * passes live in one static device array per GPU of PSX_NOALLOC_SCRATCH_BYTES,
+ * held until the process ends; grids need every dimension <= PSX_NOALLOC_MAX_LINE
+ * and nx*ny*nz*2*sizeof(T) <= PSX_NOALLOC_WORK_BYTES (128^3 in FP64, 2^22
+ * cells in FP32), else PSX_ERR_SHAPE.  stream NULL is the null stream.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (other streams, other threads) race and must be serialised by the caller.
+ * Results equal psx_fft_scaled_pair's to rounding (the twiddles are computed
+ * on the device). */
+#define PSX_NOALLOC_MAX_LINE 1024
+#define PSX_NOALLOC_WORK_BYTES (32u * 1024u * 1024u)
+#define PSX_NOALLOC_SCRATCH_BYTES (PSX_NOALLOC_WORK_BYTES + 3u * 2u * PSX_NOALLOC_MAX_LINE * 8u)
+psx_status psx_fft_scaled_pair_noalloc(int nx, int ny, int nz, psx_prec prec,
+                                       void* stream, const void* d_data,
+                                       const void* d_kdata, void* d_res);
+
 /* The rest of PsFFTBase (psbase/PsFFTBase.h:83-139, psfft/PsFFTW.cpp:
  * 121-194,226-296): device pointers of N elements of the plan's type,
  * asynchronous on the plan's stream, unnormalised transforms.

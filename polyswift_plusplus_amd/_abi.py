@@ -60,6 +60,7 @@ EXPORTS = {
     "psx_fft_size": (ctypes.c_size_t, [_vp]),
     "psx_fft_scaled_pair": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "psx_fft_scaled_pair_host": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "psx_fft_scaled_pair_noalloc": (ctypes.c_int, [ctypes.c_int] * 4 + [_vp, _vp, _vp, _vp]),
     "psx_fft_forward_abs": (ctypes.c_int, [_vp, _vp, _vp]),
     "psx_fft_convolve_re": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "psx_fft_scaled_pair_im": (ctypes.c_int, [_vp, _vp, _vp, _vp]),

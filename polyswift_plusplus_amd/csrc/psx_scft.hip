@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -573,38 +574,45 @@ extern "C" psx_status psx_fft_plan_destroy(psx_fft* p) {
 
 extern "C" size_t psx_fft_size(const psx_fft* p) { return p ? (size_t)p->N : 0; }
 
+// The pair over `na` axes with `work` (N complex of T) between the passes:
+// forward along z, y (in order), then x with the k multiply and its inverse,
+// then the inverses back out; the last axis keeps only the real part.
 // in_mode 1: data is the real part of the input, 2: its imaginary part
 // (scaledFFTPairIm).
 template <typename T>
-static psx_status fft_pair_impl(psx_fft* p, const void* data, const void* kdata, void* res,
-                                int in_mode = 1) {
-  const int na = (int)p->axes.size();
-  // forward along z, y (in order), then x with the k multiply and its inverse,
-  // then the inverses back out; the last axis keeps only the real part.
+static psx_status fft_pair_core(const Axis* axes, int na, int prec, void* work, hipStream_t s,
+                                const void* data, const void* kdata, void* res, int in_mode) {
   for (int i = 0; i < na; ++i) {
     CplxIO f{};
-    f.in = (i == 0) ? data : p->work;
+    f.in = (i == 0) ? data : work;
     f.in_mode = (i == 0) ? in_mode : 0;
     f.do_fwd = 1;
     const bool last = (i == na - 1);
     f.kmul = last ? kdata : nullptr;
     f.do_inv = last ? 1 : 0;
-    f.out = (last && na == 1) ? res : p->work;
+    f.out = (last && na == 1) ? res : work;
     f.out_mode = (last && na == 1) ? 1 : 0;
-    PSX_TRY(launch_cplx<T>(p->axes[i], p->prec, f, p->stream));
+    PSX_TRY(launch_cplx<T>(axes[i], prec, f, s));
   }
   for (int i = na - 2; i >= 0; --i) {
     CplxIO f{};
-    f.in = p->work;
+    f.in = work;
     f.in_mode = 0;
     f.do_fwd = 0;
     f.do_inv = 1;
     f.kmul = nullptr;
-    f.out = (i == 0) ? res : p->work;
+    f.out = (i == 0) ? res : work;
     f.out_mode = (i == 0) ? 1 : 0;
-    PSX_TRY(launch_cplx<T>(p->axes[i], p->prec, f, p->stream));
+    PSX_TRY(launch_cplx<T>(axes[i], prec, f, s));
   }
   return PSX_OK;
+}
+
+template <typename T>
+static psx_status fft_pair_impl(psx_fft* p, const void* data, const void* kdata, void* res,
+                                int in_mode = 1) {
+  return fft_pair_core<T>(p->axes.data(), (int)p->axes.size(), p->prec, p->work, p->stream, data,
+                          kdata, res, in_mode);
 }
 
 extern "C" psx_status psx_fft_scaled_pair(psx_fft* p, const void* d_data,
@@ -626,6 +634,115 @@ extern "C" psx_status psx_fft_scaled_pair_host(psx_fft* p, const void* h_data,
   PSX_HIP(hipMemcpyAsync(h_res, p->hbuf[2], bytes, hipMemcpyDeviceToHost, p->stream));
   PSX_HIP(hipStreamSynchronize(p->stream));
   return PSX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The FFT pair without a plan and without allocation
+// (psx_fft_scaled_pair_noalloc).  Its scratch -- the twiddle tables of the
+// grid's axes and the N complex values between the passes -- is one
+// file-scope device array per GPU.  In host code `g_noalloc_scratch` names a
+// host-side stand-in, never the device memory: every use goes through the
+// address hipGetSymbolAddress returns for the current device.
+// ---------------------------------------------------------------------------
+constexpr int kNoallocMaxLine = PSX_NOALLOC_MAX_LINE;
+constexpr size_t kNoallocWorkBytes = PSX_NOALLOC_WORK_BYTES;
+constexpr size_t kNoallocTwAxis = 2 * (size_t)kNoallocMaxLine * sizeof(double);   // one axis' table
+constexpr size_t kNoallocTwBytes = 3 * kNoallocTwAxis;
+constexpr size_t kNoallocScratchBytes = kNoallocTwBytes + kNoallocWorkBytes;
+static_assert(kNoallocScratchBytes == PSX_NOALLOC_SCRATCH_BYTES, "scratch layout");
+static __device__ __align__(256) unsigned char g_noalloc_scratch[kNoallocScratchBytes];
+
+// Twiddle table of one axis, as twiddles() lays it out: exp(-2 pi i j / n)
+// for j < n, in T (cosine and sine in FP64, then rounded).
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_noalloc_twiddles(int n, T* tw) {
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    double sn, cs;
+    sincospi(2.0 * (double)j / (double)n, &sn, &cs);
+    tw[2 * j] = (T)cs;
+    tw[2 * j + 1] = (T)(-sn);
+  }
+}
+
+namespace {
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[64] = {};   // device address of g_noalloc_scratch per HIP device
+
+// Device address of the scratch array on the current device, looked up once
+// per device with that device current.
+psx_status noalloc_scratch(void** out) {
+  int dev = -1;
+  PSX_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(PSX_ERR_HIP, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(g_noalloc_scratch));
+    if (rc != hipSuccess || !p)
+      return fail(PSX_ERR_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(rc));
+    g_noalloc_ptr[dev] = p;
+  }
+  *out = g_noalloc_ptr[dev];
+  return PSX_OK;
+}
+}  // namespace
+
+template <typename T>
+static psx_status noalloc_pair(const int dims[3], int prec, hipStream_t s, const void* data,
+                               const void* kdata, void* res) {
+  void* base = nullptr;
+  PSX_TRY(noalloc_scratch(&base));
+  unsigned char* sc = (unsigned char*)base;
+  Axis ax[3];
+  int na = 0;
+  const long long N = (long long)dims[0] * dims[1] * dims[2];
+  for (int d = 2; d >= 0; --d) {
+    if (dims[d] == 1) continue;
+    Axis& a = ax[na];
+    a.axis = d;
+    a.n = dims[d];
+    long long inner = 1;
+    for (int e = d + 1; e < 3; ++e) inner *= dims[e];
+    a.inner = (int)inner;
+    a.nlines = (int)(N / dims[d]);
+    a.contig = inner == 1;
+    if (!radix_plan(a.n, &a.pl)) return fail(PSX_ERR_SHAPE, "grid size has a prime factor above 13");
+    a.tw[prec] = sc + (size_t)na * kNoallocTwAxis;
+    ++na;
+  }
+  if (na == 0) {   // 1x1x1: one identity pass of length 1
+    Axis& a = ax[0];
+    a.axis = 2; a.n = 1; a.inner = 1; a.nlines = 1; a.contig = true;
+    radix_plan(1, &a.pl);
+    a.tw[prec] = sc;
+    na = 1;
+  }
+  for (int i = 0; i < na; ++i) {
+    const int blocks = (ax[i].n + kThreads - 1) / kThreads;
+    hipLaunchKernelGGL((k_noalloc_twiddles<T>), dim3(blocks), dim3(kThreads), 0, s, ax[i].n, (T*)ax[i].tw[prec]);
+    PSX_HIP(hipGetLastError());
+  }
+  return fft_pair_core<T>(ax, na, prec, sc + kNoallocTwBytes, s, data, kdata, res, 1);
+}
+
+extern "C" psx_status psx_fft_scaled_pair_noalloc(int nx, int ny, int nz, psx_prec prec, void* stream,
+                                                  const void* d_data, const void* d_kdata, void* d_res) {
+  if (!d_data || !d_kdata || !d_res) return fail(PSX_ERR_INVALID, "null argument");
+  if (prec != PSX_FP64 && prec != PSX_FP32) return fail(PSX_ERR_INVALID, "bad precision");
+  if (nx < 1 || ny < 1 || nz < 1) return fail(PSX_ERR_SHAPE, "grid dims must be >= 1");
+  if (nx > kNoallocMaxLine || ny > kNoallocMaxLine || nz > kNoallocMaxLine)
+    return fail(PSX_ERR_SHAPE, "no-alloc FFT pair: every grid dimension must be <= " +
+                                   std::to_string(kNoallocMaxLine));
+  const long long N = (long long)nx * ny * nz;
+  if ((unsigned long long)N * 2ull * prec_bytes(prec) > kNoallocWorkBytes)
+    return fail(PSX_ERR_SHAPE, "no-alloc FFT pair: N = " + std::to_string(N) + " cells need " +
+                                   std::to_string(N * 2 * (long long)prec_bytes(prec)) +
+                                   " bytes of complex scratch, more than its " +
+                                   std::to_string((long long)kNoallocWorkBytes));
+  const int dims[3] = {nx, ny, nz};
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == PSX_FP64) return noalloc_pair<double>(dims, prec, s, d_data, d_kdata, d_res);
+  return noalloc_pair<float>(dims, prec, s, d_data, d_kdata, d_res);
 }
 
 // ---------------------------------------------------------------------------
