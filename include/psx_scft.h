@@ -191,6 +191,33 @@ psx_status psx_scft_create_blocks(const psx_grid* grid, int nblocks,
                                   const psx_block* blocks, double ds,
                                   double chiN, psx_prec prec, void* stream,
                                   psx_scft** out);
+/* One polymer of a blend (a <Polymer> of kind blockCopolymer,
+ * pspolymer/PsPolymerHldr.cpp:35-83): volfrac, length (the first polymer's
+ * length is the scale length, lengthRatio = length / scale length), and its
+ * blocks: the next nblocks entries of the block array, whose head / tail
+ * indices count within the polymer. */
+#define PSX_MAX_POLYMERS 8
+typedef struct psx_polymer {
+  double volfrac;
+  double length;
+  int nblocks;
+} psx_polymer;
+
+/* Same for a blend of npolymers block copolymers sharing the A and B fields
+ * (e.g. a diblock with a homopolymer): block steps = nint(lengthRatio *
+ * lengthfrac / ds) (psbase/PsBlockBase.cpp:192-197); each polymer has its
+ * own Q from its first block; each block adds densWt * volfrac /
+ * lengthRatio * qqt / Q_p to its species (pspolymer/PsBlock.cpp:170-173);
+ * the species averages are sum volfrac * lengthfrac (psbase/PsBlockBase.cpp:
+ * 185-189); F takes -sum_p volfrac ln Q_p / lengthRatio
+ * (pseffhamil/PsPolymerUpdater.cpp:198-207).  chiN = chi * the first
+ * polymer's length.  Contour checkpointing, dF/dfA and batches are
+ * diblock / single-chain features (PSX_ERR_INVALID or PSX_ERR_STATE). */
+psx_status psx_scft_create_blend(const psx_grid* grid, int npolymers,
+                                 const psx_polymer* polymers,
+                                 const psx_block* blocks, double ds,
+                                 double chiN, psx_prec prec, void* stream,
+                                 psx_scft** out);
 psx_status psx_scft_destroy(psx_scft* e);
 /* Block count, propagation paths (stored slice arrays) and rounds (sets of
  * paths that run concurrently; a path in round r starts from finals of
@@ -257,6 +284,9 @@ psx_status psx_scft_spec_filter(psx_scft* e, int fields, double cutoff_factor,
 psx_status psx_scft_propagate(psx_scft* e);
 /* Q (PsBlock::calcBigQ, pspolymer/PsBlock.cpp:205-237): PSX_ERR_NAN on NaN. */
 psx_status psx_scft_Q(psx_scft* e, double* h_Q);
+/* Q of every polymer (one per polymer of a blend, polymer order; a single
+ * polymer gives psx_scft_Q's value); *h_n receives the polymer count. */
+psx_status psx_scft_polymer_Q(psx_scft* e, double* h_Q, int* h_n);
 /* Densities from the contour quadrature (PsBlock::setCalcQQTIntegral +
  * setPhysFields, pspolymer/PsBlock.cpp:154-196,256-380), FP64 device;
  * either output may be NULL (the engine keeps its own copy). */

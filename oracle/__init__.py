@@ -24,6 +24,7 @@ _dp = ctypes.POINTER(ctypes.c_double)
 
 
 ORC_MAX_JOIN = 8
+ORC_MAX_POLY = 8
 
 
 class OrcBlock(ctypes.Structure):
@@ -55,6 +56,23 @@ class Block:
         return ob
 
 
+class OrcPolymer(ctypes.Structure):
+    _fields_ = [("volfrac", ctypes.c_double), ("lratio", ctypes.c_double), ("nblocks", ctypes.c_int)]
+
+
+@dataclass
+class Polymer:
+    """One polymer of a blend: volfrac, lengthRatio (length / the first
+    polymer's length) and its blocks (Block, joined by indices within the
+    polymer)."""
+    volfrac: float
+    lratio: float
+    blocks: tuple
+
+    def c(self) -> OrcPolymer:
+        return OrcPolymer(self.volfrac, self.lratio, len(self.blocks))
+
+
 class OrcParams(ctypes.Structure):
     _fields_ = [
         ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
@@ -69,6 +87,8 @@ class OrcParams(ctypes.Structure):
         ("quad", ctypes.c_int),
         ("nblocks", ctypes.c_int),
         ("blocks", ctypes.POINTER(OrcBlock)),
+        ("npoly", ctypes.c_int),
+        ("polys", ctypes.POINTER(OrcPolymer)),
     ]
 
 
@@ -95,6 +115,7 @@ class Params:
     f_order: int = 0        # 0 reference (F after update), 1 consistent
     quad: int = 0           # 0 Simpson 0.33333 (reference), 1 trapezoid
     blocks: tuple = ()      # Block graph; empty = the diblock of fA, bA, bB
+    polymers: tuple = ()    # Polymer blend (replaces blocks); empty = one polymer
 
     @property
     def N(self) -> int:
@@ -108,13 +129,24 @@ class Params:
         cp = OrcParams(self.nx, self.ny, self.nz, self.dx, self.dy, self.dz,
                        self.ds, self.fA, self.chiN, self.bA, self.bB, self.mix,
                        self.lambda0, self.lambda1, self.depth, self.beta,
-                       self.anderson_start, self.f_order, self.quad, 0, None)
-        if self.blocks:
-            arr = (OrcBlock * len(self.blocks))(*[b.c() for b in self.blocks])
-            cp.nblocks = len(self.blocks)
+                       self.anderson_start, self.f_order, self.quad, 0, None, 0, None)
+        blocks = self.all_blocks()
+        if blocks:
+            arr = (OrcBlock * len(blocks))(*[b.c() for b in blocks])
+            cp.nblocks = len(blocks)
             cp.blocks = ctypes.cast(arr, ctypes.POINTER(OrcBlock))
             cp._keep = arr      # the array lives as long as the struct
+        if self.polymers:
+            parr = (OrcPolymer * len(self.polymers))(*[q.c() for q in self.polymers])
+            cp.npoly = len(self.polymers)
+            cp.polys = ctypes.cast(parr, ctypes.POINTER(OrcPolymer))
+            cp._keep_p = parr
         return cp
+
+    def all_blocks(self):
+        if self.polymers:
+            return tuple(b for q in self.polymers for b in q.blocks)
+        return tuple(self.blocks)
 
 
 def lib():
@@ -143,6 +175,10 @@ def lib():
         L.orc_sd_update.restype = ctypes.c_double
         L.orc_free_energy.argtypes = [ctypes.POINTER(OrcParams), ctypes.c_long] + [_dp] * 4 + [ctypes.c_double]
         L.orc_free_energy.restype = ctypes.c_double
+        L.orc_free_energy_poly.argtypes = [ctypes.POINTER(OrcParams), ctypes.c_long] + [_dp] * 5
+        L.orc_free_energy_poly.restype = ctypes.c_double
+        L.orc_polymer_count.argtypes = [ctypes.POINTER(OrcParams)]
+        L.orc_polymer_count.restype = ctypes.c_int
         L.orc_scft_run.argtypes = ([ctypes.POINTER(OrcParams)] + [_dp] * 3 +
                                    [ctypes.c_int, ctypes.c_double] + [_dp] * 5)
         L.orc_scft_run_timed.argtypes = ([ctypes.POINTER(OrcParams)] + [_dp] * 3 +
@@ -274,26 +310,30 @@ def graph_densities(P: Params, wA, wB, want_slices: bool = False):
     (steps+1, *shape) and indexed from its own starting end."""
     wA, wB = _f64(wA), _f64(wB)
     phiA, phiB = np.empty_like(wA), np.empty_like(wB)
-    Q = ctypes.c_double()
+    Qs = np.zeros(ORC_MAX_POLY)
     cp = P.c()
     total = lib().orc_graph_slice_count(ctypes.byref(cp))
     if total < 0:
         raise ValueError("bad block graph")
     sl = np.empty((total,) + wA.shape) if want_slices else None
-    rc = lib().orc_graph_densities(ctypes.byref(cp), _p(wA), _p(wB), _p(phiA), _p(phiB),
-                                   ctypes.cast(ctypes.byref(Q), _dp),
+    rc = lib().orc_graph_densities(ctypes.byref(cp), _p(wA), _p(wB), _p(phiA), _p(phiB), _p(Qs),
                                    _p(sl) if want_slices else None)
     if rc:
         raise RuntimeError(f"oracle graph densities failed rc={rc}")
+    Q = Qs[:len(P.polymers)].copy() if P.polymers else float(Qs[0])
     if not want_slices:
-        return phiA, phiB, Q.value
+        return phiA, phiB, Q
     out, i0 = [], 0
-    blocks = P.blocks or (Block(0, P.fA, P.bA, (), (1,)), Block(1, 1.0 - P.fA, P.bB, (0,), ()))
-    for b in blocks:
-        M = block_steps(b.frac, P.ds) + 1
+    if P.polymers:
+        pairs = [(b, q.lratio) for q in P.polymers for b in q.blocks]
+    else:
+        pairs = [(b, 1.0) for b in (P.blocks or (Block(0, P.fA, P.bA, (), (1,)),
+                                                  Block(1, 1.0 - P.fA, P.bB, (0,), ())))]
+    for b, lr in pairs:
+        M = block_steps(lr * b.frac, P.ds) + 1
         out.append((sl[i0:i0 + M], sl[i0 + M:i0 + 2 * M]))
         i0 += 2 * M
-    return phiA, phiB, Q.value, out
+    return phiA, phiB, Q, out
 
 
 def species_fracs(P: Params):
@@ -313,7 +353,14 @@ def sd_update(P: Params, wA, wB, p, phiA, phiB, Q):
 
 
 def free_energy(P: Params, wA, wB, phiA, phiB, Q) -> float:
+    """F of fields and densities; Q is one value per polymer for a blend."""
     cp = P.c()
+    if P.polymers:
+        Qs = _f64(np.atleast_1d(Q))
+        if Qs.size != len(P.polymers):
+            raise ValueError("a blend needs one Q per polymer")
+        return lib().orc_free_energy_poly(ctypes.byref(cp), wA.size, _p(_f64(wA)), _p(_f64(wB)),
+                                          _p(_f64(phiA)), _p(_f64(phiB)), _p(Qs))
     return lib().orc_free_energy(ctypes.byref(cp), wA.size, _p(_f64(wA)), _p(_f64(wB)),
                                  _p(_f64(phiA)), _p(_f64(phiB)), Q)
 

@@ -12,6 +12,7 @@
  */
 #include "psx_oracle.h"
 
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -404,6 +405,7 @@ struct GBlock {
   int species = 0;
   double frac = 0.0, b = 1.0;
   int steps = 0;
+  int poly = 0;                                 // polymer of the block
   std::vector<std::pair<int, int>> cnt[2];      // (block, its end) joined at each end
   double* q[2] = {nullptr, nullptr};            // M slices propagated from end e
   bool initSet[2] = {false, false};             // initial q at end e
@@ -419,11 +421,38 @@ std::vector<orc_block> graph_desc(const orc_params* P) {
   return {a, b};
 }
 
-// Blocks, steps and resolved junctions; false on a bad description.
+// Polymers: (volfrac, lengthRatio, first block, block count).  Without a
+// blend description, one polymer of volfrac = lengthRatio = 1 holds every block.
+struct GPoly { double vf, lr; int first, n; };
+
+std::vector<GPoly> poly_desc(const orc_params* P) {
+  const int nb = (int)graph_desc(P).size();
+  if (P->npoly <= 0 || !P->polys) return {GPoly{1.0, 1.0, 0, nb}};
+  std::vector<GPoly> out;
+  int first = 0;
+  for (int p = 0; p < P->npoly; ++p) {
+    out.push_back(GPoly{P->polys[p].volfrac, P->polys[p].lratio, first, P->polys[p].nblocks});
+    first += P->polys[p].nblocks;
+  }
+  return out;
+}
+
+// Blocks, steps and resolved junctions; false on a bad description.  A
+// blend's head/tail indices count within each polymer.
 bool build_graph(const orc_params* P, std::vector<GBlock>& B) {
   const std::vector<orc_block> d = graph_desc(P);
+  const std::vector<GPoly> pl = poly_desc(P);
   const int nb = (int)d.size();
+  if (pl.size() > ORC_MAX_POLY) return false;
+  int covered = 0;
+  for (const GPoly& g : pl) {
+    if (g.n < 1 || !(g.vf > 0) || !(g.lr > 0)) return false;
+    covered += g.n;
+  }
+  if (covered != nb) return false;
   B.assign(nb, GBlock());
+  for (int p = 0; p < (int)pl.size(); ++p)
+    for (int i = pl[p].first; i < pl[p].first + pl[p].n; ++i) B[i].poly = p;
   for (int i = 0; i < nb; ++i) {
     if (d[i].species != 0 && d[i].species != 1) return false;
     if (d[i].nhead < 0 || d[i].ntail < 0 || d[i].nhead > ORC_MAX_JOIN || d[i].ntail > ORC_MAX_JOIN)
@@ -431,7 +460,8 @@ bool build_graph(const orc_params* P, std::vector<GBlock>& B) {
     B[i].species = d[i].species;
     B[i].frac = d[i].frac;
     B[i].b = d[i].b;
-    B[i].steps = orc_block_steps(d[i].frac, P->ds);
+    // blockSteps = lengthRatio * lengthFrac / ds (psbase/PsBlockBase.cpp:192-197)
+    B[i].steps = orc_block_steps(pl[B[i].poly].lr * d[i].frac, P->ds);
     if (B[i].steps < 0) return false;
   }
   // A block j named at end e of block i is joined at j's tail when j's tail
@@ -440,14 +470,17 @@ bool build_graph(const orc_params* P, std::vector<GBlock>& B) {
     for (int e = 0; e < 2; ++e) {
       const int n = e == kHead ? d[i].nhead : d[i].ntail;
       const int* lst = e == kHead ? d[i].head : d[i].tail;
+      const GPoly& g = pl[B[i].poly];
       for (int k = 0; k < n; ++k) {
-        const int j = lst[k];
-        if (j < 0 || j >= nb || j == i) return false;
+        const int jl = lst[k];
+        if (jl < 0 || jl >= g.n) return false;
+        const int j = g.first + jl;
+        if (j == i) return false;
         for (const auto& c : B[i].cnt[e])
           if (c.first == j) return false;
         bool at_tail = false;
         for (int m = 0; m < d[j].ntail; ++m)
-          if (d[j].tail[m] == i) at_tail = true;
+          if (g.first + d[j].tail[m] == i) at_tail = true;
         B[i].cnt[e].push_back({j, at_tail ? kTail : kHead});
       }
     }
@@ -457,13 +490,18 @@ bool build_graph(const orc_params* P, std::vector<GBlock>& B) {
 }  // namespace
 
 extern "C" void orc_species_fracs(const orc_params* P, double* fA, double* fB) {
-  // f_alpha = sum of lengthfrac * volfrac over the species' blocks
-  // (pspolymer/PsPolymerHldr.cpp:69-83 with volfrac = 1)
+  // f_alpha = sum of lengthWeight * lengthFrac * volfrac over the species'
+  // blocks (psbase/PsBlockBase.cpp:185-189, lengthWeight = 1)
   double a = 0.0, b = 0.0;
-  for (const orc_block& d : graph_desc(P)) (d.species == 0 ? a : b) += d.frac;
+  const std::vector<orc_block> d = graph_desc(P);
+  for (const GPoly& g : poly_desc(P))
+    for (int i = g.first; i < g.first + g.n && i < (int)d.size(); ++i)
+      (d[i].species == 0 ? a : b) += 1.0 * d[i].frac * g.vf;
   if (fA) *fA = a;
   if (fB) *fB = b;
 }
+
+extern "C" int orc_polymer_count(const orc_params* P) { return (int)poly_desc(P).size(); }
 
 extern "C" long orc_graph_slice_count(const orc_params* P) {
   std::vector<GBlock> B;
@@ -540,32 +578,40 @@ extern "C" int orc_graph_densities(const orc_params* P, const double* wA,
     if (all) break;
     if (!progress) return -1;   // a cycle or a junction nothing reaches
   }
-  // Q from block 0's head: qHeadInitial * qHeadFinal (pspolymer/PsBlock.cpp:205-237)
-  const GBlock& b0 = B[0];
-  const double* qhi = b0.q[kHead];
-  const double* qhf = b0.q[kTail] + (long)b0.steps * N;
-  double sum = 0.0;
-  for (long n = 0; n < N; ++n) sum += qhi[n] * qhf[n];
-  const double Q = sum / (double)N;
-  if (Q != Q) return -2;
-  *Qout = Q;
-  // densities: each block adds its qqt into its species field
-  // (pspolymer/PsBlock.cpp:154-196, densWt = vf = lengthRatio = 1)
+  // Q of each polymer from its first block's head: qHeadInitial * qHeadFinal
+  // (pspolymer/PsBlock.cpp:205-237)
+  const std::vector<GPoly> pl = poly_desc(P);
+  std::vector<double> Qp(pl.size());
+  for (size_t p = 0; p < pl.size(); ++p) {
+    const GBlock& b0 = B[pl[p].first];
+    const double* qhi = b0.q[kHead];
+    const double* qhf = b0.q[kTail] + (long)b0.steps * N;
+    double sum = 0.0;
+    for (long n = 0; n < N; ++n) sum += qhi[n] * qhf[n];
+    Qp[p] = sum / (double)N;
+    if (Qp[p] != Qp[p]) return -2;
+    Qout[p] = Qp[p];
+  }
+  // densities: each block adds its qqt, scaled by densFac = densWt * vf /
+  // lengthRatio, into its species field (pspolymer/PsBlock.cpp:154-196,
+  // densWt = 1)
   for (long r = 0; r < N; ++r) { phiA[r] = 0.0; phiB[r] = 0.0; }
   std::vector<double> tmp(N);
   for (const GBlock& g : B) {
+    const double Q = Qp[g.poly];
     if (P->quad == 1) orc_quadrature_trap(N, g.steps, P->ds, g.q[kHead], g.q[kTail], Q, tmp.data());
     else orc_quadrature(N, g.steps, P->ds, g.q[kHead], g.q[kTail], Q, tmp.data());
+    const double fac = 1.0 * pl[g.poly].vf / pl[g.poly].lr;
     double* phi = g.species == 0 ? phiA : phiB;
-    for (long r = 0; r < N; ++r) phi[r] += tmp[r];
+    for (long r = 0; r < N; ++r) phi[r] += tmp[r] * fac;
   }
   return 0;
 }
 
-extern "C" double orc_free_energy(const orc_params* P, long N,
-                                  const double* wA, const double* wB,
-                                  const double* phiA, const double* phiB,
-                                  double Q) {
+extern "C" double orc_free_energy_poly(const orc_params* P, long N,
+                                       const double* wA, const double* wB,
+                                       const double* phiA, const double* phiB,
+                                       const double* Qs) {
   // pseffhamil/PsPolymerUpdater.cpp:173-226, calcUpdateFieldProd 303-341,
   // pseffhamil/PsFloryInteraction.cpp:275-321 (calcDisorder = true), bulk.
   double fA, fB;
@@ -576,12 +622,23 @@ extern "C" double orc_free_energy(const orc_params* P, long N,
     prod += (-(phiA[r] * wA[r]) - (phiB[r] * wB[r])) / vol;
     inter += (P->chiN * phiA[r] * phiB[r] - P->chiN * fA * fB) / vol;
   }
-  return -(1.0 * std::log(Q)) / 1.0 + (prod + inter);
+  // single-chain terms: sum_p -(volfrac_p ln Q_p) / lengthRatio_p (198-207)
+  const std::vector<GPoly> pl = poly_desc(P);
+  double chains = 0.0;
+  for (size_t p = 0; p < pl.size(); ++p) chains = chains - ((pl[p].vf * std::log(Qs[p])) / pl[p].lr);
+  return chains + (prod + inter);
 }
 
-extern "C" double orc_sd_update(const orc_params* P, long N, double* wA,
-                                double* wB, double* p, const double* phiA,
-                                const double* phiB, double Q) {
+extern "C" double orc_free_energy(const orc_params* P, long N,
+                                  const double* wA, const double* wB,
+                                  const double* phiA, const double* phiB,
+                                  double Q) {
+  return orc_free_energy_poly(P, N, wA, wB, phiA, phiB, &Q);
+}
+
+static double sd_update_q(const orc_params* P, long N, double* wA,
+                          double* wB, double* p, const double* phiA,
+                          const double* phiB, const double* Qs) {
   // pseffhamil/PsSteepDUpdater.cpp:159-203 with update_set2Fields 250-333
   double fA, fB;
   orc_species_fracs(P, &fA, &fB);
@@ -603,8 +660,14 @@ extern "C" double orc_sd_update(const orc_params* P, long N, double* wA,
   for (long r = 0; r < N; ++r) { p[r] = (wA[r] + wB[r]) * 0.5; sum += p[r]; }
   const double avg = sum / (double)N;
   for (long r = 0; r < N; ++r) p[r] -= avg;
-  if (P->f_order == 1) return orc_free_energy(P, N, wA0.data(), wB0.data(), phiA, phiB, Q);
-  return orc_free_energy(P, N, wA, wB, phiA, phiB, Q);
+  if (P->f_order == 1) return orc_free_energy_poly(P, N, wA0.data(), wB0.data(), phiA, phiB, Qs);
+  return orc_free_energy_poly(P, N, wA, wB, phiA, phiB, Qs);
+}
+
+extern "C" double orc_sd_update(const orc_params* P, long N, double* wA,
+                                double* wB, double* p, const double* phiA,
+                                const double* phiB, double Q) {
+  return sd_update_q(P, N, wA, wB, p, phiA, phiB, &Q);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -670,11 +733,12 @@ static int scft_run_impl(const orc_params* P, double* wA, double* wB,
   int done = 0;
   for (int it = 0; it < n_iters; ++it) {
     const auto t_it = std::chrono::steady_clock::now();
-    double Q;
+    double Qs[ORC_MAX_POLY];
     // pstrol/PsDomain.cpp:332-372: densities reset, propagate, Q, densities
-    int rc = P->nblocks > 0 ? orc_graph_densities(P, wA, wB, phiA.data(), phiB.data(), &Q, nullptr)
-                            : orc_diblock_densities(P, wA, wB, phiA.data(), phiB.data(), &Q, nullptr);
+    int rc = P->nblocks > 0 ? orc_graph_densities(P, wA, wB, phiA.data(), phiB.data(), Qs, nullptr)
+                            : orc_diblock_densities(P, wA, wB, phiA.data(), phiB.data(), Qs, nullptr);
     if (rc) return -1;
+    const double Q = Qs[0];
     // residual f = dH (pseffhamil/PsSteepDUpdater.cpp:174-184, Flory calcDfD)
     std::vector<double> f(2 * N), x(2 * N);
     double ff = 0.0;
@@ -688,7 +752,7 @@ static int scft_run_impl(const orc_params* P, double* wA, double* wB,
     const double err = std::sqrt(ff / (double)N);
     double F;
     if (P->mix == 0) {
-      F = orc_sd_update(P, N, wA, wB, p, phiA.data(), phiB.data(), Q);
+      F = sd_update_q(P, N, wA, wB, p, phiA.data(), phiB.data(), Qs);
     } else {
       std::vector<double> xn(2 * N);
       const int k = (int)hx.size();
@@ -746,9 +810,9 @@ static int scft_run_impl(const orc_params* P, double* wA, double* wB,
       const double avg = sum / (double)N;
       for (long r = 0; r < N; ++r) p[r] -= avg;
       if (P->f_order == 1)
-        F = orc_free_energy(P, N, wA0.data(), wB0.data(), phiA.data(), phiB.data(), Q);
+        F = orc_free_energy_poly(P, N, wA0.data(), wB0.data(), phiA.data(), phiB.data(), Qs);
       else
-        F = orc_free_energy(P, N, wA, wB, phiA.data(), phiB.data(), Q);
+        F = orc_free_energy_poly(P, N, wA, wB, phiA.data(), phiB.data(), Qs);
     }
     if (F_hist) F_hist[it] = F;
     if (Q_hist) Q_hist[it] = Q;
@@ -794,8 +858,9 @@ extern "C" int orc_box_stress(const orc_params* P, const double* wA,
   if (!build_graph(P, B)) return -1;
   const long total = orc_graph_slice_count(P);
   std::vector<double> S(total * N), phiA(N), phiB(N);
-  double Q;
-  if (orc_graph_densities(P, wA, wB, phiA.data(), phiB.data(), &Q, S.data())) return -1;
+  double Qs[ORC_MAX_POLY];
+  if (orc_graph_densities(P, wA, wB, phiA.data(), phiB.data(), Qs, S.data())) return -1;
+  const std::vector<GPoly> pl = poly_desc(P);
   const double twopi = 2.0 * std::acos(-1.0);
   const int g[3] = {nx, ny, nz};
   const double h[3] = {P->dx, P->dy, P->dz};
@@ -818,11 +883,12 @@ extern "C" int orc_box_stress(const orc_params* P, const double* wA,
     wf[0][r] = std::exp(-0.5 * P->ds * wA[r]);
     wf[1][r] = std::exp(-0.5 * P->ds * wB[r]);
   }
-  double acc[3] = {0, 0, 0};
+  std::vector<std::array<double, 3>> accp(pl.size(), std::array<double, 3>{0.0, 0.0, 0.0});
   std::vector<cd> a(N), b(N);
   std::vector<double> K(N);
   const double* base = S.data();
   for (const GBlock& blk : B) {
+    double* acc = accp[blk.poly].data();
     // step t of the block: q_t from the head, q+ after steps-1-t steps from the tail
     const long M = blk.steps + 1;
     const double* qh = base;
@@ -843,11 +909,16 @@ extern "C" int orc_box_stress(const orc_params* P, const double* wA,
       }
     }
   }
+  // F = -sum_p (vf_p / lr_p) ln Q_p + (terms without L at fixed fields)
   for (int d = 0; d < 3; ++d) {
     const double L = g[d] * h[d];
-    // <x, F^-1 y> = (1/N) <F x, y>, then Q = (N Q) / N
-    const double dQ = acc[d] / L / (double)N / (double)N;
-    dFdL[d] = g[d] > 1 ? -dQ / Q : 0.0;
+    double sum = 0.0;
+    for (size_t p = 0; p < pl.size(); ++p) {
+      // <x, F^-1 y> = (1/N) <F x, y>, then Q = (N Q) / N
+      const double dQ = accp[p][d] / L / (double)N / (double)N;
+      sum += (pl[p].vf / pl[p].lr) * (dQ / Qs[p]);
+    }
+    dFdL[d] = g[d] > 1 ? -sum : 0.0;
   }
   return 0;
 }
@@ -1109,10 +1180,11 @@ extern "C" int orc_scft_run_noise(const orc_params* P, double* wA, double* wB, d
   double fA, fB;
   orc_species_fracs(P, &fA, &fB);
   for (int it = 0; it < n_iters; ++it) {
-    double Q;
-    int rc = P->nblocks > 0 ? orc_graph_densities(P, wA, wB, phiA.data(), phiB.data(), &Q, nullptr)
-                            : orc_diblock_densities(P, wA, wB, phiA.data(), phiB.data(), &Q, nullptr);
+    double Qs[ORC_MAX_POLY];
+    int rc = P->nblocks > 0 ? orc_graph_densities(P, wA, wB, phiA.data(), phiB.data(), Qs, nullptr)
+                            : orc_diblock_densities(P, wA, wB, phiA.data(), phiB.data(), Qs, nullptr);
     if (rc) return -1;
+    const double Q = Qs[0];
     double ff = 0.0;
     std::vector<double> dHA(N), dHB(N);
     for (long r = 0; r < N; ++r) {
@@ -1131,7 +1203,7 @@ extern "C" int orc_scft_run_noise(const orc_params* P, double* wA, double* wB, d
     for (long r = 0; r < N; ++r) { p[r] = (wA[r] + wB[r]) * 0.5; sum += p[r]; }
     const double avg = sum / (double)N;
     for (long r = 0; r < N; ++r) p[r] -= avg;
-    const double F = orc_free_energy(P, N, wA, wB, phiA.data(), phiB.data(), Q);
+    const double F = orc_free_energy_poly(P, N, wA, wB, phiA.data(), phiB.data(), Qs);
     if (F_hist) F_hist[it] = F;
     if (Q_hist) Q_hist[it] = Q;
     if (err_hist) err_hist[it] = std::sqrt(ff / (double)N);

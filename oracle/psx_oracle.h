@@ -48,8 +48,20 @@ typedef struct orc_block {
   int head[ORC_MAX_JOIN], tail[ORC_MAX_JOIN];
 } orc_block;
 
+/* One polymer of a blend (a <Polymer> of the deck, pspolymer/PsPolymerHldr.cpp:
+ * 69-83): volfrac, lengthRatio = length / the first polymer's length, and
+ * its blocks: the next nblocks entries of orc_params.blocks, whose
+ * head/tail indices count within the polymer. */
+#define ORC_MAX_POLY 8
+typedef struct orc_polymer {
+  double volfrac;
+  double lratio;
+  int nblocks;
+} orc_polymer;
+
 /* Run parameters of one AB melt (bulk, no walls, no solvent): the diblock of
- * fA, bA, bB, or the block graph `blocks` when nblocks > 0. */
+ * fA, bA, bB, or the block graph `blocks` when nblocks > 0, or the blend of
+ * the npoly polymers `polys` (blocks then holds all their blocks in order). */
 typedef struct orc_params {
   int nx, ny, nz;        /* grid cells; nz = 1 (and ny = 1) for 2D (1D) */
   double dx, dy, dz;     /* cell sizes in Rg (reference "cellSizes") */
@@ -66,6 +78,8 @@ typedef struct orc_params {
   int quad;              /* 0: Simpson 0.33333 (reference); 1: trapezoid */
   int nblocks;           /* 0: diblock of fA; > 0: the block graph below */
   const orc_block* blocks;
+  int npoly;             /* 0: one polymer of volfrac = lengthRatio = 1 */
+  const orc_polymer* polys;
 } orc_params;
 
 /* Thread count used by the FFT line loops and element-wise loops (OpenMP). */
@@ -112,7 +126,8 @@ int orc_diblock_densities(const orc_params* P, const double* wA,
 
 /* Full propagation of a block graph (pspolymer/PsBlockCopolymer.cpp:189-313,
  * pspolymer/PsBlock.cpp:146-147,154-237,387-472): fills the species densities
- * and Q.  If slices != NULL it receives, block by block, the M = steps+1
+ * and Q (for a blend, one Q per polymer; each block's density is scaled by
+ * its polymer's volfrac / lengthRatio, pspolymer/PsBlock.cpp:170-173).  If slices != NULL it receives, block by block, the M = steps+1
  * slices propagated from the head (index from the head) followed by the M
  * slices propagated from the tail (index from the tail).  Returns -1 on a bad
  * graph (block steps, indices, a cycle), -2 on NaN Q.  Works for the diblock
@@ -124,7 +139,8 @@ int orc_graph_densities(const orc_params* P, const double* wA,
 /* Total slices that orc_graph_densities writes (in fields of N), or -1. */
 long orc_graph_slice_count(const orc_params* P);
 
-/* Species totals f_A, f_B (sum of lengthfrac over the species' blocks). */
+/* Species totals f_A, f_B (sum of lengthfrac * volfrac over the species'
+ * blocks, psbase/PsBlockBase.cpp:185-189). */
 void orc_species_fracs(const orc_params* P, double* fA, double* fB);
 
 /* One steepest-descent update + pressure shift + F, in reference order
@@ -137,6 +153,13 @@ double orc_sd_update(const orc_params* P, long n, double* wA, double* wB,
 double orc_free_energy(const orc_params* P, long n, const double* wA,
                        const double* wB, const double* phiA,
                        const double* phiB, double Q);
+/* The same for a blend: Qs holds one Q per polymer, and the chain term is
+ * -sum_p volfrac_p ln Q_p / lengthRatio_p (PsPolymerUpdater.cpp:198-207). */
+double orc_free_energy_poly(const orc_params* P, long n, const double* wA,
+                            const double* wB, const double* phiA,
+                            const double* phiB, const double* Qs);
+/* Number of polymers (1 unless npoly > 1). */
+int orc_polymer_count(const orc_params* P);
 
 /* Run SCFT iterations until n_iters or residual RMS < tol (tol <= 0: never).
  * Histories (length n_iters, may be NULL): F, Q, residual RMS.

@@ -44,6 +44,13 @@ class BlockDesc(ctypes.Structure):
                 ("head", ctypes.c_int * PSX_MAX_JOIN), ("tail", ctypes.c_int * PSX_MAX_JOIN)]
 
 
+PSX_MAX_POLYMERS = 8
+
+
+class PolymerDesc(ctypes.Structure):
+    _fields_ = [("volfrac", ctypes.c_double), ("length", ctypes.c_double), ("nblocks", ctypes.c_int)]
+
+
 class Mixing(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("lambda0", ctypes.c_double),
                 ("lambda1", ctypes.c_double), ("depth", ctypes.c_int),
@@ -79,6 +86,9 @@ EXPORTS = {
     "psx_scft_create_blocks": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.c_int, ctypes.POINTER(BlockDesc),
                                               ctypes.c_double, ctypes.c_double, ctypes.c_int, _vp,
                                               ctypes.POINTER(_vp)]),
+    "psx_scft_create_blend": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.c_int, ctypes.POINTER(PolymerDesc),
+                                             ctypes.POINTER(BlockDesc), ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_int, _vp, ctypes.POINTER(_vp)]),
     "psx_scft_destroy": (ctypes.c_int, [_vp]),
     "psx_scft_graph_info": (ctypes.c_int, [_vp, _ip, _ip, _ip]),
     "psx_scft_get_block_slice_host": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
@@ -96,6 +106,7 @@ EXPORTS = {
     "psx_scft_set_noise": (ctypes.c_int, [_vp, ctypes.c_double, _vp]),
     "psx_scft_propagate": (ctypes.c_int, [_vp]),
     "psx_scft_Q": (ctypes.c_int, [_vp, _dp]),
+    "psx_scft_polymer_Q": (ctypes.c_int, [_vp, _dp, _ip]),
     "psx_scft_densities": (ctypes.c_int, [_vp, _vp, _vp]),
     "psx_scft_get_densities_host": (ctypes.c_int, [_vp, _dp, _dp]),
     "psx_scft_get_slice_host": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),

@@ -233,8 +233,10 @@ k_cplx_lines(LineGeom g, CplxIO f) {
 
 // ------------------------------------------------------------------------
 // Device-resident iteration state (one per engine).
+constexpr int kMaxPoly = 8;   // polymers of a blend (PSX_MAX_POLYMERS)
+
 struct DevState {
-  double Q;           // partition function of the last sweep
+  double Q;           // partition function of the last sweep (of polymer 0)
   double pbar;        // mean of p_raw; the pressure is p = p_raw - pbar
   double err;         // residual RMS of the current iteration
   double F;           // free energy of the last completed iteration
@@ -246,6 +248,11 @@ struct DevState {
   double beta_used;
   double coef[kMaxDepth];
   double gram[(kMaxDepth + 1) * (kMaxDepth + 1)];
+  // blends: Q of every polymer, and its volfrac and lengthRatio (set at
+  // creation; npoly = 1 for a single block copolymer)
+  int npoly;
+  double Qp[kMaxPoly];
+  double pvf[kMaxPoly], plr[kMaxPoly];
 };
 
 // Block-wide sum of NV doubles; the result is valid in thread 0.
@@ -325,14 +332,16 @@ k_junction(JunctionArgs a) {
 }
 
 // Q = sum(part) / N  (PsBlock::calcBigQ: mean of qHeadInitial qHeadFinal).
+// Q of polymer `poly` from its partial sums (polymer 0's is also st->Q).
 __global__ void k_finish_Q(const double* __restrict__ part, int nb, double invN,
-                           DevState* st) {
+                           DevState* st, int poly) {
   double v[1] = {0.0};
   for (int i = threadIdx.x; i < nb; i += blockDim.x) v[0] += part[i];
   block_sum<1>(v);
   if (threadIdx.x == 0) {
     const double Q = v[0] * invN;
-    st->Q = Q;
+    st->Qp[poly] = Q;
+    if (poly == 0) st->Q = Q;
     if (!(Q == Q) || Q == 1.0 / 0.0 || Q == -1.0 / 0.0) st->nanflag = 1;
   }
 }
@@ -395,6 +404,8 @@ struct QuadBlock {
   const void* H;
   const void* T;
   int M, woff, species;
+  int poly;        // polymer of the block (blends)
+  double dens;     // its volfrac / lengthRatio (blends)
 };
 
 struct QuadArgs {
@@ -413,7 +424,14 @@ struct QuadArgs {
   int mode, f_order;
   int acc_in;                    // add the raw sums of earlier chunks held in phiA/phiB
   const double* noise;           // steepest descent: noise added to w_A, w_B (2N), or null
+  int npoly;                     // > 1: a blend, each block scaled by dens / Q of its polymer
 };
+
+// Scale of block b's contour sums in a blend: densWt * volfrac / lengthRatio
+// over its polymer's Q (pspolymer/PsBlock.cpp:170-173; qqtIntegral holds 1/Q).
+__device__ __forceinline__ double blend_scale(const QuadArgs& a, int b) {
+  return a.blk[b].dens * (1.0 / a.st->Qp[a.blk[b].poly]);
+}
 
 // V consecutive cells of T in one aligned load (16 bytes for V = 16 / sizeof(T)).
 template <typename T, int V>
@@ -508,6 +526,11 @@ k_quad_update(QuadArgs a) {
 #pragma unroll
           for (int v = 0; v < V; ++v) s[v] += ct * ((double)h.v[v] * (double)q.v[v]);
         }
+        if (a.npoly > 1) {
+          const double f = blend_scale(a, b);
+#pragma unroll
+          for (int v = 0; v < V; ++v) s[v] = s[v] * f;
+        }
 #pragma unroll
         for (int v = 0; v < V; ++v) {
           if (a.blk[b].species == 0) qa[v] += s[v];
@@ -521,8 +544,8 @@ k_quad_update(QuadArgs a) {
       }
 #pragma unroll
       for (int v = 0; v < V; ++v) {
-        phA[v] = qa[v] * invQ * a.dens;
-        phB[v] = qb[v] * invQ * a.dens;
+        phA[v] = a.npoly > 1 ? qa[v] : qa[v] * invQ * a.dens;
+        phB[v] = a.npoly > 1 ? qb[v] : qb[v] * invQ * a.dens;
         a.phiA[r0 + v] = phA[v];
         a.phiB[r0 + v] = phB[v];
       }
@@ -583,6 +606,11 @@ k_quad_split(QuadArgs a) {
 #pragma unroll
           for (int v = 0; v < V; ++v) sm[v] += ct * ((double)h.v[v] * (double)q.v[v]);
         }
+        if (a.npoly > 1) {
+          const double f = blend_scale(a, b);
+#pragma unroll
+          for (int v = 0; v < V; ++v) sm[v] = sm[v] * f;
+        }
 #pragma unroll
         for (int v = 0; v < V; ++v) {
           if (a.blk[b].species == 0) qa[v] += sm[v];
@@ -612,7 +640,8 @@ k_quad_split(QuadArgs a) {
           a.phiB[r0 + v] = sb;
           continue;
         }
-        const double phA = sa * invQ * a.dens, phB = sb * invQ * a.dens;
+        const double phA = a.npoly > 1 ? sa : sa * invQ * a.dens;
+        const double phB = a.npoly > 1 ? sb : sb * invQ * a.dens;
         a.phiA[r0 + v] = phA;
         a.phiB[r0 + v] = phB;
         if (a.mode != QM_DENS) quad_tail<T>(a, r0 + v, phA, phB, acc, slot, pbar);
@@ -641,7 +670,11 @@ __global__ void k_finish(const double* __restrict__ part, int nb, double invN,
   block_sum<kNPart>(v);
   if (threadIdx.x != 0) return;
   if (mode == FIN_ERR) { st->err = sqrt(v[3] * invN); return; }
-  const double F = -log(st->Q) + (v[1] * invN + v[2] * invN) - chiN * fA * fB;
+  // -sum_p volfrac ln Q_p / lengthRatio (pseffhamil/PsPolymerUpdater.cpp:198-207);
+  // one polymer of volfrac = lengthRatio = 1 gives -ln Q exactly
+  double chains = 0.0;
+  for (int p = 0; p < st->npoly; ++p) chains = chains - ((st->pvf[p] * log(st->Qp[p])) / st->plr[p]);
+  const double F = chains + (v[1] * invN + v[2] * invN) - chiN * fA * fB;
   st->F = F;
   if (mode == FIN_FE) return;
   if (mode == FIN_ITER_SD) st->err = sqrt(v[3] * invN);

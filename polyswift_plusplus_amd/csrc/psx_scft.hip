@@ -1023,6 +1023,7 @@ struct BlockInfo {
   int species = 0;                 // 0 = A, 1 = B
   double frac = 0.0, b = 1.0;
   int steps = 0;
+  int poly = 0;                    // polymer of a blend
   int bclass = 0;                  // index of the spectral tables for this b
   std::vector<std::pair<int, int>> cnt[2];   // (block, its end 0 head / 1 tail) joined at head, tail
   int path[2] = {-1, -1};          // path of the propagator from the head (0) / tail (1)
@@ -1116,8 +1117,12 @@ struct psx_scft {
   int dims[3];
   long long N;
   double cell[3];
-  double fA, fB, ds, chiN;              // fA, fB: lengthfrac summed per species
+  double fA, fB, ds, chiN;              // fA, fB: volfrac * lengthfrac summed per species
   int Ns;                               // contour steps summed over the blocks
+  // Polymers of a blend: volfrac, lengthRatio, their blocks first .. first + n - 1
+  // (one polymer of volfrac = lengthRatio = 1 for a single block copolymer)
+  struct PolyInfo { double vf = 1.0, lr = 1.0; int first = 0, n = 0; };
+  std::vector<PolyInfo> polys;
   std::vector<BlockInfo> blocks;
   std::vector<double> bval;             // distinct bSegRatio values (b classes)
   std::vector<PathInfo> paths;
@@ -1665,27 +1670,31 @@ psx_status ckpt_chunk(psx_scft* e, size_t ci) {
 
 // Q = mean(qHeadInitial qHeadFinal) of block 0 (pspolymer/PsBlock.cpp:205-237);
 // a free head has qHeadInitial = 1.
+// Q of every polymer: mean(qHeadInitial qHeadFinal) of its first block
+// (pspolymer/PsBlock.cpp:205-237).
 psx_status launch_Q(psx_scft* e) {
-  const BlockInfo& b0 = e->blocks[0];
-  const void* fin;
-  const void* ini = nullptr;
-  if (e->ckpt_K > 0) {   // block 0's head is the free start of the chain
-    fin = (const char*)e->sw[e->prec].ck + (size_t)(e->nck - 1) * e->N * prec_bytes(e->prec);
-  } else {
-    fin = slice_ptr(e, e->prec, b0.path[1], b0.off[1] + b0.steps);
-    if (!b0.cnt[0].empty()) ini = slice_ptr(e, e->prec, b0.path[0], b0.off[0]);
+  for (size_t p = 0; p < e->polys.size(); ++p) {
+    const BlockInfo& b0 = e->blocks[e->polys[p].first];
+    const void* fin;
+    const void* ini = nullptr;
+    if (e->ckpt_K > 0) {   // block 0's head is the free start of the chain
+      fin = (const char*)e->sw[e->prec].ck + (size_t)(e->nck - 1) * e->N * prec_bytes(e->prec);
+    } else {
+      fin = slice_ptr(e, e->prec, b0.path[1], b0.off[1] + b0.steps);
+      if (!b0.cnt[0].empty()) ini = slice_ptr(e, e->prec, b0.path[0], b0.off[0]);
+    }
+    const int g = e->nb;
+    if (e->prec == PSX_FP64) {
+      if (ini) hipLaunchKernelGGL(k_partial_dot<double>, dim3(g), dim3(kThreads), 0, e->stream, (const double*)ini, (const double*)fin, e->N, e->part);
+      else hipLaunchKernelGGL(k_partial_sum<double>, dim3(g), dim3(kThreads), 0, e->stream, (const double*)fin, e->N, e->part);
+    } else {
+      if (ini) hipLaunchKernelGGL(k_partial_dot<float>, dim3(g), dim3(kThreads), 0, e->stream, (const float*)ini, (const float*)fin, e->N, e->part);
+      else hipLaunchKernelGGL(k_partial_sum<float>, dim3(g), dim3(kThreads), 0, e->stream, (const float*)fin, e->N, e->part);
+    }
+    hipLaunchKernelGGL(k_finish_Q, dim3(1), dim3(kThreads), 0, e->stream, (const double*)e->part, g,
+                       1.0 / (double)e->N, e->st, (int)p);
+    PSX_HIP(hipGetLastError());
   }
-  const int g = e->nb;
-  if (e->prec == PSX_FP64) {
-    if (ini) hipLaunchKernelGGL(k_partial_dot<double>, dim3(g), dim3(kThreads), 0, e->stream, (const double*)ini, (const double*)fin, e->N, e->part);
-    else hipLaunchKernelGGL(k_partial_sum<double>, dim3(g), dim3(kThreads), 0, e->stream, (const double*)fin, e->N, e->part);
-  } else {
-    if (ini) hipLaunchKernelGGL(k_partial_dot<float>, dim3(g), dim3(kThreads), 0, e->stream, (const float*)ini, (const float*)fin, e->N, e->part);
-    else hipLaunchKernelGGL(k_partial_sum<float>, dim3(g), dim3(kThreads), 0, e->stream, (const float*)fin, e->N, e->part);
-  }
-  hipLaunchKernelGGL(k_finish_Q, dim3(1), dim3(kThreads), 0, e->stream, (const double*)e->part, g,
-                     1.0 / (double)e->N, e->st);
-  PSX_HIP(hipGetLastError());
   return PSX_OK;
 }
 
@@ -1702,8 +1711,11 @@ QuadArgs quad_args(psx_scft* e, int mode, const psx_mixing* mix) {
     a.blk[b].M = bi.steps + 1;
     a.blk[b].woff = woff;
     a.blk[b].species = bi.species;
+    a.blk[b].poly = bi.poly;
+    a.blk[b].dens = 1.0 * e->polys[bi.poly].vf / e->polys[bi.poly].lr;
     woff += bi.steps + 1;
   }
+  a.npoly = (int)e->polys.size();
   a.N = e->N;
   a.cw = e->cw;
   a.phiA = e->phiA; a.phiB = e->phiB;
@@ -1715,7 +1727,9 @@ QuadArgs quad_args(psx_scft* e, int mode, const psx_mixing* mix) {
   a.lam0 = mix ? mix->lambda0 : 0.0;
   a.lam1 = mix ? mix->lambda1 : 0.0;
   a.halfds = 0.5 * e->ds;
-  a.dens = 1.0;   // volfrac / lengthRatio (pspolymer/PsBlock.cpp:170-173)
+  // one polymer: its volfrac / lengthRatio (pspolymer/PsBlock.cpp:170-173)
+  // after the 1/Q; a blend scales every block by its own (blend_scale)
+  a.dens = 1.0 * e->polys[0].vf / e->polys[0].lr;
   a.st = e->st;
   a.part = e->part;
   a.mode = mode;
@@ -1794,6 +1808,8 @@ psx_status launch_quad_points(psx_scft* e, int mode, const psx_mixing* mix,
       q.M = hi - lo + 1;
       q.woff = woff + (lo - bs);
       q.species = bi.species;
+      q.poly = bi.poly;
+      q.dens = 1.0 * e->polys[bi.poly].vf / e->polys[bi.poly].lr;
     }
     woff += bi.steps + 1;
   }
@@ -2060,10 +2076,15 @@ static bool plane_available(const psx_scft* e) {
   return e->axes.size() == 3 && e->axes[0].n == e->axes[1].n && plane_length(e->axes[0].n);
 }
 
+// The block graph of all polymers.  e->polys is set: polymer p holds blocks
+// first .. first + n - 1, and the head/tail indices of `d` count within the
+// block's polymer.
 static psx_status build_graph(psx_scft* e, int nb, const psx_block* d) {
   if (nb < 1 || nb > PSX_MAX_BLOCKS)
     return fail(PSX_ERR_INVALID, "block count must be in [1, " + std::to_string(PSX_MAX_BLOCKS) + "]");
   e->blocks.assign(nb, BlockInfo());
+  for (size_t p = 0; p < e->polys.size(); ++p)
+    for (int i = e->polys[p].first; i < e->polys[p].first + e->polys[p].n; ++i) e->blocks[i].poly = (int)p;
   e->bval.clear();
   for (int i = 0; i < nb; ++i) {
     const psx_block& b = d[i];
@@ -2075,10 +2096,12 @@ static psx_status build_graph(psx_scft* e, int nb, const psx_block* d) {
     bi.species = b.species;
     bi.frac = b.lengthfrac;
     bi.b = b.bSegRatio;
-    bi.steps = block_steps(b.lengthfrac, e->ds);
+    // blockSteps = lengthRatio * lengthFrac / ds (psbase/PsBlockBase.cpp:192-197)
+    const double lr = e->polys[bi.poly].lr;
+    bi.steps = block_steps(lr * b.lengthfrac, e->ds);
     if (bi.steps < 0)
       return fail(PSX_ERR_STEPS, "block " + std::to_string(i) + ": non-integral or too few block steps "
-                                 "(lengthfrac/ds = " + std::to_string(b.lengthfrac / e->ds) + ")");
+                                 "(lengthRatio*lengthfrac/ds = " + std::to_string(lr * b.lengthfrac / e->ds) + ")");
     bi.bclass = -1;
     for (size_t c = 0; c < e->bval.size(); ++c)
       if (e->bval[c] == b.bSegRatio) bi.bclass = (int)c;
@@ -2090,15 +2113,18 @@ static psx_status build_graph(psx_scft* e, int nb, const psx_block* d) {
     for (int E = 0; E < 2; ++E) {
       const int n = E == 0 ? d[i].nhead : d[i].ntail;
       const int* lst = E == 0 ? d[i].head : d[i].tail;
+      const auto& pi = e->polys[e->blocks[i].poly];
       for (int k = 0; k < n; ++k) {
-        const int j = lst[k];
-        if (j < 0 || j >= nb || j == i)
+        if (lst[k] < 0 || lst[k] >= pi.n)
           return fail(PSX_ERR_INVALID, "block " + std::to_string(i) + ": joined block index out of range");
+        const int j = pi.first + lst[k];
+        if (j == i)
+          return fail(PSX_ERR_INVALID, "block " + std::to_string(i) + ": joined to itself");
         for (const auto& c : e->blocks[i].cnt[E])
           if (c.first == j) return fail(PSX_ERR_INVALID, "block " + std::to_string(i) + " joins block " + std::to_string(j) + " twice");
         bool at_tail = false;
         for (int m = 0; m < d[j].ntail; ++m)
-          if (d[j].tail[m] == i) at_tail = true;
+          if (pi.first + d[j].tail[m] == i) at_tail = true;
         e->blocks[i].cnt[E].push_back({j, at_tail ? 1 : 0});
       }
       // junction products run in ascending block index (std::map order)
@@ -2163,14 +2189,15 @@ static psx_status build_graph(psx_scft* e, int nb, const psx_block* d) {
   }
   (void)na;
   plan_launches(e);
-  // species totals, contour steps
+  // species totals (lengthWeight * lengthFrac * volfrac, psbase/PsBlockBase.cpp:
+  // 185-189), contour steps
   e->fA = e->fB = 0.0;
   e->Ns = 0;
   for (const BlockInfo& b : e->blocks) {
-    (b.species == 0 ? e->fA : e->fB) += b.frac;
+    (b.species == 0 ? e->fA : e->fB) += 1.0 * b.frac * e->polys[b.poly].vf;
     e->Ns += b.steps;
   }
-  e->diblock = nb == 2 && e->blocks[0].species == 0 && e->blocks[1].species == 1 &&
+  e->diblock = nb == 2 && e->polys.size() == 1 && e->blocks[0].species == 0 && e->blocks[1].species == 1 &&
                d[0].nhead == 0 && d[0].ntail == 1 && d[0].tail[0] == 1 &&
                d[1].nhead == 1 && d[1].head[0] == 0 && d[1].ntail == 0;
   return PSX_OK;
@@ -2236,7 +2263,8 @@ static void plan_quad_overlap(psx_scft* e) {
 }
 
 static psx_status create_engine(const psx_grid* g, int nb, const psx_block* blocks, double ds,
-                                double chiN, psx_prec prec, void* stream, psx_scft** out) {
+                                double chiN, psx_prec prec, void* stream, psx_scft** out,
+                                int npoly = 0, const psx_polymer* polymers = nullptr) {
   if (!out || !g || !blocks) return fail(PSX_ERR_INVALID, "null argument");
   *out = nullptr;
   if (g->nx < 1 || g->ny < 1 || g->nz < 1) return fail(PSX_ERR_SHAPE, "grid dims must be >= 1");
@@ -2252,6 +2280,26 @@ static psx_status create_engine(const psx_grid* g, int nb, const psx_block* bloc
   e->chiN = chiN;
   e->prec = prec;
   psx_status st = make_axes(e->dims, &e->axes);
+  if (npoly <= 0) {
+    e->polys.assign(1, psx_scft::PolyInfo());
+    e->polys[0].n = nb;
+  } else {
+    // lengthRatio = length / the first polymer's length (pspolymer/PsPolymerHldr.cpp:69-83)
+    int first = 0;
+    for (int p = 0; p < npoly && st == PSX_OK; ++p) {
+      const psx_polymer& q = polymers[p];
+      if (!(q.volfrac > 0) || !(q.length > 0) || q.nblocks < 1)
+        st = fail(PSX_ERR_INVALID, "polymer " + std::to_string(p) + ": need volfrac > 0, length > 0 and a block");
+      psx_scft::PolyInfo pi;
+      pi.vf = q.volfrac;
+      pi.lr = q.length / polymers[0].length;
+      pi.first = first;
+      pi.n = q.nblocks;
+      first += q.nblocks;
+      e->polys.push_back(pi);
+    }
+    if (st == PSX_OK && first != nb) st = fail(PSX_ERR_INVALID, "the polymers' block counts do not add up to nblocks");
+  }
   e->plane = plane_available(e);
   if (const char* v = getenv("PSX_PLANE")) e->plane = e->plane && v[0] != '0';
   if (st == PSX_OK) st = build_graph(e, nb, blocks);
@@ -2302,6 +2350,12 @@ static psx_status create_engine(const psx_grid* g, int nb, const psx_block* bloc
     DevState h;
     std::memset(&h, 0, sizeof(h));
     h.Q = 1.0;
+    h.npoly = (int)e->polys.size();
+    for (int p = 0; p < kMaxPoly; ++p) {
+      h.Qp[p] = 1.0;
+      h.pvf[p] = p < h.npoly ? e->polys[p].vf : 0.0;
+      h.plr[p] = p < h.npoly ? e->polys[p].lr : 1.0;
+    }
     st = (hipMemcpy(e->st, &h, sizeof(h), hipMemcpyHostToDevice) == hipSuccess)
              ? PSX_OK : fail(PSX_ERR_HIP, "state upload failed");
   }
@@ -2339,6 +2393,18 @@ extern "C" psx_status psx_scft_create_blocks(const psx_grid* g, int nblocks, con
                                              double ds, double chiN, psx_prec prec, void* stream,
                                              psx_scft** out) {
   return create_engine(g, nblocks, blocks, ds, chiN, prec, stream, out);
+}
+
+extern "C" psx_status psx_scft_create_blend(const psx_grid* g, int npolymers, const psx_polymer* polymers,
+                                            const psx_block* blocks, double ds, double chiN, psx_prec prec,
+                                            void* stream, psx_scft** out) {
+  if (!out || !polymers) return fail(PSX_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (npolymers < 1 || npolymers > PSX_MAX_POLYMERS)
+    return fail(PSX_ERR_INVALID, "polymer count must be in [1, " + std::to_string(PSX_MAX_POLYMERS) + "]");
+  int nb = 0;
+  for (int p = 0; p < npolymers; ++p) nb += polymers[p].nblocks > 0 ? polymers[p].nblocks : 0;
+  return create_engine(g, nb, blocks, ds, chiN, prec, stream, out, npolymers, polymers);
 }
 
 extern "C" psx_status psx_scft_graph_info(const psx_scft* e, int* nblocks, int* npaths, int* nrounds) {
@@ -2579,6 +2645,17 @@ extern "C" psx_status psx_scft_Q(psx_scft* e, double* Q) {
     (void)hipMemcpy(&e->st->nanflag, &z, sizeof(int), hipMemcpyHostToDevice);
     return fail(PSX_ERR_NAN, "Q is NaN/Inf (PsBlock::calcBigQ check)");
   }
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_scft_polymer_Q(psx_scft* e, double* Qs, int* n) {
+  if (!e || !Qs) return fail(PSX_ERR_INVALID, "null argument");
+  double Q0;
+  PSX_TRY(psx_scft_Q(e, &Q0));
+  DevState h;
+  PSX_TRY(read_state(e, &h));
+  for (int p = 0; p < (int)e->polys.size(); ++p) Qs[p] = h.Qp[p];
+  if (n) *n = (int)e->polys.size();
   return PSX_OK;
 }
 
@@ -2891,7 +2968,7 @@ static psx_status stress_step(psx_scft* e, const T* f, const T* gg, const BlockI
 // the same pairs, chunk by chunk (step t of the chain pairs forward slice t
 // with backward slice Ns-1-t).
 template <typename T>
-static psx_status stress_sweep(psx_scft* e) {
+static psx_status stress_sweep(psx_scft* e, int poly) {
   PSX_HIP(hipMemsetAsync(e->sacc, 0, 4 * sizeof(double), e->stream));
   if (e->ckpt_K > 0) {
     SweepBufs& s = e->sw[e->prec];
@@ -2910,6 +2987,7 @@ static psx_status stress_sweep(psx_scft* e) {
     return PSX_OK;
   }
   for (const BlockInfo& blk : e->blocks) {
+    if (blk.poly != poly) continue;
     for (int t = 0; t < blk.steps; ++t) {
       const T* f = (const T*)slice_ptr(e, e->prec, blk.path[0], blk.off[0] + t);
       const T* gg = (const T*)slice_ptr(e, e->prec, blk.path[1], blk.off[1] + blk.steps - 1 - t);
@@ -2923,19 +3001,26 @@ extern "C" psx_status psx_scft_box_stress(psx_scft* e, double* dFdL) {
   if (!e || !dFdL) return fail(PSX_ERR_INVALID, "null argument");
   if (!e->swept) return fail(PSX_ERR_STATE, "propagate before the box stress");
   PSX_TRY(ensure_stress(e));
-  double Q;
-  PSX_TRY(psx_scft_Q(e, &Q));
-  if (e->prec == PSX_FP64) PSX_TRY(stress_sweep<double>(e));
-  else PSX_TRY(stress_sweep<float>(e));
-  double acc[3];
-  PSX_HIP(hipMemcpyAsync(acc, e->sacc, 3 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
-  PSX_HIP(hipStreamSynchronize(e->stream));
+  double Qs[PSX_MAX_POLYMERS];
+  int np = 0;
+  PSX_TRY(psx_scft_polymer_Q(e, Qs, &np));
+  // F = -sum_p (volfrac_p / lengthRatio_p) ln Q_p + terms without L at fixed fields
+  double sum[3] = {0.0, 0.0, 0.0};
   const double N = (double)e->N;
-  for (int d = 0; d < 3; ++d) {
-    const double L = e->dims[d] * e->cell[d];
-    const double dQ = acc[d] / L / N / N;
-    dFdL[d] = e->dims[d] > 1 ? -dQ / Q : 0.0;
+  for (int p = 0; p < np; ++p) {
+    if (e->prec == PSX_FP64) PSX_TRY(stress_sweep<double>(e, p));
+    else PSX_TRY(stress_sweep<float>(e, p));
+    double acc[3];
+    PSX_HIP(hipMemcpyAsync(acc, e->sacc, 3 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    PSX_HIP(hipStreamSynchronize(e->stream));
+    const double c = e->polys[p].vf / e->polys[p].lr;
+    for (int d = 0; d < 3; ++d) {
+      const double L = e->dims[d] * e->cell[d];
+      const double dQ = acc[d] / L / N / N;
+      sum[d] += c * (dQ / Qs[p]);
+    }
   }
+  for (int d = 0; d < 3; ++d) dFdL[d] = e->dims[d] > 1 ? -sum[d] : 0.0;
   return PSX_OK;
 }
 

@@ -393,6 +393,59 @@ class MeltConfig:
     # monomerDens PhysFields in deck order: each draws N uniforms for its
     # random initial conjugate field (psphysf/PsMonomerDens.cpp:116-126)
     init_fields: tuple = ()
+    # every Polymer (BlendPolymer) in deck order; blocks / block_names above
+    # are the first one's
+    polymers: list = field(default_factory=list)
+
+    @property
+    def is_blend(self) -> bool:
+        return len(self.polymers) > 1 or any(p.volfrac != 1.0 for p in self.polymers)
+
+
+@dataclass
+class BlendPolymer:
+    """One <Polymer> of the deck: volfrac, length and its blocks (head/tail
+    indices within the polymer)."""
+    name: str
+    volfrac: float
+    length: float
+    blocks: list
+    block_names: list
+
+
+def _polymer_blocks(bnodes, fields, ds_vals):
+    """The flexPseudoSpec Blocks of one Polymer as engine Blocks (joins by
+    index within the polymer); adds each block's ds to ds_vals."""
+    from .scft import Block
+    names = [b.name for b in bnodes]
+    idx = {nm: i for i, nm in enumerate(names)}
+    blocks = []
+    for b in bnodes:
+        if b.get("kind") != "flexPseudoSpec":
+            raise DeckError(f"Block kind {b.get('kind')!r} unsupported (flexPseudoSpec only)")
+        sc = b.get("scfield")
+        if sc not in fields:
+            raise DeckError(f"block {b.name}: scfield {sc!r} is not one of the flory scfields")
+
+        def joins(key):
+            v = b.get(key)
+            if v is None:
+                raise DeckError(f"block {b.name}: {key} not set")
+            v = v if isinstance(v, list) else [v]
+            out = []
+            for nm in v:
+                if nm == "freeEnd":
+                    continue
+                if nm not in idx:
+                    raise DeckError(f"block {b.name}: {key} names unknown block {nm!r}")
+                out.append(idx[nm])
+            return tuple(out)
+
+        ds_vals.add(float(b.get("ds")))
+        # lengthRatio (length / scale length) enters the engine per polymer
+        blocks.append(Block("A" if sc == fields[0] else "B", float(b.get("lengthfrac")),
+                            float(b.get("bSegRatio", 1.0)), joins("headjoined"), joins("tailjoined")))
+    return blocks, names
 
 
 _SUPPORTED_TOP = {"Grid", "Decomp", "Comm", "FFT", "PhysField", "EffHamil", "Polymer", "History",
@@ -401,8 +454,6 @@ _SUPPORTED_TOP = {"Grid", "Decomp", "Comm", "FFT", "PhysField", "EffHamil", "Pol
 
 def configure(root: Node) -> MeltConfig:
     """Map a parsed deck onto the engine (see the module docstring)."""
-    from .scft import Block
-
     for c in root.children:
         if c.kind not in _SUPPORTED_TOP:
             raise DeckError(f"<{c.kind} {c.name}> is outside the bulk block-copolymer melt subset")
@@ -419,14 +470,14 @@ def configure(root: Node) -> MeltConfig:
     cell = tuple(float(x) for x in h) + (1.0,) * (3 - len(h))
 
     polys = root.all("Polymer")
-    if len(polys) != 1:
-        raise DeckError(f"one Polymer supported (melt of one block copolymer), found {len(polys)}")
-    poly = polys[0]
-    if poly.get("kind") != "blockCopolymer":
-        raise DeckError(f"Polymer kind {poly.get('kind')!r} unsupported (blockCopolymer only)")
-    if float(poly.get("volfrac", 1.0)) != 1.0:
-        raise DeckError("volfrac != 1 needs another species (blend), unsupported")
-    length = float(poly.get("length", 1.0))
+    if not 1 <= len(polys) <= 8:
+        raise DeckError(f"1 to 8 Polymers supported (a melt or a blend of block copolymers), "
+                        f"found {len(polys)}")
+    for poly in polys:
+        if poly.get("kind") != "blockCopolymer":
+            raise DeckError(f"Polymer kind {poly.get('kind')!r} unsupported (blockCopolymer only)")
+    # the first polymer's length scales every length (pspolymer/PsPolymerHldr.cpp:69-83)
+    length = float(polys[0].get("length", 1.0))
 
     hams = root.all("EffHamil")
     if len(hams) != 1:
@@ -444,6 +495,8 @@ def configure(root: Node) -> MeltConfig:
     fields = inter.get("scfields")
     if not (isinstance(fields, list) and len(fields) == 2):
         raise DeckError("flory scfields must name two fields")
+    if inter.get("chi") is None:
+        raise DeckError(f"flory Interaction {inter.name}: chi not set")
     chiN = float(inter.get("chi")) * length
     ups = ham.all("Updater")
     sds = [u for u in ups if u.get("kind") == "steepestDescent"]
@@ -493,38 +546,19 @@ def configure(root: Node) -> MeltConfig:
                                          float(u.get("filterStrength", 0.0)), cells, sched, nm,
                                          before=pos < seq.index(sd.name)))
 
-    bnodes = poly.all("Block")
-    if not bnodes:
-        raise DeckError("Polymer has no blocks")
-    names = [b.name for b in bnodes]
-    idx = {nm: i for i, nm in enumerate(names)}
     ds_vals = set()
-    blocks = []
-    for b in bnodes:
-        if b.get("kind") != "flexPseudoSpec":
-            raise DeckError(f"Block kind {b.get('kind')!r} unsupported (flexPseudoSpec only)")
-        sc = b.get("scfield")
-        if sc not in fields:
-            raise DeckError(f"block {b.name}: scfield {sc!r} is not one of the flory scfields")
+    polymers = []
+    for poly in polys:
+        bnodes = poly.all("Block")
+        if not bnodes:
+            raise DeckError(f"Polymer {poly.name} has no blocks")
+        pblocks, pnames = _polymer_blocks(bnodes, fields, ds_vals)
+        polymers.append(BlendPolymer(poly.name, float(poly.get("volfrac", 1.0)),
+                                     float(poly.get("length", 1.0)), pblocks, pnames))
+    blocks, names = polymers[0].blocks, polymers[0].block_names
+    if len(ds_vals) != 1:
+        raise DeckError(f"blocks must share one ds, found {sorted(ds_vals)}")
 
-        def joins(key):
-            v = b.get(key)
-            if v is None:
-                raise DeckError(f"block {b.name}: {key} not set")
-            v = v if isinstance(v, list) else [v]
-            out = []
-            for nm in v:
-                if nm == "freeEnd":
-                    continue
-                if nm not in idx:
-                    raise DeckError(f"block {b.name}: {key} names unknown block {nm!r}")
-                out.append(idx[nm])
-            return tuple(out)
-
-        ds_vals.add(float(b.get("ds")))
-        # lengthRatio = length / scaleLength = 1 for the only polymer
-        blocks.append(Block("A" if sc == fields[0] else "B", float(b.get("lengthfrac")),
-                            float(b.get("bSegRatio", 1.0)), joins("headjoined"), joins("tailjoined")))
     if len(ds_vals) != 1:
         raise DeckError(f"blocks must share one ds, found {sorted(ds_vals)}")
     hist = [hh for hh in root.all("History") if hh.get("kind") == "freeEnergy"]
@@ -541,6 +575,7 @@ def configure(root: Node) -> MeltConfig:
         init_fields.append(pf.name)
     dump = root.get("dumpPeriodicity")
     return MeltConfig(shape=shape, cell=cell, blocks=blocks, block_names=names, ds=ds_vals.pop(),
+                      polymers=polymers,
                       chiN=chiN, lambdas=lam, noise=noise, nsteps=int(root.get("nsteps", 1)),
                       seed=None if seed is None else int(seed), species_fields=tuple(fields),
                       history_period=period, constraint_field=cons[0] if cons else "defaultPressure",
@@ -549,8 +584,12 @@ def configure(root: Node) -> MeltConfig:
 
 
 def build(cfg: MeltConfig, prec="fp64", stream=None):
-    """The engine for a configured deck."""
-    from .scft import BlockCopolymerSCFT
+    """The engine for a configured deck: one block copolymer, or a blend."""
+    from .scft import BlendSCFT, BlockCopolymerSCFT, Polymer
+    if cfg.is_blend:
+        return BlendSCFT(cfg.shape, cfg.cell,
+                         [Polymer(p.volfrac, p.length, tuple(p.blocks)) for p in cfg.polymers],
+                         cfg.ds, cfg.chiN, prec=prec, stream=stream)
     return BlockCopolymerSCFT(cfg.shape, cfg.cell, cfg.blocks, cfg.ds, cfg.chiN, prec=prec,
                               stream=stream)
 

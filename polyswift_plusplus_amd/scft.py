@@ -480,6 +480,58 @@ class BlockCopolymerSCFT(DiblockSCFT):
             self.setQuadrature(quadrature)
 
 
+@dataclass
+class Polymer:
+    """One ``<Polymer>`` of a blend (kind blockCopolymer,
+    pspolymer/PsPolymerHldr.cpp:35-83): volfrac, length (the first polymer's
+    length is the scale length) and its ``Block``s, joined by indices
+    within the polymer."""
+    volfrac: float
+    length: float
+    blocks: tuple
+
+
+class BlendSCFT(DiblockSCFT):
+    """Bulk melt of several AB block copolymers sharing the A and B fields
+    (e.g. a diblock with a homopolymer, the reference's
+    pstests/tests/diblock/abHomopMix.pre): each polymer has its own Q and
+    contributes volfrac / lengthRatio of its qqt to the densities.  Same
+    methods as ``DiblockSCFT``; ``calcBigQ`` is the first polymer's Q and
+    ``polymerQ`` gives all of them."""
+
+    def __init__(self, shape, cell, polymers, ds, chiN, prec="fp64", stream=None,
+                 quadrature="reference"):
+        self._L = _abi.load()
+        self.shape = tuple(int(s) for s in shape)
+        self.cell = tuple(float(c) for c in cell)
+        self.N = int(np.prod(self.shape))
+        self.polymers = [p if isinstance(p, Polymer) else Polymer(*p) for p in polymers]
+        for p in self.polymers:
+            p.blocks = tuple(b if isinstance(b, Block) else Block(*b) for b in p.blocks)
+        self.blocks = [b for p in self.polymers for b in p.blocks]
+        self.ds, self.chiN = float(ds), float(chiN)
+        self.fA = sum(p.volfrac * b.lengthfrac for p in self.polymers for b in p.blocks
+                      if b.species in ("A", 0))
+        g = _abi.Grid(*self.shape, *self.cell)
+        parr = (_abi.PolymerDesc * len(self.polymers))(
+            *[_abi.PolymerDesc(float(p.volfrac), float(p.length), len(p.blocks)) for p in self.polymers])
+        arr = (_abi.BlockDesc * len(self.blocks))(*[b.c() for b in self.blocks])
+        h = ctypes.c_void_p()
+        check(self._L.psx_scft_create_blend(ctypes.byref(g), len(self.polymers), parr, arr, self.ds,
+                                            self.chiN, _prec(prec), stream, ctypes.byref(h)))
+        self._h = h
+        self._steps()
+        if quadrature != "reference":
+            self.setQuadrature(quadrature)
+
+    def polymerQ(self) -> np.ndarray:
+        """Q of every polymer, in polymer order."""
+        out = np.zeros(_abi.PSX_MAX_POLYMERS)
+        n = ctypes.c_int()
+        check(self._L.psx_scft_polymer_Q(self._h, _ptr(out), ctypes.byref(n)))
+        return out[:n.value].copy()
+
+
 class BatchSCFT:
     """Independent diblock instances (parameter sweep) advanced together.
 

@@ -77,9 +77,29 @@ def test_unsupported_physics_raises():
     base = open(os.path.join(DECKS, "melt_diblock.pre")).read()
     with pytest.raises(DeckError, match="Solvent"):
         deck.configure(deck.parse(base + "\n<Solvent s>\n kind = simpleSolvent\n</Solvent>\n"))
-    with pytest.raises(DeckError, match="one Polymer"):
-        extra = base[base.index("<Polymer"):base.index("</Polymer>") + len("</Polymer>")]
-        deck.configure(deck.parse(base + "\n" + extra.replace("diblock1", "diblock2")))
+    extra = base[base.index("<Polymer"):base.index("</Polymer>") + len("</Polymer>")]
+    with pytest.raises(DeckError, match="1 to 8 Polymers"):
+        deck.configure(deck.parse(base + "".join("\n" + extra.replace("diblock1", f"copy{i}")
+                                                 for i in range(8))))
+    with pytest.raises(DeckError, match="blockCopolymer only"):
+        deck.configure(deck.parse(base + "\n" + extra.replace("diblock1", "d2").replace(
+            "kind = blockCopolymer", "kind = polydisperse")))
+
+
+def test_blend_deck():
+    """A diblock with an A homopolymer of half its length (the shape of the
+    reference's pstests/tests/diblock/abHomopMix.pre, with a constant chi)."""
+    cfg = deck.configure(deck.load(os.path.join(DECKS, "ab_homop_mix.pre")))
+    assert cfg.is_blend and len(cfg.polymers) == 2
+    d, h = cfg.polymers
+    assert (d.volfrac, d.length, h.volfrac, h.length) == (0.9, 100.0, 0.1, 50.0)
+    assert [(b.species, b.lengthfrac, b.head, b.tail) for b in d.blocks] == [
+        ("A", 0.5, (), (1,)), ("B", 0.5, (0,), ())]
+    assert [(b.species, b.lengthfrac, b.head, b.tail) for b in h.blocks] == [("A", 1.0, (), ())]
+    assert h.block_names == ["blockHomoA"]
+    assert cfg.chiN == pytest.approx(0.12 * 100) and cfg.ds == 0.05
+    single = deck.configure(deck.load(os.path.join(DECKS, "melt_diblock.pre")))
+    assert not single.is_blend and len(single.polymers) == 1
 
 
 # ---- the reference's own decks (read from the reference tree when present)
@@ -125,8 +145,9 @@ def test_reference_example_with_expressions():
 @pytest.mark.parametrize("name", ["chargedAB.pre", "abSolventMix.pre", "abHomopMix.pre",
                                   "polydBulk.pre", "pspi.pre", "tri3abc.pre"])
 def test_reference_decks_outside_the_subset_raise(name):
-    """Decks with solvents, blends, charges, polydispersity, chi functions or
-    three species parse, then stop with a DeckError naming the feature."""
+    """Decks with solvents, charges, polydispersity, chi functions (the blend
+    abHomopMix.pre ramps its chi with an STFunc) or three species parse, then
+    stop with a DeckError naming the feature."""
     path = os.path.join(REF_DECKS, name)
     try:
         root = deck.load(path)
@@ -134,6 +155,27 @@ def test_reference_decks_outside_the_subset_raise(name):
         return
     with pytest.raises(DeckError):
         deck.configure(root)
+
+
+@needs_ref
+def test_reference_blend_deck_polymers():
+    """abHomopMix.pre's two Polymers configure as a blend once its chi
+    ramp (an STFunc) is replaced by a constant chi
+    (pstests/tests/diblock/abHomopMix.pre:117-160)."""
+    root = deck.load(os.path.join(REF_DECKS, "abHomopMix.pre"))
+    inter = root.all("EffHamil")[0].all("Interaction")[0]
+    with pytest.raises(DeckError, match="chi functions"):
+        deck.configure(root)
+    inter.children = []
+    with pytest.raises(DeckError, match="chi not set"):
+        deck.configure(root)
+    inter.attrs["chi"] = 0.12
+    cfg = deck.configure(root)
+    assert cfg.chiN == pytest.approx(12.0)
+    assert cfg.is_blend
+    assert [(p.name, p.volfrac, p.length, len(p.blocks)) for p in cfg.polymers] == [
+        ("diblock1", 0.9, 100.0, 2), ("homopolymerA", 0.1, 50.0, 1)]
+    assert cfg.polymers[1].blocks[0].species == "A" and cfg.polymers[1].blocks[0].lengthfrac == 1.0
 
 
 @needs_ref
