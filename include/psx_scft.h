@@ -338,6 +338,27 @@ psx_status psx_scft_free_energy(psx_scft* e, double* h_F);
  * analytic stress).  Requires a preceding propagate. */
 psx_status psx_scft_box_stress(psx_scft* e, double* h_dFdL);
 
+/* Reverse mode of the forward op (w_A, w_B, box lengths) -> (phi_A, phi_B,
+ * Q) at the current fields, for a scalar L = <g_phiA, phi_A> +
+ * <g_phiB, phi_B> + g_Q Q (sums over cells): given the cotangents (device,
+ * N each, NULL = zero) it writes dL/dw_A and dL/dw_B (device, N each) and,
+ * when h_dLdbox is not NULL, dL/dL_x,y,z (host, 3).  The adjoint of each
+ * stored propagator runs as one more sweep with the same step operators,
+ * sources from the other propagator; the box terms reuse the stress's
+ * spectral sums.  With g_phi = 0, g_Q = -1/Q it gives the derivatives of
+ * -ln Q, the box stress at fixed fields.  Requires a preceding propagate,
+ * FP64 sweeps with every slice stored, and one linear chain oriented head to
+ * tail from block 0 (the diblock, linear multiblocks): PSX_ERR_STATE /
+ * PSX_ERR_INVALID otherwise.  No reference counterpart. */
+psx_status psx_scft_backward(psx_scft* e, const double* d_gphiA,
+                             const double* d_gphiB, double g_Q, double* d_dLdwA,
+                             double* d_dLdwB, double* h_dLdbox);
+/* Same with host arrays. */
+psx_status psx_scft_backward_host(psx_scft* e, const double* h_gphiA,
+                                  const double* h_gphiB, double g_Q,
+                                  double* h_dLdwA, double* h_dLdwB,
+                                  double* h_dLdbox);
+
 /* Contour quadrature of the densities: 0 = the reference's Simpson rule
  * with the literal 0.33333 and a trapezoid tail (pspolymer/PsBlock.cpp:
  * 256-380; the default), 1 = trapezoid weights ds [1/2, 1, ..., 1, 1/2],

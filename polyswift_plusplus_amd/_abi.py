@@ -116,6 +116,8 @@ EXPORTS = {
     "psx_scft_reset_history": (ctypes.c_int, [_vp]),
     "psx_scft_free_energy": (ctypes.c_int, [_vp, _dp]),
     "psx_scft_box_stress": (ctypes.c_int, [_vp, _dp]),
+    "psx_scft_backward": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, _vp, _vp, _dp]),
+    "psx_scft_backward_host": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_double, _dp, _dp, _dp]),
     "psx_scft_set_quadrature": (ctypes.c_int, [_vp, ctypes.c_int]),
     "psx_scft_set_quad_kernel": (ctypes.c_int, [_vp, ctypes.c_int]),
     "psx_scft_set_checkpointing": (ctypes.c_int, [_vp, ctypes.c_int]),

@@ -908,6 +908,55 @@ __global__ void k_stress_pack(const T* __restrict__ f, const T* __restrict__ g,
   }
 }
 
+// ------------------------------------------------------------------------
+// Reverse mode of the forward op w -> (phi, Q) (psx_scft_backward).  For a
+// chain with forward slices F_t and backward slices G_u, the cotangents g_A,
+// g_B enter through J = sum_t sum_r h_t F_t G_{Ns-t} with
+// h_t = cA_t g_A + cB_t g_B (the quadrature weights of the blocks at point t
+// by species).  The adjoint of the forward sweep runs backwards from the
+// tail, lam_t = T_t lam_{t+1} + h_t G_{Ns-t}, and every step adds
+// dJ/dw(r) = -(ds/2) (lam_{t+1} F_{t+1} + F_t T_t lam_{t+1}) to its
+// species; the backward sweep's adjoint is the same with F and G swapped.
+
+// lam = h * src (the first adjoint slice); g_A / g_B may be null (zero).
+__global__ void k_adj_init(const double* __restrict__ src, const double* __restrict__ gA,
+                           const double* __restrict__ gB, double cA, double cB, long long N,
+                           double* __restrict__ lam) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double h = (gA ? cA * gA[i] : 0.0) + (gB ? cB * gB[i] : 0.0);
+    lam[i] = h * src[i];
+  }
+}
+
+// One adjoint step after u = T lnext: lcur = u + h src, grad += -(ds/2)
+// (lnext x1 + x0 u); with qgrad, also the step's trapezoid terms of dQ/dw,
+// qgrad += x1 y0 + x0 y1 (F_{t+1} G_{Ns-1-t} + F_t G_{Ns-t}).
+__global__ void __launch_bounds__(kThreads)
+k_adj_step(const double* __restrict__ u, const double* __restrict__ lnext,
+           const double* __restrict__ x0, const double* __restrict__ x1,
+           const double* __restrict__ src, const double* __restrict__ gA,
+           const double* __restrict__ gB, double cA, double cB, double halfds, long long N,
+           double* __restrict__ lcur, double* __restrict__ grad,
+           const double* __restrict__ y0, const double* __restrict__ y1, double* __restrict__ qgrad) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double uu = u[i];
+    const double h = (gA ? cA * gA[i] : 0.0) + (gB ? cB * gB[i] : 0.0);
+    lcur[i] = uu + h * src[i];
+    grad[i] += -halfds * (lnext[i] * x1[i] + x0[i] * uu);
+    if (qgrad) qgrad[i] += x1[i] * y0[i] + x0[i] * y1[i];
+  }
+}
+
+// dL/dw = grad / Q + coef * qgrad (coef = (g_Q - <g, phi> / Q) * (-(ds/2) / N)).
+__global__ void k_adj_combine(const double* __restrict__ grad, const double* __restrict__ qgrad,
+                              double invQ, double coef, long long N, double* __restrict__ out) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long long)gridDim.x * blockDim.x)
+    out[i] = grad[i] * invQ + coef * qgrad[i];
+}
+
 struct StressArgs {
   const double2* z;
   int nx, ny, nz;

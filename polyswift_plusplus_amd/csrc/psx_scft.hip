@@ -1137,6 +1137,12 @@ struct psx_scft {
   bool plane = false;
   bool in_batch = false;                // owned by a psx_batch (its tables point at our slices)
   int quad_kernel = 0;                  // 0 auto, 1 always k_quad_update (psx_scft_set_quad_kernel)
+  int quad_mode = 0;                    // contour weights: 0 reference Simpson, 1 trapezoid
+  // reverse mode (psx_scft_backward): two adjoint slices, the step result,
+  // dJ/dw and the dQ/dw sums per species, and the line-pass table
+  double* adj = nullptr;                // 7 N
+  PairIO* adj_tab = nullptr;
+  size_t adj_tab_rows = 0;
   psx_fft* fplan = nullptr;             // FP64 plan of the spectral filters (first use)
   long long graph_gen = 0;              // bumped whenever captured graphs go stale
   // steepest-descent noise (PsPolymerUpdater::addRandUpdateConjgFields)
@@ -1990,6 +1996,7 @@ extern "C" psx_status psx_scft_destroy(psx_scft* e) {
   dfree((void*&)e->part); dfree((void*&)e->st); dfree((void*&)e->hist);
   dfree((void*&)e->ringx); dfree((void*&)e->ringf); dfree(e->hstage);
   dfree(e->zbuf); dfree((void*&)e->sacc);
+  dfree((void*&)e->adj); dfree((void*&)e->adj_tab);
   for (auto& p : e->kd2) dfree((void*&)p);
   for (auto& b : e->kf1) for (auto& p : b) dfree((void*&)p);
   for (auto& a : e->caxes) { dfree(a.tw[0]); dfree(a.tw[1]); }
@@ -2446,6 +2453,7 @@ extern "C" psx_status psx_scft_set_quadrature(psx_scft* e, int mode) {
   std::vector<double> w = all_weights(e, mode);
   PSX_HIP(hipStreamSynchronize(e->stream));
   PSX_HIP(hipMemcpy(e->cw, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
+  e->quad_mode = mode;
   e->dens_valid = false;
   return PSX_OK;
 }
@@ -3021,6 +3029,182 @@ extern "C" psx_status psx_scft_box_stress(psx_scft* e, double* dFdL) {
     }
   }
   for (int d = 0; d < 3; ++d) dFdL[d] = e->dims[d] > 1 ? -sum[d] : 0.0;
+  return PSX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Reverse mode of the forward op (w_A, w_B, box) -> (phi_A, phi_B, Q)
+// ---------------------------------------------------------------------------
+static bool oriented_chain(const psx_scft* e) {
+  bool chain = e->rounds.size() == 1 && e->paths.size() == 2 && e->paths[0].init.empty() &&
+               e->blocks[0].cnt[0].empty() && e->polys.size() == 1;
+  for (const BlockInfo& b : e->blocks) chain = chain && b.path[0] == 0 && b.path[1] == 1;
+  return chain;
+}
+
+// Quadrature weights per contour point t by species: point t of block b at
+// index j = t - off_b gets the block's weight j (the junction point gets one
+// from each block).
+static void point_weights(const psx_scft* e, std::vector<double>* cA, std::vector<double>* cB) {
+  const int Ns = e->paths[0].len;
+  cA->assign(Ns + 1, 0.0);
+  cB->assign(Ns + 1, 0.0);
+  const std::vector<double> w = all_weights(e, e->quad_mode);
+  size_t woff = 0;
+  for (const BlockInfo& b : e->blocks) {
+    for (int j = 0; j <= b.steps; ++j) (b.species == 0 ? *cA : *cB)[b.off[0] + j] += w[woff + j];
+    woff += b.steps + 1;
+  }
+}
+
+static double host_dot(psx_scft* e, const double* x, const double* y) {
+  hipLaunchKernelGGL(k_partial_dot<double>, dim3(e->nb), dim3(kThreads), 0, e->stream, x, y, e->N, e->part);
+  std::vector<double> h(e->nb);
+  if (hipMemcpyAsync(h.data(), e->part, e->nb * sizeof(double), hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
+      hipStreamSynchronize(e->stream) != hipSuccess)
+    return std::nan("");
+  double v = 0.0;
+  for (double x2 : h) v += x2;
+  return v;
+}
+
+extern "C" psx_status psx_scft_backward(psx_scft* e, const double* gA, const double* gB, double gQ,
+                                        double* gwA, double* gwB, double* gL) {
+  if (!e || !gwA || !gwB) return fail(PSX_ERR_INVALID, "null argument");
+  if (!e->swept) return fail(PSX_ERR_STATE, "propagate before the backward pass");
+  if (e->prec != PSX_FP64) return fail(PSX_ERR_STATE, "the backward pass runs on FP64 sweeps");
+  if (e->ckpt_K > 0) return fail(PSX_ERR_STATE, "the backward pass reads every stored slice (checkpointing on)");
+  if (!oriented_chain(e))
+    return fail(PSX_ERR_INVALID, "the backward pass needs one linear chain oriented head to tail from block 0");
+  const long long N = e->N;
+  const int Ns = e->paths[0].len;
+  const size_t fb = (size_t)N * sizeof(double);
+  const size_t rows = (size_t)2 * Ns * e->axes.size();
+  if (!e->adj) PSX_TRY(dalloc((void**)&e->adj, 7 * fb));
+  if (e->adj_tab_rows < rows) {
+    dfree((void*&)e->adj_tab);
+    PSX_TRY(dalloc((void**)&e->adj_tab, rows * sizeof(PairIO)));
+    e->adj_tab_rows = rows;
+  }
+  double* lam[2] = {e->adj, e->adj + N};
+  double* u = e->adj + 2 * N;
+  double* grad[2] = {e->adj + 3 * N, e->adj + 4 * N};
+  double* qg[2] = {e->adj + 5 * N, e->adj + 6 * N};
+  SweepBufs& s = e->sw[PSX_FP64];
+  auto F = [&](int t) { return (const double*)slice_ptr(e, PSX_FP64, 0, t); };
+  auto G = [&](int t) { return (const double*)slice_ptr(e, PSX_FP64, 1, t); };
+  double Q;
+  PSX_TRY(psx_scft_Q(e, &Q));
+  if (!e->dens_valid) PSX_TRY(psx_scft_densities(e, nullptr, nullptr));
+  std::vector<double> cA, cB;
+  point_weights(e, &cA, &cB);
+  // line passes of both adjoint sweeps: step k of sweep d applies the step
+  // operator of contour step s (d = 0: s = Ns-1-k, from the tail; d = 1:
+  // s = k, in the backward sweep's order), the first pass times W, the last
+  // pass times W
+  const int na = (int)e->axes.size();
+  std::vector<PairIO> tab(rows);
+  for (int d = 0; d < 2; ++d)
+    for (int k = 0; k < Ns; ++k) {
+      const int sstep = d == 0 ? Ns - 1 - k : k;
+      const BlockInfo& blk = e->blocks[e->paths[0].step_block[sstep]];
+      for (int a = 0; a < na; ++a) {
+        PairIO& r = tab[((size_t)d * Ns + k) * na + a];
+        std::memset(&r, 0, sizeof(r));
+        r.in = a == 0 ? (const void*)lam[k % 2] : (const void*)u;
+        r.out = u;
+        r.pre = a == 0 ? s.wf[blk.species] : nullptr;
+        r.post = a == na - 1 ? s.wf[blk.species] : nullptr;
+        r.emul = s.emul[blk.bclass][a];
+      }
+    }
+  PSX_HIP(hipMemcpyAsync(e->adj_tab, tab.data(), rows * sizeof(PairIO), hipMemcpyHostToDevice, e->stream));
+  const int g = grid_1d(N);
+  const double halfds = 0.5 * e->ds;
+  const bool box = gL != nullptr;
+  double accJ[3] = {0.0, 0.0, 0.0};
+  if (box) PSX_TRY(ensure_stress(e));
+  hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kThreads), 0, e->stream, e->adj + 3 * N, 4 * N, 0.0);
+  for (int d = 0; d < 2; ++d) {
+    // d = 0: adjoint of the forward sweep, from lam_Ns = h_Ns G_0 down to t = 0;
+    // d = 1: adjoint of the backward sweep, from mu_Ns = h_0 F_0
+    if (box) PSX_HIP(hipMemsetAsync(e->sacc, 0, 4 * sizeof(double), e->stream));
+    const int t0 = d == 0 ? Ns : 0;
+    hipLaunchKernelGGL(k_adj_init, dim3(g), dim3(kThreads), 0, e->stream, d == 0 ? G(0) : F(0), gA, gB,
+                       cA[t0], cB[t0], N, lam[0]);
+    for (int k = 0; k < Ns; ++k) {
+      const int sstep = d == 0 ? Ns - 1 - k : k;
+      const BlockInfo& blk = e->blocks[e->paths[0].step_block[sstep]];
+      for (int a = 0; a < na; ++a)
+        PSX_TRY(launch_pair<double>(e->axes[a], PSX_FP64, e->adj_tab + ((size_t)d * Ns + k) * na + a, 1,
+                                    e->stream, 0));
+      const int sp = blk.species;
+      if (d == 0) {
+        // forward step s = Ns-1-k: lam_s = T lam_{s+1} + h_s G_{Ns-s}
+        const int st = Ns - 1 - k;
+        hipLaunchKernelGGL(k_adj_step, dim3(g), dim3(kThreads), 0, e->stream, u, lam[k % 2], F(st), F(st + 1),
+                           G(Ns - st), gA, gB, cA[st], cB[st], halfds, N, lam[(k + 1) % 2], grad[sp],
+                           G(Ns - 1 - st), G(Ns - st), qg[sp]);
+        if (box) PSX_TRY(stress_step<double>(e, F(st), lam[k % 2], blk));
+      } else {
+        // backward step u = k (the operator of forward step Ns-1-k):
+        // mu_u = T mu_{u+1} + h_{Ns-u} F_{Ns-u}
+        const int uu = Ns - 1 - k;
+        hipLaunchKernelGGL(k_adj_step, dim3(g), dim3(kThreads), 0, e->stream, u, lam[k % 2], G(uu), G(uu + 1),
+                           F(Ns - uu), gA, gB, cA[Ns - uu], cB[Ns - uu], halfds, N, lam[(k + 1) % 2],
+                           grad[sp], nullptr, nullptr, nullptr);
+        if (box) PSX_TRY(stress_step<double>(e, G(uu), lam[k % 2], blk));
+      }
+      PSX_HIP(hipGetLastError());
+    }
+    if (box) {
+      double acc[3];
+      PSX_HIP(hipMemcpyAsync(acc, e->sacc, 3 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+      PSX_HIP(hipStreamSynchronize(e->stream));
+      for (int i = 0; i < 3; ++i) accJ[i] += acc[i];
+    }
+  }
+  // L = J / Q + gQ Q with J = Q <g, phi>
+  double gphi = 0.0;
+  if (gA) gphi += host_dot(e, gA, e->phiA);
+  if (gB) gphi += host_dot(e, gB, e->phiB);
+  if (!(gphi == gphi)) return fail(PSX_ERR_HIP, "reduction failed");
+  // dL/dw = (1/Q) dJ/dw + (gQ - <g, phi> / Q) dQ/dw, dQ/dw = -(ds / 2N) qgrad
+  const double coefQ = (gQ - gphi / Q) * (-halfds / (double)N);
+  hipLaunchKernelGGL(k_adj_combine, dim3(g), dim3(kThreads), 0, e->stream, grad[0], qg[0], 1.0 / Q, coefQ, N, gwA);
+  hipLaunchKernelGGL(k_adj_combine, dim3(g), dim3(kThreads), 0, e->stream, grad[1], qg[1], 1.0 / Q, coefQ, N, gwB);
+  PSX_HIP(hipGetLastError());
+  if (box) {
+    // dQ/dL from the pairs (F_t, G_{Ns-1-t}), as the box stress
+    PSX_TRY(stress_sweep<double>(e, 0));
+    double accQ[3];
+    PSX_HIP(hipMemcpyAsync(accQ, e->sacc, 3 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    PSX_HIP(hipStreamSynchronize(e->stream));
+    for (int dd = 0; dd < 3; ++dd) {
+      const double L = e->dims[dd] * e->cell[dd];
+      const double dJ = accJ[dd] / L / (double)N;
+      const double dQ = accQ[dd] / L / (double)N / (double)N;
+      gL[dd] = e->dims[dd] > 1 ? dJ / Q + (gQ - gphi / Q) * dQ : 0.0;
+    }
+  }
+  PSX_HIP(hipStreamSynchronize(e->stream));
+  return PSX_OK;
+}
+
+extern "C" psx_status psx_scft_backward_host(psx_scft* e, const double* h_gA, const double* h_gB, double gQ,
+                                             double* h_gwA, double* h_gwB, double* h_gL) {
+  if (!e || !h_gwA || !h_gwB) return fail(PSX_ERR_INVALID, "null argument");
+  const size_t fb = (size_t)e->N * sizeof(double);
+  double* buf = nullptr;
+  PSX_TRY(dalloc((void**)&buf, 4 * fb));
+  struct Free { double*& p; ~Free() { dfree((void*&)p); } } fr{buf};
+  const long long N = e->N;
+  if (h_gA) PSX_HIP(hipMemcpy(buf, h_gA, fb, hipMemcpyHostToDevice));
+  if (h_gB) PSX_HIP(hipMemcpy(buf + N, h_gB, fb, hipMemcpyHostToDevice));
+  PSX_TRY(psx_scft_backward(e, h_gA ? buf : nullptr, h_gB ? buf + N : nullptr, gQ, buf + 2 * N, buf + 3 * N,
+                            h_gL));
+  PSX_HIP(hipMemcpy(h_gwA, buf + 2 * N, fb, hipMemcpyDeviceToHost));
+  PSX_HIP(hipMemcpy(h_gwB, buf + 3 * N, fb, hipMemcpyDeviceToHost));
   return PSX_OK;
 }
 

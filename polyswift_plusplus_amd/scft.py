@@ -338,6 +338,22 @@ class DiblockSCFT:
         check(self._L.psx_scft_box_stress(self._h, _ptr(out)))
         return out
 
+    def backward(self, g_phiA=None, g_phiB=None, g_Q: float = 0.0, box: bool = True):
+        """Reverse mode of (w_A, w_B, box) -> (phi_A, phi_B, Q) at the current
+        fields (after solveQ): for L = <g_phiA, phi_A> + <g_phiB, phi_B> +
+        g_Q Q, returns (dL/dw_A, dL/dw_B, dL/dL_xyz or None)."""
+        a = None if g_phiA is None else _f64(g_phiA).reshape(-1)
+        b = None if g_phiB is None else _f64(g_phiB).reshape(-1)
+        for x in (a, b):
+            if x is not None and x.size != self.N:
+                raise PsxError(2, "cotangent size differs from the grid")
+        gA, gB = np.empty(self.N), np.empty(self.N)
+        gL = np.zeros(3) if box else None
+        check(self._L.psx_scft_backward_host(self._h, None if a is None else _ptr(a),
+                                             None if b is None else _ptr(b), float(g_Q),
+                                             _ptr(gA), _ptr(gB), None if gL is None else _ptr(gL)))
+        return gA.reshape(self.shape), gB.reshape(self.shape), gL
+
     def setQuadrature(self, mode) -> None:
         """'reference' (Simpson 0.33333, default) or 'trapezoid' (split-step
         consistent: the discrete F is stationary at the fixed point)."""
