@@ -230,8 +230,9 @@ psx_status psx_scft_graph_info(const psx_scft* e, int* h_nblocks,
 psx_status psx_scft_get_block_slice_host(psx_scft* e, int block,
                                          int from_tail, int s, double* h_out);
 /* Switch the sweep precision (allocates the other slice set on first use).
- * The Anderson history is stored in the sweep's type, so a switch restarts
- * it (as psx_scft_reset_history).  Engines of a batch switch through
+ * The Anderson history is stored in the sweep's type: FP32 -> FP64 converts
+ * it exactly and the history carries on (call psx_scft_reset_history for a
+ * fresh one); FP64 -> FP32 restarts it.  Engines of a batch switch through
  * psx_batch_set_precision (PSX_ERR_STATE here). */
 psx_status psx_scft_set_precision(psx_scft* e, psx_prec prec);
 /* Contour steps of the A blocks and of the B blocks (for a diblock, the

@@ -881,6 +881,13 @@ k_mix(QuadArgs a) {
 __global__ void k_state_set_call(DevState* st) { st->call_iter = 0; }
 __global__ void k_state_reset_hist(DevState* st) { st->since_reset = 0; }
 
+// One Anderson ring slot from FP32 to FP64 values (exact), into tmp.
+__global__ void k_slot_f32_to_f64(const float* __restrict__ src, long long n, double* __restrict__ tmp) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    tmp[i] = (double)src[i];
+}
+
 }  // namespace psx
 
 namespace psx {

@@ -2458,15 +2458,44 @@ extern "C" psx_status psx_scft_set_quadrature(psx_scft* e, int mode) {
   return PSX_OK;
 }
 
+// FP32 ring slots (x and f) to FP64 in place, highest slot first: the FP64
+// copy of slot s lies above every FP32 slot below s, and slot 0 goes
+// through a temporary.
+static psx_status ring_to_f64(psx_scft* e) {
+  const long long n = 2 * e->N;
+  double* tmp = nullptr;
+  PSX_TRY(dalloc((void**)&tmp, (size_t)n * sizeof(double)));
+  psx_status st = PSX_OK;
+  for (void* ring : {e->ringx, e->ringf})
+    for (int s = e->ring_m1 - 1; s >= 0 && st == PSX_OK; --s) {
+      hipLaunchKernelGGL(k_slot_f32_to_f64, dim3(grid_1d(n)), dim3(kThreads), 0, e->stream,
+                         (const float*)ring + (size_t)s * n, n, tmp);
+      if (hipGetLastError() != hipSuccess ||
+          hipMemcpyAsync((double*)ring + (size_t)s * n, tmp, (size_t)n * sizeof(double),
+                         hipMemcpyDeviceToDevice, e->stream) != hipSuccess)
+        st = fail(PSX_ERR_HIP, "ring conversion failed");
+    }
+  if (hipStreamSynchronize(e->stream) != hipSuccess && st == PSX_OK) st = fail(PSX_ERR_HIP, "sync failed");
+  dfree((void*&)tmp);
+  return st;
+}
+
 static psx_status set_precision_impl(psx_scft* e, psx_prec prec) {
   if (e->prec == (int)prec) return PSX_OK;
   PSX_TRY(ensure_sweep(e, prec));
+  const int old = e->prec;
   e->prec = prec;
-  // the Anderson ring is kept in the sweep's type: its old entries are not
-  // readable in the new one, so the history starts again
+  // The Anderson ring is kept in the sweep's type.  FP32 -> FP64 converts
+  // its entries exactly, so the history (and the Gram matrix built from
+  // them) carries on; FP64 -> FP32 would round them, so the history starts
+  // again.
   if (e->ringx) {
-    hipLaunchKernelGGL(k_state_reset_hist, dim3(1), dim3(1), 0, e->stream, e->st);
-    PSX_HIP(hipGetLastError());
+    if (old == PSX_FP32 && prec == PSX_FP64) {
+      PSX_TRY(ring_to_f64(e));
+    } else {
+      hipLaunchKernelGGL(k_state_reset_hist, dim3(1), dim3(1), 0, e->stream, e->st);
+      PSX_HIP(hipGetLastError());
+    }
   }
   e->sw[prec].wfac_valid = false;
   e->swept = false;

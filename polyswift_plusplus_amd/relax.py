@@ -42,16 +42,21 @@ def tail_mixing(mix: Mixing, depth: int = 30) -> Mixing:
 
 def mixed_precision_solve(e: DiblockSCFT, tol: float = 1e-8, switch_tol: float = 2e-5,
                           max_iters: int = 5000, mixing: Mixing | None = None,
-                          tail: Mixing | None = None) -> dict:
+                          tail: Mixing | None = None, keep_history: bool = False) -> dict:
     """FP32 sweeps until the residual RMS falls below switch_tol (the FP32
-    noise floor is near 1e-6), then FP64 iterations to tol with a fresh
-    Anderson history (``tail``, by default ``tail_mixing(mixing)``).  Q, the
-    quadrature, the update and F are FP64 throughout."""
+    noise floor is near 1e-6), then FP64 iterations to tol.  The FP64 tail
+    starts a fresh Anderson history (``tail``, by default
+    ``tail_mixing(mixing)``), or with ``keep_history`` carries the FP32
+    phase's history on (converted exactly to FP64) with the same mixing.  Q,
+    the quadrature, the update and F are FP64 throughout."""
     mix = mixing or DEFAULT_MIXING
     e.setPrecision("fp32")
     h32 = e.iterate(max_iters, mix, tol=switch_tol)
     e.setPrecision("fp64")
-    e.resetHistory()
+    if keep_history:
+        tail = mix
+    else:
+        e.resetHistory()
     h64 = e.iterate(max_iters, tail or tail_mixing(mix), tol=tol)
     return dict(fp32_iters=h32["iters"], fp64_iters=h64["iters"], F=h64["F"][-1] if h64["iters"] else None,
                 err=h64["err"][-1] if h64["iters"] else None, h32=h32, h64=h64)

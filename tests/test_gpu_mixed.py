@@ -27,3 +27,27 @@ def test_mixed_protocol_reaches_fp64_free_energy():
     assert tail_mixing(mix).depth == 30 and tail_mixing(Mixing("steepest")).kind == "steepest"
     ref.close()
     e.close()
+
+
+def test_mixed_protocol_with_history_kept_across_the_switch():
+    """FP32 -> FP64 converts the Anderson ring exactly and the history
+    carries on: the protocol with the history kept reaches the FP64 free
+    energy too, along a different path than a fresh FP64 history."""
+    shape, cell = (32, 32, 32), (4.0 / 32,) * 3
+    mix = Mixing("anderson", lambda0=0.2, depth=20, beta=1.0, anderson_start=0.05)
+    wA, wB = fields.seeded_fields(shape, cell, sigma=0.1, seed=3, seed_kind="lamellar", amplitude=1.0)
+    ref = DiblockSCFT(shape, cell, 0.35, 0.01, 18.0)
+    ref.setFields(wA, wB)
+    h = ref.iterate(3000, mix, tol=1e-8)
+    out = {}
+    for keep in (True, False):
+        e = DiblockSCFT(shape, cell, 0.35, 0.01, 18.0, prec="fp32")
+        e.setFields(wA, wB)
+        r = mixed_precision_solve(e, tol=1e-8, max_iters=3000, mixing=mix, keep_history=keep,
+                                  tail=None if keep else mix)
+        print(f"keep={keep}: {r['fp32_iters']} FP32 + {r['fp64_iters']} FP64 iterations", flush=True)
+        assert r["err"] < 1e-8 and abs(r["F"] - h["F"][-1]) < 1e-8
+        out[keep] = r["h64"]["F"]
+        e.close()
+    assert out[True][0] != out[False][0]     # the kept history changed the first FP64 step
+    ref.close()
