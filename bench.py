@@ -86,6 +86,44 @@ def time_iters(e, mix, warmup, steps):
     return dt, h
 
 
+def config5_relax_leg(tol_stress=1e-7, tol_scft=1e-10, seed_kind="hex", checkpoint=20):
+    """Box relaxation at the differentiation workload's full size: 96^3, box
+    (7.2, 7.6, 8.0) Rg to start, fA = 0.32, chiN = 22, Ns = 200 (ds = 0.005),
+    FP64 sweeps with contour checkpoints every 20 steps, the trapezoid contour
+    rule (under which the adjoint stress is the exact derivative of F), to
+    |dF/dL| < 1e-7.  Seeded N(0, 0.1) fields plus an ordered seed: from the
+    Gaussian fields alone the iteration settles on the disordered phase,
+    whose stress vanishes at every box.  The fields are first converged with
+    the mixed-precision protocol, then every outer step re-converges them in
+    FP64 from the last ones."""
+    from polyswift_plusplus_amd import DiblockSCFT, Mixing, fields
+    from polyswift_plusplus_amd.relax import mixed_precision_solve, relax_box
+    shape, box = (96, 96, 96), (7.2, 7.6, 8.0)
+    cell = tuple(b / n for b, n in zip(box, shape))
+    mix = Mixing("anderson", lambda0=0.2, depth=20, beta=1.0, anderson_start=0.05)
+    e = DiblockSCFT(shape, cell, 0.32, 0.005, 22.0, prec="fp32", quadrature="trapezoid")
+    e.setFields(*fields.seeded_fields(shape, cell, sigma=0.1, seed=5, seed_kind=seed_kind, amplitude=1.0))
+    t0 = time.perf_counter()
+    r0 = mixed_precision_solve(e, tol=tol_scft, switch_tol=3e-5, max_iters=20000, mixing=mix, keep_history=True)
+    print(f"[bench] config5 relax: first solve {r0['fp32_iters']} FP32 + {r0['fp64_iters']} FP64 iterations, "
+          f"err {r0['err']:.2e} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    if checkpoint:
+        e.setCheckpointing(checkpoint)
+    t1 = time.perf_counter()
+    r = relax_box(e, tol_stress=tol_stress, tol_scft=tol_scft, max_outer=30, first_step=5.0, mixing=mix,
+                  max_iters=20000, log=lambda m: print("[bench] config5 relax: " + m, file=sys.stderr, flush=True))
+    dt = time.perf_counter() - t1
+    e.close()
+    hist = [dict(L=[float(x) for x in h["L"]], dFdL=[float(x) for x in h["dFdL"]], F=float(h["F"]),
+                 iters=h["iters"], scft_err=h["scft_err"]) for h in r["history"]]
+    return {"workload": "AB diblock 96^3, box (7.2, 7.6, 8.0) Rg to start, fA=0.32, chiN=22, Ns=200 "
+                        f"(ds=0.005), FP64, checkpoint every {checkpoint} steps, trapezoid contour rule, "
+                        f"N(0, 0.1) + {seed_kind} seed",
+            "converged": r["converged"], "reason": r["reason"], "L": [float(x) for x in r["L"]],
+            "max_abs_dFdL": float(np.max(np.abs(r["history"][-1]["dFdL"]))), "outer_steps": len(hist),
+            "scft_iterations": int(sum(h["iters"] for h in hist)), "seconds": dt, "history": hist}
+
+
 def solution_state(e):
     """(phiA, phiB, Q, F) of one evaluation at the engine's fields, in its
     current precision."""
@@ -335,6 +373,9 @@ def main():
     ap.add_argument("--config5", action="store_true",
                     help="only the differentiation workload (96^3, Ns = 200, FP64): iterations/s and "
                          "adjoint box stress time, with and without contour checkpointing")
+    ap.add_argument("--config5-relax", action="store_true",
+                    help="only the box relaxation of the differentiation workload (96^3, Ns = 200, FP64, "
+                         "checkpointed) to |dF/dL| < 1e-7; prints progress per outer step")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3 counters: the dual contour step of "
                          "both precisions (5 reps each) and one FP64 copy of known size")
@@ -355,6 +396,9 @@ def main():
         return
     if args.config5:
         print(json.dumps(config5_leg()))
+        return
+    if args.config5_relax:
+        print(json.dumps(config5_relax_leg()))
         return
     if args.profile:
         # Counter calibration: one k_to_f64<double> over N cells moves exactly

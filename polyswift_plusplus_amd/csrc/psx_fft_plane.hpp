@@ -48,7 +48,12 @@ struct PlaneShape {
 // Round trips along y, then z (second step: z, then y) of one x-plane per
 // block.  g.ntiles = planes per field (n_x), g.nfields = fields; PairIO as
 // for the line launches with the plane spectral tables above.
-template <typename T, int N1, int N2, int NW, bool REUSE>
+// DIRECT: the first y round trips load their column pairs straight from
+// global memory (no staging phase: each wave starts as soon as its own
+// columns arrive) and the last ones store theirs straight back (no final
+// barrier or staging pass).  A wave's column pairs are 2 CL elements of a
+// row, so this suits splits whose CL pairs span 64 bytes or more.
+template <typename T, int N1, int N2, int NW, bool REUSE, bool DIRECT = false>
 __global__ void __launch_bounds__(64 * NW)
 k_plane(LineGeom g, const PairIO* __restrict__ fio) {
   typedef typename Cx<T>::type C;
@@ -76,6 +81,26 @@ k_plane(LineGeom g, const PairIO* __restrict__ fio) {
   const C* __restrict__ tw = (const C*)g.tw;
   C* twl = (C*)(smem_raw + S::BUF);
   T* em = (T*)(twl + n);               // e_z, e_y of the first step, then of the second
+  C vd[DIRECT ? N1 : 1];
+  if (DIRECT) {
+    // the first y round trips' column pairs, times pre, all in flight
+    const T* __restrict__ src = (const T*)fd.in;
+    const T* __restrict__ pre = (const T*)fd.pre;
+#pragma unroll
+    for (int r = 0; r < N1; ++r) {
+      const long long ga = base + (long long)(t + N2 * r) * n + 2 * cl;
+      if (lv) vd[r] = *(const C*)(src + ga);
+      else { vd[r].x = (T)0; vd[r].y = (T)0; }
+    }
+    if (pre && lv) {
+#pragma unroll
+      for (int r = 0; r < N1; ++r) {
+        const C w = *(const C*)(pre + base + (long long)(t + N2 * r) * n + 2 * cl);
+        vd[r].x *= w.x;
+        vd[r].y *= w.y;
+      }
+    }
+  }
   for (int i = tid; i < n; i += 64 * NW) twl[i] = tw[i];
   for (int i = tid; i < 2 * n; i += 64 * NW) {
     em[i] = ((const T*)fd.emul)[i];
@@ -95,7 +120,18 @@ k_plane(LineGeom g, const PairIO* __restrict__ fio) {
     *col = (c - *row * CPR) * VW;
     return !CPART || c < NCH;
   };
-  {
+  if (DIRECT) {
+    // the first y round trips, each wave as soon as its own columns arrive:
+    // twiddles and factors from global memory (their LDS copies are not yet
+    // visible), then the plane into LDS
+    line_roundtrip<T, N1, N2, true>(vd, L, tw, tw, (const T*)fd.emul + n, t);
+    if (REUSE) __syncthreads();
+    else lds_sync<true>();
+    if (lv) {
+#pragma unroll
+      for (int r = 0; r < N1; ++r) *(C*)(P + (t + N2 * r) * RS + 2 * cl) = vd[r];
+    }
+  } else {
     const T* __restrict__ src = (const T*)fd.in;
     const T* __restrict__ pre = (const T*)fd.pre;
     V v[IT];
@@ -142,8 +178,10 @@ k_plane(LineGeom g, const PairIO* __restrict__ fio) {
   // step boundary sits between the two z round trips of a wave's own rows,
   // so post, the store of mid and pre2 happen in registers, and each wave's
   // w-factor loads and stores overlap the other waves' arithmetic.
-  ypass(em + n);
-  __syncthreads();
+  if (!DIRECT) {
+    ypass(em + n);
+    __syncthreads();
+  }
   {
     C v[N1];
 #pragma unroll
@@ -201,6 +239,28 @@ k_plane(LineGeom g, const PairIO* __restrict__ fio) {
     }
   }
   __syncthreads();
+  if (DIRECT) {
+    // the last y round trips, straight to global memory (times post2)
+    C v[N1];
+#pragma unroll
+    for (int r = 0; r < N1; ++r) {
+      if (lv) v[r] = *(const C*)(P + (t + N2 * r) * RS + 2 * cl);
+      else { v[r].x = (T)0; v[r].y = (T)0; }
+    }
+    if (REUSE) __syncthreads();
+    line_roundtrip<T, N1, N2, true>(v, L, tw, twl, em + 3 * n, t);
+    if (!lv) return;
+    const T* __restrict__ post2 = (const T*)fd.post2;
+    T* __restrict__ out = (T*)fd.out;
+#pragma unroll
+    for (int r = 0; r < N1; ++r) {
+      const long long ga = base + (long long)(t + N2 * r) * n + 2 * cl;
+      C o = v[r];
+      if (post2) { const C w = *(const C*)(post2 + ga); o.x *= w.x; o.y *= w.y; }
+      *(C*)(out + ga) = o;
+    }
+    return;
+  }
   ypass(em + 3 * n);
   __syncthreads();
   asm volatile("v_mov_b32 %0, %1" : "=v"(tidc) : "v"(tid));

@@ -344,20 +344,21 @@ psx_status launch_pair(const Axis& a, int prec, const PairIO* fio,
 // 3D grids with n_y = n_z = n of a register split below.
 bool plane_length(int n) { return n == 32 || n == 48 || n == 64 || n == 96 || n == 128; }
 
-template <typename T, int N1, int N2, int NW, bool REUSE>
+template <typename T, int N1, int N2, int NW, bool REUSE, bool DIRECT = false>
 psx_status plane_kernel(const Axis& az, int prec, const PairIO* fio, int nfields, int nplanes,
                         hipStream_t s, bool prepare) {
   typedef PlaneShape<T, N1, N2, NW, REUSE> S;
   static_assert(S::SMEM <= 160 * 1024, "plane kernel exceeds the LDS of a CU");
   if (prepare) {
-    PSX_HIP(hipFuncSetAttribute((const void*)k_plane<T, N1, N2, NW, REUSE>,
+    PSX_HIP(hipFuncSetAttribute((const void*)k_plane<T, N1, N2, NW, REUSE, DIRECT>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, S::SMEM));
     return PSX_OK;
   }
   LineGeom g = geom(az, prec, 0);
   g.ntiles = nplanes;
   g.nfields = nfields;
-  hipLaunchKernelGGL((k_plane<T, N1, N2, NW, REUSE>), dim3(nplanes * nfields), dim3(64 * NW), S::SMEM, s, g, fio);
+  hipLaunchKernelGGL((k_plane<T, N1, N2, NW, REUSE, DIRECT>), dim3(nplanes * nfields), dim3(64 * NW), S::SMEM, s,
+                     g, fio);
   return PSX_OK;
 }
 
@@ -415,13 +416,20 @@ psx_status launch_plane(const Axis& az, int prec, const PairIO* fio, int nfields
       if constexpr (F32) PSX_TRY((plane_kernel_fit<T, 8, 8, 4>(az, prec, fio, nfields, nplanes, s, prepare)));
       else PSX_TRY((plane_kernel<T, 8, 8, 4, false>(az, prec, fio, nfields, nplanes, s, prepare)));
       break;
+    // FP64 at 96 and 128: the first and last y round trips straight from and
+    // to global memory (DIRECT; a wave's column pairs are 128-byte row
+    // segments): 128^3 plane launches 19.7 -> 18.4 us per dual step, 96^3
+    // 13.9 -> 13.1 us.  FP32 rows of 32 or 64 bytes lost (128^3 10.4 ->
+    // 11.9 us; 96^3 8.9 -> 9.7 us; 64^3 4.4 -> 5.0 us), and so did the
+    // (16, 8) FP32 split at 128 with them (13.2 us aliased, 12.7 us with the
+    // transposes behind the plane).
     case 96:
       if constexpr (F32) PSX_TRY((plane_kernel_fit<T, 12, 8, 6>(az, prec, fio, nfields, nplanes, s, prepare)));
-      else PSX_TRY((plane_kernel<T, 12, 8, 6, false>(az, prec, fio, nfields, nplanes, s, prepare)));
+      else PSX_TRY((plane_kernel<T, 12, 8, 6, false, true>(az, prec, fio, nfields, nplanes, s, prepare)));
       break;
     case 128:
       if constexpr (F32) PSX_TRY((plane_kernel<T, 8, 16, 16, false>(az, prec, fio, nfields, nplanes, s, prepare)));
-      else PSX_TRY((plane_kernel<T, 16, 8, 8, true>(az, prec, fio, nfields, nplanes, s, prepare)));
+      else PSX_TRY((plane_kernel<T, 16, 8, 8, true, true>(az, prec, fio, nfields, nplanes, s, prepare)));
       break;
     default: return fail(PSX_ERR_INVALID, "no plane kernel for n = " + std::to_string(az.n));
   }

@@ -55,3 +55,23 @@ def test_config5_checkpointed_sweep_and_stress():
     finally:
         e.close()
         c.close()
+
+
+def test_config5_box_relaxation_to_zero_stress():
+    """The box relaxation at full size (checkpoints every 20 contour steps,
+    trapezoid contour rule) to |dF/dL| < 1e-7 from seeded N(0, 0.1) fields
+    plus the hexagonal-cylinder seed.  Size-independent check: the cylinders
+    run along z, so dF/dL_z vanishes and L_z stays, and the relaxed x-y cell
+    is the regular hexagonal one for the seed's wavevectors (2, 0), (-1, +-2):
+    |k| equal for all three gives L_y / L_x = 2 / sqrt(3)."""
+    import bench
+    print("config 5 box relaxation", flush=True)
+    r = bench.config5_relax_leg()
+    h = r["history"]
+    print(f"  {r['outer_steps']} outer steps, L = {r['L']}, max|dF/dL| = {r['max_abs_dFdL']:.2e}, "
+          f"F {h[0]['F']:.10f} -> {h[-1]['F']:.10f}", flush=True)
+    assert r["converged"] and r["max_abs_dFdL"] < 1e-7
+    assert all(step["scft_err"] < 1e-10 for step in h)
+    assert abs(r["L"][2] - 8.0) < 1e-12
+    assert abs(r["L"][1] / r["L"][0] - 2.0 / np.sqrt(3.0)) < 1e-4
+    assert h[-1]["F"] < h[0]["F"] - 1e-3
