@@ -124,6 +124,27 @@ def config5_relax_leg(tol_stress=1e-7, tol_scft=1e-10, seed_kind="hex", checkpoi
             "scft_iterations": int(sum(h["iters"] for h in hist)), "seconds": dt, "history": hist}
 
 
+def cache_probe(reps=10, sizes_mb=(8, 16, 32, 64, 128, 256, 512, 1024, 2048)):
+    """Back-to-back read-only sweeps and copies over buffers of each size
+    (psx_probe_stream): GB/s of the bytes each sweep touches."""
+    import ctypes
+    from polyswift_plusplus_amd import _abi
+    L = _abi.load()
+    out = []
+    for mb in sizes_mb:
+        row = {"MiB": mb}
+        for mode, name in ((0, "read"), (1, "copy")):
+            ms = ctypes.c_double()
+            _abi.check(L.psx_probe_stream(mb << 20, reps, mode, ctypes.byref(ms)))
+            moved = (mb << 20) * (1 if mode == 0 else 2)
+            row[name + "_us"] = ms.value * 1e3
+            row[name + "_GBs"] = moved / (ms.value * 1e-3) / 1e9
+        out.append(row)
+        print(f"[bench] probe {mb} MiB: read {row['read_GBs']:.0f} GB/s, copy {row['copy_GBs']:.0f} GB/s",
+              file=sys.stderr, flush=True)
+    return out
+
+
 def solution_state(e):
     """(phiA, phiB, Q, F) of one evaluation at the engine's fields, in its
     current precision."""
@@ -376,6 +397,9 @@ def main():
     ap.add_argument("--config5-relax", action="store_true",
                     help="only the box relaxation of the differentiation workload (96^3, Ns = 200, FP64, "
                          "checkpointed) to |dF/dL| < 1e-7; prints progress per outer step")
+    ap.add_argument("--cache-probe", action="store_true",
+                    help="only the memory probe: read-only sweeps and copies over 8 MiB .. 2 GiB buffers "
+                         "(--steps sweeps each), to tell cache-resident rates from HBM's")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3 counters: the dual contour step of "
                          "both precisions (5 reps each) and one FP64 copy of known size")
@@ -399,6 +423,9 @@ def main():
         return
     if args.config5_relax:
         print(json.dumps(config5_relax_leg()))
+        return
+    if args.cache_probe:
+        print(json.dumps(cache_probe(reps=args.steps)))
         return
     if args.profile:
         # Counter calibration: one k_to_f64<double> over N cells moves exactly

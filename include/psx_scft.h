@@ -133,6 +133,13 @@ psx_status psx_buffer_create(size_t bytes, void** d_out);
 psx_status psx_buffer_destroy(void* d);
 psx_status psx_buffer_upload(void* d_dst, const void* h_src, size_t bytes);
 psx_status psx_buffer_download(void* h_dst, const void* d_src, size_t bytes);
+/* Memory probe (no reference counterpart): allocates a buffer of `bytes`,
+ * then times `reps` back-to-back sweeps over it after one warm sweep (mode
+ * 0: read-only, 16-byte loads; mode 1: copy into a second buffer of the
+ * same size) with hipEvents on the null stream; *h_ms = average ms per
+ * sweep.  Repeated sweeps over buffers that fit the L2 or the Infinity
+ * Cache show those levels' rates; larger ones, HBM's. */
+psx_status psx_probe_stream(size_t bytes, int reps, int mode, double* h_ms);
 
 /* ===================================================================== */
 /* SCFT engine for a bulk AB diblock melt: the block propagators          */

@@ -81,6 +81,7 @@ EXPORTS = {
     "psx_buffer_destroy": (ctypes.c_int, [_vp]),
     "psx_buffer_upload": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
     "psx_buffer_download": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
+    "psx_probe_stream": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_int, ctypes.c_int, _dp]),
     "psx_scft_create": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Diblock), ctypes.c_int,
                                        _vp, ctypes.POINTER(_vp)]),
     "psx_scft_create_blocks": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.c_int, ctypes.POINTER(BlockDesc),

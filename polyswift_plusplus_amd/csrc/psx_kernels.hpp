@@ -881,6 +881,27 @@ k_mix(QuadArgs a) {
 __global__ void k_state_set_call(DevState* st) { st->call_iter = 0; }
 __global__ void k_state_reset_hist(DevState* st) { st->since_reset = 0; }
 
+// Memory probes (psx_probe_stream): a read-only sweep (one sum per block
+// kept, so the loads stay) and a copy, 16 bytes per access.
+__global__ void __launch_bounds__(kThreads)
+k_probe_read(const double2* __restrict__ x, long long n, double* __restrict__ part) {
+  double v[1] = {0.0};
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double2 a = x[i];
+    v[0] += a.x + a.y;
+  }
+  block_sum<1>(v);
+  if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_probe_copy(const double2* __restrict__ x, long long n, double2* __restrict__ y) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    y[i] = x[i];
+}
+
 // One Anderson ring slot from FP32 to FP64 values (exact), into tmp.
 __global__ void k_slot_f32_to_f64(const float* __restrict__ src, long long n, double* __restrict__ tmp) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;

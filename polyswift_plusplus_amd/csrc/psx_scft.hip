@@ -999,6 +999,41 @@ extern "C" psx_status psx_fft_synchronize(psx_fft* p) {
   return PSX_OK;
 }
 
+extern "C" psx_status psx_probe_stream(size_t bytes, int reps, int mode, double* ms) {
+  if (!ms || reps < 1 || (mode != 0 && mode != 1) || bytes < 16) return fail(PSX_ERR_INVALID, "bad argument");
+  const long long n = (long long)(bytes / 16);
+  void *a = nullptr, *b = nullptr, *part = nullptr;
+  const int blocks = 4096;
+  psx_status st = dalloc(&a, (size_t)n * 16);
+  if (st == PSX_OK && mode == 1) st = dalloc(&b, (size_t)n * 16);
+  if (st == PSX_OK) st = dalloc(&part, blocks * sizeof(double));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (st == PSX_OK && (hipMemset(a, 0, (size_t)n * 16) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+                       hipEventCreate(&e1) != hipSuccess))
+    st = fail(PSX_ERR_HIP, "probe setup failed");
+  auto launch = [&]() {
+    if (mode == 0)
+      hipLaunchKernelGGL(k_probe_read, dim3(blocks), dim3(kThreads), 0, nullptr, (const double2*)a, n, (double*)part);
+    else
+      hipLaunchKernelGGL(k_probe_copy, dim3(blocks), dim3(kThreads), 0, nullptr, (const double2*)a, n, (double2*)b);
+  };
+  if (st == PSX_OK) {
+    launch();   // warm
+    (void)hipEventRecord(e0, nullptr);
+    for (int r = 0; r < reps; ++r) launch();
+    (void)hipEventRecord(e1, nullptr);
+    float t = 0.0f;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess ||
+        hipGetLastError() != hipSuccess)
+      st = fail(PSX_ERR_HIP, "probe failed");
+    *ms = (double)t / reps;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  dfree(a); dfree(b); dfree(part);
+  return st;
+}
+
 extern "C" psx_status psx_buffer_create(size_t bytes, void** d_out) {
   if (!d_out) return fail(PSX_ERR_INVALID, "null argument");
   return dalloc(d_out, bytes);
