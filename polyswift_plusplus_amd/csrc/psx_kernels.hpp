@@ -437,6 +437,23 @@ __device__ __forceinline__ double blend_scale(const QuadArgs& a, int b) {
 template <typename T, int V>
 struct alignas(sizeof(T) * V) CellVec { T v[V]; };
 
+// A slice vector read once by the contour sums: a non-temporal load (the
+// slices were written long before and are not read again this iteration;
+// 128^3 FP32 iteration 2.45 -> 2.38 ms, FP64 unchanged).
+template <typename T, int V>
+__device__ __forceinline__ CellVec<T, V> slice_load(const CellVec<T, V>* p) {
+  CellVec<T, V> r;
+  if constexpr (V == 1) {
+    r.v[0] = __builtin_nontemporal_load((const T*)p);
+  } else {
+    typedef T NV __attribute__((ext_vector_type(V)));
+    const NV x = __builtin_nontemporal_load((const NV*)p);
+#pragma unroll
+    for (int i = 0; i < V; ++i) r.v[i] = x[i];
+  }
+  return r;
+}
+
 // Everything after the densities of one cell: the Flory dH, then the
 // steepest-descent update (w, p_raw, w-factors) or the Anderson ring entry,
 // and the partial sums.
@@ -520,8 +537,8 @@ k_quad_update(QuadArgs a) {
         constexpr int UNR = V == 1 ? 8 : 4;
 #pragma unroll UNR
         for (int t = 0; t < M; ++t) {
-          const VT h = H[(long long)t * NV];
-          const VT q = Tq[(long long)(M - 1 - t) * NV];
+          const VT h = slice_load(H + (long long)t * NV);
+          const VT q = slice_load(Tq + (long long)(M - 1 - t) * NV);
           const double ct = c[t];
 #pragma unroll
           for (int v = 0; v < V; ++v) s[v] += ct * ((double)h.v[v] * (double)q.v[v]);
@@ -600,8 +617,8 @@ k_quad_split(QuadArgs a) {
         for (int v = 0; v < V; ++v) sm[v] = 0.0;
 #pragma unroll 4
         for (int t = w; t < M; t += S) {
-          const VT h = H[(long long)t * NV];
-          const VT q = Tq[(long long)(M - 1 - t) * NV];
+          const VT h = slice_load(H + (long long)t * NV);
+          const VT q = slice_load(Tq + (long long)(M - 1 - t) * NV);
           const double ct = c[t];
 #pragma unroll
           for (int v = 0; v < V; ++v) sm[v] += ct * ((double)h.v[v] * (double)q.v[v]);
