@@ -193,6 +193,57 @@ __device__ __forceinline__ void reg_roundtrip(C* v, C* L, const C* __restrict__ 
   dft_reg<N1, +1, T>(v, tw, N2);
 }
 
+// Lane-pair splits (N1 % N2 != 0) end their forward transform with a radix 2
+// across a lane pair and start the inverse with one.  Between the two the
+// spectral multiply is diagonal, so the three steps are one 2x2 operator per
+// pair of spectral points k = k1 + N1 j and k + n/2 (W = W_N2^j, K the row of
+// spectral factors, a = K(k) + K(k + n/2), b = K(k) - K(k + n/2)):
+//   lane h = 0 (own E_0, partner E_1):  E_0 <- a E_0 + b W E_1
+//   lane h = 1 (own E_1, partner E_0):  E_1 <- a E_1 + b conj(W) E_0
+// Entry 2k + h of a pair table holds a and c = b W (h = 0) or b conj(W)
+// (h = 1): one exchange and 6 multiply-adds per point instead of two
+// exchanges, two twiddle products, the selects of the lane roles and the
+// multiply.  The FP32 plane kernel at n = 128 went from 2266 to 1709 VALU
+// instructions (dual contour step 18.2 -> 17.9 us at 128^3, 18.3 -> 16.7 us
+// at 96^3).
+template <typename T>
+struct __align__(4 * sizeof(T)) PairCoef { T a, cr, ci, pad; };
+
+template <int N1, int N2>
+constexpr bool pair_split() { return N1 % N2 != 0; }
+
+// FP32 lane-pair kernels keep pair tables in LDS beside the twiddles.
+template <typename T, int N1, int N2>
+constexpr bool fused_pairs() { return sizeof(T) == 4 && pair_split<N1, N2>(); }
+
+// LDS bytes of a kernel's table copies: the twiddles (n complex) and `rows`
+// rows of spectral factors, as pair tables where the kernel fuses them.
+template <typename T, int N1, int N2>
+constexpr int table_bytes(int rows) {
+  return N1 * N2 * 2 * (int)sizeof(T) +
+         rows * N1 * N2 * (fused_pairs<T, N1, N2>() ? (int)sizeof(PairCoef<T>) : (int)sizeof(T));
+}
+
+// Entries i0, i0 + step, ... of the pair table of the spectral row K
+// (n values); tw is the size-n twiddle table.
+template <typename T, int N1, int N2, typename C>
+__device__ __forceinline__ void pair_coef_fill(PairCoef<T>* pc, const T* K, const C* __restrict__ tw,
+                                               int i0, int step) {
+  constexpr int n = N1 * N2;
+  for (int i = i0; i < n; i += step) {
+    const int k = i >> 1, j = k / N1;
+    const T k0 = K[k], k1 = K[k + n / 2];
+    const C w = tw[j * N1];           // W_N2^j = W_n^{N1 j}
+    const T b = k0 - k1;
+    PairCoef<T> e;
+    e.a = k0 + k1;
+    e.cr = b * w.x;
+    e.ci = (i & 1) ? -(b * w.y) : b * w.y;
+    e.pad = (T)0;
+    pc[i] = e;
+  }
+}
+
 // Exchange with the partner lane (lane ^ 1) through a DPP quad permutation
 // [1, 0, 3, 2] (VALU moves, no LDS).
 __device__ __forceinline__ float pair_swap(float x) {
@@ -204,6 +255,19 @@ __device__ __forceinline__ double pair_swap(double x) {
   return __hiloint2double(hi, lo);
 }
 
+// The fused radix 2 / multiply / inverse radix 2 of one point (pair table
+// entry q, see PairCoef).
+template <typename T, typename C>
+__device__ __forceinline__ void pair_apply(C& y, const PairCoef<T>& q) {
+  C o;
+  o.x = pair_swap(y.x);
+  o.y = pair_swap(y.y);
+  C r;
+  r.x = q.a * y.x + q.cr * o.x - q.ci * o.y;
+  r.y = q.a * y.y + q.cr * o.y + q.ci * o.x;
+  y = r;
+}
+
 // Lane-pair round trip for n = 2 N1^2 on N2 = 2 N1 lanes, N1 points per lane
 // (lane t owns x[t + N2 r]): half the registers per lane and twice the lines
 // in flight of the N1 = N2^2 / n split.  Forward, with k = k1 + N1 k2g:
@@ -213,10 +277,13 @@ __device__ __forceinline__ double pair_swap(double x) {
 //   in-register DFT_N1 over m: E_h[k2] = sum_m W_N1^{m k2} z[h + 2m];
 //   radix 2 across the pair: lane h keeps X[k1 + N1 (k2 + N1 h)] =
 //   E_0[k2] + (-1)^h W_N2^{k2} E_1[k2], k2 < N1.
-// The spectral multiply follows, then the same steps inverted.
-template <typename T, int N1, bool WS, typename C>
+// The spectral multiply follows, then the same steps inverted.  FUSED: the
+// radix 2, the multiply and the inverse radix 2 through the pair table pc
+// (entry t + 2 N1 k2 for lane t and point k2).
+template <typename T, int N1, bool WS, bool FUSED, typename C>
 __device__ __forceinline__ void pair_roundtrip(C* v, C* L, const C* __restrict__ tw,
-                                               const C* twl, const T* emul, int t) {
+                                               const C* twl, const T* emul, const PairCoef<T>* pc,
+                                               int t) {
   constexpr int N2 = 2 * N1;
   dft_reg<N1, -1, T>(v, tw, N2);
 #pragma unroll
@@ -230,38 +297,43 @@ __device__ __forceinline__ void pair_roundtrip(C* v, C* L, const C* __restrict__
 #pragma unroll
   for (int m = 0; m < N1; ++m) v[m] = L[k1 * N2 + ((h + 2 * m) ^ sw)];
   dft_reg<N1, -1, T>(v, tw, N2);
-  // radix 2 across the lane pair
+  if constexpr (FUSED) {
 #pragma unroll
-  for (int k2 = 0; k2 < N1; ++k2) {
-    C o;
-    o.x = pair_swap(v[k2].x);
-    o.y = pair_swap(v[k2].y);
-    const C e0 = h ? o : v[k2];
-    const C e1 = h ? v[k2] : o;
-    const C w1 = tw_mul<-1, T>(e1, k2, 2 * N1, tw, k2 * N1);
-    C x;
-    if (h) { x.x = e0.x - w1.x; x.y = e0.y - w1.y; }
-    else { x.x = e0.x + w1.x; x.y = e0.y + w1.y; }
-    const T m = emul[k1 + N1 * (k2 + N1 * h)];
-    x.x *= m; x.y *= m;
-    v[k2] = x;
-  }
-  // inverse radix 2: E_0 = Y[k2] + Y[k2 + N1], E_1 = (Y[k2] - Y[k2 + N1]) W_N2^{-k2}
+    for (int k2 = 0; k2 < N1; ++k2) pair_apply<T>(v[k2], pc[t + 2 * N1 * k2]);
+  } else {
+    // radix 2 across the lane pair
 #pragma unroll
-  for (int k2 = 0; k2 < N1; ++k2) {
-    C o;
-    o.x = pair_swap(v[k2].x);
-    o.y = pair_swap(v[k2].y);
-    const C lo = h ? o : v[k2];
-    const C hi = h ? v[k2] : o;
-    C e;
-    if (h) {
-      C d; d.x = lo.x - hi.x; d.y = lo.y - hi.y;
-      e = tw_mul<+1, T>(d, k2, 2 * N1, tw, k2 * N1);
-    } else {
-      e.x = lo.x + hi.x; e.y = lo.y + hi.y;
+    for (int k2 = 0; k2 < N1; ++k2) {
+      C o;
+      o.x = pair_swap(v[k2].x);
+      o.y = pair_swap(v[k2].y);
+      const C e0 = h ? o : v[k2];
+      const C e1 = h ? v[k2] : o;
+      const C w1 = tw_mul<-1, T>(e1, k2, 2 * N1, tw, k2 * N1);
+      C x;
+      if (h) { x.x = e0.x - w1.x; x.y = e0.y - w1.y; }
+      else { x.x = e0.x + w1.x; x.y = e0.y + w1.y; }
+      const T m = emul[k1 + N1 * (k2 + N1 * h)];
+      x.x *= m; x.y *= m;
+      v[k2] = x;
     }
-    v[k2] = e;
+    // inverse radix 2: E_0 = Y[k2] + Y[k2 + N1], E_1 = (Y[k2] - Y[k2 + N1]) W_N2^{-k2}
+#pragma unroll
+    for (int k2 = 0; k2 < N1; ++k2) {
+      C o;
+      o.x = pair_swap(v[k2].x);
+      o.y = pair_swap(v[k2].y);
+      const C lo = h ? o : v[k2];
+      const C hi = h ? v[k2] : o;
+      C e;
+      if (h) {
+        C d; d.x = lo.x - hi.x; d.y = lo.y - hi.y;
+        e = tw_mul<+1, T>(d, k2, 2 * N1, tw, k2 * N1);
+      } else {
+        e.x = lo.x + hi.x; e.y = lo.y + hi.y;
+      }
+      v[k2] = e;
+    }
   }
   dft_reg<N1, +1, T>(v, tw, N2);
   lds_sync<WS>();
@@ -281,10 +353,12 @@ __device__ __forceinline__ void pair_roundtrip(C* v, C* L, const C* __restrict__
 // each; per column an in-register DFT_H over m and a radix 2 across the
 // pair, as in pair_roundtrip (its P = 1 case).  Registers: column u holds
 // v[u H .. u H + H - 1].  With N1 = 12, N2 = 8 a line of 96 takes 8 lanes
-// of 12 points: twice the lines per wave of the 24 x 4 split.
-template <typename T, int N1, int N2, bool WS, typename C>
+// of 12 points: twice the lines per wave of the 24 x 4 split.  FUSED as in
+// pair_roundtrip (entry t + 2 H u + 2 N1 j).
+template <typename T, int N1, int N2, bool WS, bool FUSED, typename C>
 __device__ __forceinline__ void pairs_roundtrip(C* v, C* L, const C* __restrict__ tw,
-                                                const C* twl, const T* emul, int t) {
+                                                const C* twl, const T* emul, const PairCoef<T>* pc,
+                                                int t) {
   constexpr int n = N1 * N2;
   constexpr int H = N2 / 2;
   constexpr int P = N1 / H;
@@ -312,6 +386,10 @@ __device__ __forceinline__ void pairs_roundtrip(C* v, C* L, const C* __restrict_
     const int k1 = k + H * u;
 #pragma unroll
     for (int j = 0; j < H; ++j) {
+      if constexpr (FUSED) {
+        pair_apply<T>(v[u * H + j], pc[t + 2 * H * u + 2 * N1 * j]);
+        continue;
+      }
       C& y = v[u * H + j];
       C o;
       o.x = pair_swap(y.x);
@@ -358,12 +436,15 @@ __device__ __forceinline__ void pairs_roundtrip(C* v, C* L, const C* __restrict_
 // WS: the transpose buffer L is private to the wave, so its write/read
 // hand-overs need only wave-level ordering (multi-wave blocks whose callers
 // separate the shared uses of that memory with block barriers).
-template <typename T, int N1, int N2, bool WS = false, typename C>
+// FUSED (lane-pair splits only): the pair table pc replaces emul.
+template <typename T, int N1, int N2, bool WS = false, bool FUSED = false, typename C>
 __device__ __forceinline__ void line_roundtrip(C* v, C* L, const C* __restrict__ tw,
-                                               const C* twl, const T* emul, int t) {
-  if constexpr (N2 == 2 * N1) pair_roundtrip<T, N1, WS>(v, L, tw, twl, emul, t);
+                                               const C* twl, const T* emul, int t,
+                                               const PairCoef<T>* pc = nullptr) {
+  static_assert(!FUSED || pair_split<N1, N2>(), "pair tables belong to lane-pair splits");
+  if constexpr (N2 == 2 * N1) pair_roundtrip<T, N1, WS, FUSED>(v, L, tw, twl, emul, pc, t);
   else if constexpr (N1 % N2 == 0) reg_roundtrip<T, N1, N2, WS>(v, L, tw, twl, emul, t);
-  else pairs_roundtrip<T, N1, N2, WS>(v, L, tw, twl, emul, t);
+  else pairs_roundtrip<T, N1, N2, WS, FUSED>(v, L, tw, twl, emul, pc, t);
 }
 
 // Round trip(s) along one axis for 2*64/N2 real lines per wave:
@@ -420,14 +501,23 @@ k_line_reg(LineGeom g, const PairIO* __restrict__ fio) {
   const C* twl = tw;
   const T* e1 = (const T*)fd.emul;
   const T* e2 = (const T*)fd.emul2;
+  constexpr bool FUSE = fused_pairs<T, N1, N2>();
+  PairCoef<T>* pc1 = (PairCoef<T>*)(smem_raw + BUF + n * sizeof(C));
+  PairCoef<T>* pc2 = pc1 + n;
   if (TLDS) {
     C* tws = (C*)(smem_raw + BUF);
     T* e1s = (T*)(tws + n);
     T* e2s = e1s + n;
     for (int i = lane; i < n; i += 64) {
       tws[i] = tw[i];
-      e1s[i] = e1[i];
-      if (fused) e2s[i] = e2[i];
+      if (!FUSE) {
+        e1s[i] = e1[i];
+        if (fused) e2s[i] = e2[i];
+      }
+    }
+    if (FUSE) {
+      pair_coef_fill<T, N1, N2>(pc1, e1, tw, lane, 64);
+      if (fused) pair_coef_fill<T, N1, N2>(pc2, e2, tw, lane, 64);
     }
     __syncthreads();
     twl = tws; e1 = e1s; e2 = e2s;
@@ -489,7 +579,7 @@ k_line_reg(LineGeom g, const PairIO* __restrict__ fio) {
     }
   }
 
-  line_roundtrip<T, N1, N2>(v, L, tw, twl, e1, t);
+  line_roundtrip<T, N1, N2, false, FUSE>(v, L, tw, twl, e1, t, pc1);
 
   // ---- store: the step result (post) to mid and on through the second
   // round trip, or straight to out
@@ -561,7 +651,7 @@ k_line_reg(LineGeom g, const PairIO* __restrict__ fio) {
       }
     }
     if (last) break;
-    line_roundtrip<T, N1, N2>(v, L, tw, twl, e2, t);
+    line_roundtrip<T, N1, N2, false, FUSE>(v, L, tw, twl, e2, t, pc2);
   }
 }
 
@@ -604,17 +694,26 @@ k_line_tile(LineGeom g, const PairIO* __restrict__ fio) {
   const int j0 = wave * RL + 2 * p;               // this lane's real-line pair in the tile
 
   constexpr bool TLDS = sizeof(T) == 4;
+  constexpr bool FUSE = fused_pairs<T, N1, N2>();
   const C* twl = tw;
   const T* e1 = (const T*)fd.emul;
   const T* e2 = (const T*)fd.emul2;
+  PairCoef<T>* pc1 = (PairCoef<T>*)(smem_raw + BUF + n * sizeof(C));
+  PairCoef<T>* pc2 = pc1 + n;
   if (TLDS) {
     C* tws = (C*)(smem_raw + BUF);
     T* e1s = (T*)(tws + n);
     T* e2s = e1s + n;
     for (int i = tid; i < n; i += 64 * W) {
       tws[i] = tw[i];
-      e1s[i] = e1[i];
-      if (fused) e2s[i] = e2[i];
+      if (!FUSE) {
+        e1s[i] = e1[i];
+        if (fused) e2s[i] = e2[i];
+      }
+    }
+    if (FUSE) {
+      pair_coef_fill<T, N1, N2>(pc1, e1, tw, tid, 64 * W);
+      if (fused) pair_coef_fill<T, N1, N2>(pc2, e2, tw, tid, 64 * W);
     }
     twl = tws; e1 = e1s; e2 = e2s;
   }
@@ -641,7 +740,7 @@ k_line_tile(LineGeom g, const PairIO* __restrict__ fio) {
     v[r].y = S[k * LS + j0 + 1];
   }
   __syncthreads();
-  line_roundtrip<T, N1, N2, true>(v, L, tw, twl, e1, t);
+  line_roundtrip<T, N1, N2, true, FUSE>(v, L, tw, twl, e1, t, pc1);
 
   for (int stage = 0; stage < 2; ++stage) {
     const bool last = stage == 1 || !fused;
@@ -677,7 +776,7 @@ k_line_tile(LineGeom g, const PairIO* __restrict__ fio) {
       v[r].y = S[k * LS + j0 + 1];
     }
     __syncthreads();
-    line_roundtrip<T, N1, N2, true>(v, L, tw, twl, e2, t);
+    line_roundtrip<T, N1, N2, true, FUSE>(v, L, tw, twl, e2, t, pc2);
   }
 }
 

@@ -202,7 +202,8 @@ void launch_reg(const Axis& a, int prec, const PairIO* fio, int nfields,
   const size_t tbuf = (size_t)TBuf<N1, N2>::wave * 2 * sizeof(T);
   const size_t sbuf = (size_t)RL * (N1 * N2 + N2 / 2) * sizeof(T);
   // FP32: + LDS copies of the twiddles (n complex) and two spectral multipliers
-  const size_t smem = (tbuf > sbuf ? tbuf : sbuf) + (sizeof(T) == 4 ? (size_t)N1 * N2 * 4 * sizeof(T) : 0);
+  // (pair tables for lane-pair splits)
+  const size_t smem = (tbuf > sbuf ? tbuf : sbuf) + (sizeof(T) == 4 ? (size_t)table_bytes<T, N1, N2>(2) : 0);
   dim3 grid(g.ntiles * nfields);
   if (a.contig)
     hipLaunchKernelGGL((k_line_reg<T, N1, N2, true, false>), grid, dim3(64), smem, s, g, fio);
@@ -229,7 +230,7 @@ void launch_tile(const Axis& a, int prec, const PairIO* fio, int nfields, hipStr
   const size_t sb = (size_t)n * (LT + 2) * sizeof(T);
   const size_t tb = (size_t)W * TBuf<N1, N2>::wave * 2 * sizeof(T);
   const size_t buf = ((sb > tb ? sb : tb) + 15) / 16 * 16;
-  const size_t smem = buf + (sizeof(T) == 4 ? (size_t)n * 4 * sizeof(T) : 0);
+  const size_t smem = buf + (sizeof(T) == 4 ? (size_t)table_bytes<T, N1, N2>(2) : 0);
   hipLaunchKernelGGL((k_line_tile<T, N1, N2, W>), dim3(g.ntiles * nfields), dim3(64 * W), smem, s, g, fio);
 }
 
