@@ -19,8 +19,9 @@
 
 namespace psx {
 
-// Spectral factors of a plane launch: emul (and emul2) point at 2n values,
-// the z factors followed by the y factors (each includes 1/n).
+// Spectral factors of a plane launch: emul (and emul2) point at 4n values,
+// the z factors and the y factors (each includes 1/n), then their (a, b)
+// rows for lane-pair splits (PairAB).
 //
 // NW waves of 64 / N2 complex lines each cover the plane's n / 2 complex
 // lines; where they do not divide evenly (n = 48: 2 waves of 16 for 24
@@ -40,8 +41,7 @@ struct PlaneShape {
   static constexpr int PB = n * RS * (int)sizeof(T);       // plane bytes
   static constexpr int TB = NW * TBuf<N1, N2>::wave * 2 * (int)sizeof(T);   // transpose buffers
   static constexpr int BUF = ((REUSE ? (PB > TB ? PB : TB) : PB + TB) + 15) / 16 * 16;
-  // + twiddles (n complex) + up to four n-point spectral factor rows (pair
-  // tables for FP32 lane-pair splits)
+  // + twiddles (n complex) + up to four n-point spectral rows
   static constexpr int SMEM = BUF + table_bytes<T, N1, N2>(4);
   static_assert(2 * NW * CL >= n && 2 * (NW - 1) * CL < n, "the waves must cover the plane's complex lines");
 };
@@ -82,9 +82,12 @@ k_plane(LineGeom g, const PairIO* __restrict__ fio) {
   const C* __restrict__ tw = (const C*)g.tw;
   C* twl = (C*)(smem_raw + S::BUF);
   T* em = (T*)(twl + n);               // e_z, e_y of the first step, then of the second
-  constexpr bool FUSE = fused_pairs<T, N1, N2>();
-  static_assert(!(FUSE && DIRECT), "the direct round trips read global spectral factors");
-  PairCoef<T>* pc = (PairCoef<T>*)(twl + n);   // FUSE: the same rows as pair tables
+  // global row r of the round trips (0, 1: z, y of the first step; 2, 3: of
+  // the second): the factors, or for lane-pair splits their (a, b) rows
+  auto grow = [&](int r) -> const T* {
+    const T* b = (const T*)(r < 2 ? fd.emul : fd.emul2);
+    return b + (pair_split<N1, N2>() ? 2 * n : 0) + (r & 1) * n;
+  };
   C vd[DIRECT ? N1 : 1];
   if (DIRECT) {
     // the first y round trips' column pairs, times pre, all in flight
@@ -106,18 +109,11 @@ k_plane(LineGeom g, const PairIO* __restrict__ fio) {
     }
   }
   for (int i = tid; i < n; i += 64 * NW) twl[i] = tw[i];
-  if (FUSE) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      if (r >= 2 && !fused) break;
-      const T* K = (const T*)(r < 2 ? fd.emul : fd.emul2) + (r & 1) * n;
-      pair_coef_fill<T, N1, N2>(pc + r * n, K, tw, tid, 64 * NW);
-    }
-  } else {
-    for (int i = tid; i < 2 * n; i += 64 * NW) {
-      em[i] = ((const T*)fd.emul)[i];
-      if (fused) em[2 * n + i] = ((const T*)fd.emul2)[i];
-    }
+  for (int r = 0; r < 4; ++r) {
+    if (r >= 2 && !fused) break;
+    const T* src = grow(r);
+    for (int i = tid; i < n; i += 64 * NW) em[r * n + i] = src[i];
   }
   // chunk i of this thread: row = c / CPR, column (c % CPR) * VW, c = tid + i 64 NW;
   // false past the plane's last chunk
@@ -137,7 +133,7 @@ k_plane(LineGeom g, const PairIO* __restrict__ fio) {
     // the first y round trips, each wave as soon as its own columns arrive:
     // twiddles and factors from global memory (their LDS copies are not yet
     // visible), then the plane into LDS
-    line_roundtrip<T, N1, N2, true>(vd, L, tw, tw, (const T*)fd.emul + n, t);
+    line_roundtrip<T, N1, N2, true>(vd, L, tw, tw, grow(1), t);
     if (REUSE) __syncthreads();
     else lds_sync<true>();
     if (lv) {
@@ -178,7 +174,7 @@ k_plane(LineGeom g, const PairIO* __restrict__ fio) {
       else { v[r].x = (T)0; v[r].y = (T)0; }
     }
     if (REUSE) __syncthreads();
-    line_roundtrip<T, N1, N2, true, FUSE>(v, L, tw, twl, ey, t, pc + (ey - em));
+    line_roundtrip<T, N1, N2, true>(v, L, tw, twl, ey, t);
     if (REUSE) __syncthreads();
     else lds_sync<true>();
     if (lv) {
@@ -204,7 +200,7 @@ k_plane(LineGeom g, const PairIO* __restrict__ fio) {
       v[r].y = lv ? P[(2 * cl + 1) * RS + k] : (T)0;
     }
     if (REUSE) __syncthreads();
-    line_roundtrip<T, N1, N2, true, FUSE>(v, L, tw, twl, em, t, pc);
+    line_roundtrip<T, N1, N2, true>(v, L, tw, twl, em, t);
     // register r of this lane: points t + N2 r of rows 2cl (x) and 2cl + 1 (y)
     const long long ra = base + (long long)(2 * cl) * n + t;
     const T* __restrict__ post = (const T*)fd.post;
@@ -239,7 +235,7 @@ k_plane(LineGeom g, const PairIO* __restrict__ fio) {
 #pragma unroll
       for (int r = 0; r < N1; ++r) { v[r].x *= wx[r]; v[r].y *= wy[r]; }
     }
-    line_roundtrip<T, N1, N2, true, FUSE>(v, L, tw, twl, em + 2 * n, t, pc + 2 * n);
+    line_roundtrip<T, N1, N2, true>(v, L, tw, twl, em + 2 * n, t);
     if (REUSE) __syncthreads();
     else lds_sync<true>();
     if (lv) {
@@ -261,7 +257,7 @@ k_plane(LineGeom g, const PairIO* __restrict__ fio) {
       else { v[r].x = (T)0; v[r].y = (T)0; }
     }
     if (REUSE) __syncthreads();
-    line_roundtrip<T, N1, N2, true, FUSE>(v, L, tw, twl, em + 3 * n, t, pc + 3 * n);
+    line_roundtrip<T, N1, N2, true>(v, L, tw, twl, em + 3 * n, t);
     if (!lv) return;
     const T* __restrict__ post2 = (const T*)fd.post2;
     T* __restrict__ out = (T*)fd.out;

@@ -200,48 +200,33 @@ __device__ __forceinline__ void reg_roundtrip(C* v, C* L, const C* __restrict__ 
 // spectral factors, a = K(k) + K(k + n/2), b = K(k) - K(k + n/2)):
 //   lane h = 0 (own E_0, partner E_1):  E_0 <- a E_0 + b W E_1
 //   lane h = 1 (own E_1, partner E_0):  E_1 <- a E_1 + b conj(W) E_0
-// Entry 2k + h of a pair table holds a and c = b W (h = 0) or b conj(W)
-// (h = 1): one exchange and 6 multiply-adds per point instead of two
-// exchanges, two twiddle products, the selects of the lane roles and the
-// multiply.  The FP32 plane kernel at n = 128 went from 2266 to 1709 VALU
-// instructions (dual contour step 18.2 -> 17.9 us at 128^3, 18.3 -> 16.7 us
-// at 96^3).
+// Entry k < n/2 of a row's (a, b) row holds a and b; the host stores it
+// behind every row of spectral factors (spectral_rows), and W comes from the
+// twiddle table.  Per point: one exchange and 4-9 multiply-adds instead of
+// two exchanges, two twiddle products, the selects of the lane roles and the
+// multiply.  Measured at 128^3 (static VALU instructions, dual contour step):
+// the FP32 plane kernel 2266 -> 1731 and 18.2 -> 17.7 us; the FP64 x launch
+// 1527 -> 1120 at 126 instead of 129 VGPRs and 32.5 -> 31.7 us; at 96^3 the
+// FP32 step 18.3 -> 16.5 us, the FP64 step 25.0 -> 22.0 us.
 template <typename T>
-struct __align__(4 * sizeof(T)) PairCoef { T a, cr, ci, pad; };
+struct __align__(2 * sizeof(T)) PairAB { T a, b; };
 
 template <int N1, int N2>
 constexpr bool pair_split() { return N1 % N2 != 0; }
 
-// FP32 lane-pair kernels keep pair tables in LDS beside the twiddles.
-template <typename T, int N1, int N2>
-constexpr bool fused_pairs() { return sizeof(T) == 4 && pair_split<N1, N2>(); }
-
 // LDS bytes of a kernel's table copies: the twiddles (n complex) and `rows`
-// rows of spectral factors, as pair tables where the kernel fuses them.
+// rows of n spectral values (the factors, or their (a, b) rows).
 template <typename T, int N1, int N2>
 constexpr int table_bytes(int rows) {
-  return N1 * N2 * 2 * (int)sizeof(T) +
-         rows * N1 * N2 * (fused_pairs<T, N1, N2>() ? (int)sizeof(PairCoef<T>) : (int)sizeof(T));
+  return N1 * N2 * 2 * (int)sizeof(T) + rows * N1 * N2 * (int)sizeof(T);
 }
 
-// Entries i0, i0 + step, ... of the pair table of the spectral row K
-// (n values); tw is the size-n twiddle table.
-template <typename T, int N1, int N2, typename C>
-__device__ __forceinline__ void pair_coef_fill(PairCoef<T>* pc, const T* K, const C* __restrict__ tw,
-                                               int i0, int step) {
-  constexpr int n = N1 * N2;
-  for (int i = i0; i < n; i += step) {
-    const int k = i >> 1, j = k / N1;
-    const T k0 = K[k], k1 = K[k + n / 2];
-    const C w = tw[j * N1];           // W_N2^j = W_n^{N1 j}
-    const T b = k0 - k1;
-    PairCoef<T> e;
-    e.a = k0 + k1;
-    e.cr = b * w.x;
-    e.ci = (i & 1) ? -(b * w.y) : b * w.y;
-    e.pad = (T)0;
-    pc[i] = e;
-  }
+// The row a round trip reads for the spectral row K (n values, its (a, b)
+// row the n values behind it in line launches): the (a, b) row for lane-pair
+// splits, K itself otherwise.
+template <int N1, int N2, typename T>
+__device__ __forceinline__ const T* pair_row(const T* K) {
+  return pair_split<N1, N2>() ? K + N1 * N2 : K;
 }
 
 // Exchange with the partner lane (lane ^ 1) through a DPP quad permutation
@@ -255,17 +240,32 @@ __device__ __forceinline__ double pair_swap(double x) {
   return __hiloint2double(hi, lo);
 }
 
-// The fused radix 2 / multiply / inverse radix 2 of one point (pair table
-// entry q, see PairCoef).
-template <typename T, typename C>
-__device__ __forceinline__ void pair_apply(C& y, const PairCoef<T>& q) {
+// The fused radix 2 / multiply / inverse radix 2 of one point from its (a, b)
+// entry: y <- a y + b W_h o with o the partner's value, W_h = W_N2^j for lane
+// h = 0 and conj(W_N2^j) for h = 1 (sg = +1 / -1), W_N2^j = tw[tj].
+// j is compile-time: j = 0 takes no twiddle, j = N2/4 (W = -i) a sign.
+template <int N2, typename T, typename C>
+__device__ __forceinline__ void pair_apply_ab(C& y, const PairAB<T>& q, int j, T sg,
+                                              const C* __restrict__ tw, int tj) {
   C o;
   o.x = pair_swap(y.x);
   o.y = pair_swap(y.y);
-  C r;
-  r.x = q.a * y.x + q.cr * o.x - q.ci * o.y;
-  r.y = q.a * y.y + q.cr * o.y + q.ci * o.x;
-  y = r;
+  if (j == 0) {
+    y.x = q.a * y.x + q.b * o.x;
+    y.y = q.a * y.y + q.b * o.y;
+  } else if ((4 * j) % N2 == 0) {   // W_h o = sg (o.y, -o.x)
+    const T bs = sg * q.b;
+    y.x = q.a * y.x + bs * o.y;
+    y.y = q.a * y.y - bs * o.x;
+  } else {
+    const C w = tw[tj];
+    const T s = sg * w.y;
+    C u;
+    u.x = w.x * o.x - s * o.y;
+    u.y = w.x * o.y + s * o.x;
+    y.x = q.a * y.x + q.b * u.x;
+    y.y = q.a * y.y + q.b * u.y;
+  }
 }
 
 // Lane-pair round trip for n = 2 N1^2 on N2 = 2 N1 lanes, N1 points per lane
@@ -277,13 +277,12 @@ __device__ __forceinline__ void pair_apply(C& y, const PairCoef<T>& q) {
 //   in-register DFT_N1 over m: E_h[k2] = sum_m W_N1^{m k2} z[h + 2m];
 //   radix 2 across the pair: lane h keeps X[k1 + N1 (k2 + N1 h)] =
 //   E_0[k2] + (-1)^h W_N2^{k2} E_1[k2], k2 < N1.
-// The spectral multiply follows, then the same steps inverted.  FUSED: the
-// radix 2, the multiply and the inverse radix 2 through the pair table pc
-// (entry t + 2 N1 k2 for lane t and point k2).
-template <typename T, int N1, bool WS, bool FUSED, typename C>
+// That radix 2, the spectral multiply and the inverse radix 2 are one
+// operator (pair_apply_ab) from the (a, b) row ab (entry k1 + N1 k2); then
+// the forward steps inverted.
+template <typename T, int N1, bool WS, typename C>
 __device__ __forceinline__ void pair_roundtrip(C* v, C* L, const C* __restrict__ tw,
-                                               const C* twl, const T* emul, const PairCoef<T>* pc,
-                                               int t) {
+                                               const C* twl, const PairAB<T>* ab, int t) {
   constexpr int N2 = 2 * N1;
   dft_reg<N1, -1, T>(v, tw, N2);
 #pragma unroll
@@ -297,44 +296,9 @@ __device__ __forceinline__ void pair_roundtrip(C* v, C* L, const C* __restrict__
 #pragma unroll
   for (int m = 0; m < N1; ++m) v[m] = L[k1 * N2 + ((h + 2 * m) ^ sw)];
   dft_reg<N1, -1, T>(v, tw, N2);
-  if constexpr (FUSED) {
+  const T sg = h ? (T)-1 : (T)1;
 #pragma unroll
-    for (int k2 = 0; k2 < N1; ++k2) pair_apply<T>(v[k2], pc[t + 2 * N1 * k2]);
-  } else {
-    // radix 2 across the lane pair
-#pragma unroll
-    for (int k2 = 0; k2 < N1; ++k2) {
-      C o;
-      o.x = pair_swap(v[k2].x);
-      o.y = pair_swap(v[k2].y);
-      const C e0 = h ? o : v[k2];
-      const C e1 = h ? v[k2] : o;
-      const C w1 = tw_mul<-1, T>(e1, k2, 2 * N1, tw, k2 * N1);
-      C x;
-      if (h) { x.x = e0.x - w1.x; x.y = e0.y - w1.y; }
-      else { x.x = e0.x + w1.x; x.y = e0.y + w1.y; }
-      const T m = emul[k1 + N1 * (k2 + N1 * h)];
-      x.x *= m; x.y *= m;
-      v[k2] = x;
-    }
-    // inverse radix 2: E_0 = Y[k2] + Y[k2 + N1], E_1 = (Y[k2] - Y[k2 + N1]) W_N2^{-k2}
-#pragma unroll
-    for (int k2 = 0; k2 < N1; ++k2) {
-      C o;
-      o.x = pair_swap(v[k2].x);
-      o.y = pair_swap(v[k2].y);
-      const C lo = h ? o : v[k2];
-      const C hi = h ? v[k2] : o;
-      C e;
-      if (h) {
-        C d; d.x = lo.x - hi.x; d.y = lo.y - hi.y;
-        e = tw_mul<+1, T>(d, k2, 2 * N1, tw, k2 * N1);
-      } else {
-        e.x = lo.x + hi.x; e.y = lo.y + hi.y;
-      }
-      v[k2] = e;
-    }
-  }
+  for (int k2 = 0; k2 < N1; ++k2) pair_apply_ab<N2>(v[k2], ab[k1 + N1 * k2], k2, sg, tw, k2 * N1);
   dft_reg<N1, +1, T>(v, tw, N2);
   lds_sync<WS>();
 #pragma unroll
@@ -350,15 +314,14 @@ __device__ __forceinline__ void pair_roundtrip(C* v, C* L, const C* __restrict__
 // Lane-pair round trip for the general split 2 N1 % N2 == 0 (N2 a power of
 // two): the pair k = t >> 1 takes the P = 2 N1 / N2 columns k1 = k + H u
 // (H = N2 / 2, u < P), lane h of the pair the points t' = h + 2 m (m < H) of
-// each; per column an in-register DFT_H over m and a radix 2 across the
-// pair, as in pair_roundtrip (its P = 1 case).  Registers: column u holds
+// each; per column an in-register DFT_H over m and the fused radix 2 /
+// multiply / inverse radix 2 across the pair ((a, b) entry k1 + N1 j), as in
+// pair_roundtrip (its P = 1 case).  Registers: column u holds
 // v[u H .. u H + H - 1].  With N1 = 12, N2 = 8 a line of 96 takes 8 lanes
-// of 12 points: twice the lines per wave of the 24 x 4 split.  FUSED as in
-// pair_roundtrip (entry t + 2 H u + 2 N1 j).
-template <typename T, int N1, int N2, bool WS, bool FUSED, typename C>
+// of 12 points: twice the lines per wave of the 24 x 4 split.
+template <typename T, int N1, int N2, bool WS, typename C>
 __device__ __forceinline__ void pairs_roundtrip(C* v, C* L, const C* __restrict__ tw,
-                                                const C* twl, const T* emul, const PairCoef<T>* pc,
-                                                int t) {
+                                                const C* twl, const PairAB<T>* ab, int t) {
   constexpr int n = N1 * N2;
   constexpr int H = N2 / 2;
   constexpr int P = N1 / H;
@@ -379,40 +342,12 @@ __device__ __forceinline__ void pairs_roundtrip(C* v, C* L, const C* __restrict_
   }
 #pragma unroll
   for (int u = 0; u < P; ++u) dft_reg<H, -1, T>(v + u * H, tw, n / H);
-  // radix 2 across the pair, the spectral multiply at k1 + N1 (j + H h),
-  // and the inverse radix 2
+  const T sg = h ? (T)-1 : (T)1;
 #pragma unroll
   for (int u = 0; u < P; ++u) {
     const int k1 = k + H * u;
 #pragma unroll
-    for (int j = 0; j < H; ++j) {
-      if constexpr (FUSED) {
-        pair_apply<T>(v[u * H + j], pc[t + 2 * H * u + 2 * N1 * j]);
-        continue;
-      }
-      C& y = v[u * H + j];
-      C o;
-      o.x = pair_swap(y.x);
-      o.y = pair_swap(y.y);
-      const C e0 = h ? o : y;
-      const C e1 = h ? y : o;
-      const C w1 = tw_mul<-1, T>(e1, j, N2, tw, j * N1);
-      C x;
-      if (h) { x.x = e0.x - w1.x; x.y = e0.y - w1.y; }
-      else { x.x = e0.x + w1.x; x.y = e0.y + w1.y; }
-      const T m = emul[k1 + N1 * (j + H * h)];
-      x.x *= m; x.y *= m;
-      o.x = pair_swap(x.x);
-      o.y = pair_swap(x.y);
-      const C lo = h ? o : x;
-      const C hi = h ? x : o;
-      if (h) {
-        C d; d.x = lo.x - hi.x; d.y = lo.y - hi.y;
-        y = tw_mul<+1, T>(d, j, N2, tw, j * N1);
-      } else {
-        y.x = lo.x + hi.x; y.y = lo.y + hi.y;
-      }
-    }
+    for (int j = 0; j < H; ++j) pair_apply_ab<N2>(v[u * H + j], ab[k1 + N1 * j], j, sg, tw, j * N1);
   }
 #pragma unroll
   for (int u = 0; u < P; ++u) dft_reg<H, +1, T>(v + u * H, tw, n / H);
@@ -436,15 +371,14 @@ __device__ __forceinline__ void pairs_roundtrip(C* v, C* L, const C* __restrict_
 // WS: the transpose buffer L is private to the wave, so its write/read
 // hand-overs need only wave-level ordering (multi-wave blocks whose callers
 // separate the shared uses of that memory with block barriers).
-// FUSED (lane-pair splits only): the pair table pc replaces emul.
-template <typename T, int N1, int N2, bool WS = false, bool FUSED = false, typename C>
+// Lane-pair splits read the (a, b) row of the spectral factors (emul =
+// pair_row of the factor row); the other splits the factors themselves.
+template <typename T, int N1, int N2, bool WS = false, typename C>
 __device__ __forceinline__ void line_roundtrip(C* v, C* L, const C* __restrict__ tw,
-                                               const C* twl, const T* emul, int t,
-                                               const PairCoef<T>* pc = nullptr) {
-  static_assert(!FUSED || pair_split<N1, N2>(), "pair tables belong to lane-pair splits");
-  if constexpr (N2 == 2 * N1) pair_roundtrip<T, N1, WS, FUSED>(v, L, tw, twl, emul, pc, t);
+                                               const C* twl, const T* emul, int t) {
+  if constexpr (N2 == 2 * N1) pair_roundtrip<T, N1, WS>(v, L, tw, twl, (const PairAB<T>*)emul, t);
   else if constexpr (N1 % N2 == 0) reg_roundtrip<T, N1, N2, WS>(v, L, tw, twl, emul, t);
-  else pairs_roundtrip<T, N1, N2, WS, FUSED>(v, L, tw, twl, emul, pc, t);
+  else pairs_roundtrip<T, N1, N2, WS>(v, L, tw, twl, (const PairAB<T>*)emul, t);
 }
 
 // Round trip(s) along one axis for 2*64/N2 real lines per wave:
@@ -499,25 +433,16 @@ k_line_reg(LineGeom g, const PairIO* __restrict__ fio) {
                           ? TBB
                           : 2 * 64 / N2 * (n + N2 / 2) * (int)sizeof(T);
   const C* twl = tw;
-  const T* e1 = (const T*)fd.emul;
-  const T* e2 = (const T*)fd.emul2;
-  constexpr bool FUSE = fused_pairs<T, N1, N2>();
-  PairCoef<T>* pc1 = (PairCoef<T>*)(smem_raw + BUF + n * sizeof(C));
-  PairCoef<T>* pc2 = pc1 + n;
+  const T* e1 = pair_row<N1, N2>((const T*)fd.emul);
+  const T* e2 = fused ? pair_row<N1, N2>((const T*)fd.emul2) : nullptr;
   if (TLDS) {
     C* tws = (C*)(smem_raw + BUF);
     T* e1s = (T*)(tws + n);
     T* e2s = e1s + n;
     for (int i = lane; i < n; i += 64) {
       tws[i] = tw[i];
-      if (!FUSE) {
-        e1s[i] = e1[i];
-        if (fused) e2s[i] = e2[i];
-      }
-    }
-    if (FUSE) {
-      pair_coef_fill<T, N1, N2>(pc1, e1, tw, lane, 64);
-      if (fused) pair_coef_fill<T, N1, N2>(pc2, e2, tw, lane, 64);
+      e1s[i] = e1[i];
+      if (fused) e2s[i] = e2[i];
     }
     __syncthreads();
     twl = tws; e1 = e1s; e2 = e2s;
@@ -579,7 +504,7 @@ k_line_reg(LineGeom g, const PairIO* __restrict__ fio) {
     }
   }
 
-  line_roundtrip<T, N1, N2, false, FUSE>(v, L, tw, twl, e1, t, pc1);
+  line_roundtrip<T, N1, N2>(v, L, tw, twl, e1, t);
 
   // ---- store: the step result (post) to mid and on through the second
   // round trip, or straight to out
@@ -651,7 +576,7 @@ k_line_reg(LineGeom g, const PairIO* __restrict__ fio) {
       }
     }
     if (last) break;
-    line_roundtrip<T, N1, N2, false, FUSE>(v, L, tw, twl, e2, t, pc2);
+    line_roundtrip<T, N1, N2>(v, L, tw, twl, e2, t);
   }
 }
 
@@ -694,26 +619,17 @@ k_line_tile(LineGeom g, const PairIO* __restrict__ fio) {
   const int j0 = wave * RL + 2 * p;               // this lane's real-line pair in the tile
 
   constexpr bool TLDS = sizeof(T) == 4;
-  constexpr bool FUSE = fused_pairs<T, N1, N2>();
   const C* twl = tw;
-  const T* e1 = (const T*)fd.emul;
-  const T* e2 = (const T*)fd.emul2;
-  PairCoef<T>* pc1 = (PairCoef<T>*)(smem_raw + BUF + n * sizeof(C));
-  PairCoef<T>* pc2 = pc1 + n;
+  const T* e1 = pair_row<N1, N2>((const T*)fd.emul);
+  const T* e2 = fused ? pair_row<N1, N2>((const T*)fd.emul2) : nullptr;
   if (TLDS) {
     C* tws = (C*)(smem_raw + BUF);
     T* e1s = (T*)(tws + n);
     T* e2s = e1s + n;
     for (int i = tid; i < n; i += 64 * W) {
       tws[i] = tw[i];
-      if (!FUSE) {
-        e1s[i] = e1[i];
-        if (fused) e2s[i] = e2[i];
-      }
-    }
-    if (FUSE) {
-      pair_coef_fill<T, N1, N2>(pc1, e1, tw, tid, 64 * W);
-      if (fused) pair_coef_fill<T, N1, N2>(pc2, e2, tw, tid, 64 * W);
+      e1s[i] = e1[i];
+      if (fused) e2s[i] = e2[i];
     }
     twl = tws; e1 = e1s; e2 = e2s;
   }
@@ -740,7 +656,7 @@ k_line_tile(LineGeom g, const PairIO* __restrict__ fio) {
     v[r].y = S[k * LS + j0 + 1];
   }
   __syncthreads();
-  line_roundtrip<T, N1, N2, true, FUSE>(v, L, tw, twl, e1, t, pc1);
+  line_roundtrip<T, N1, N2, true>(v, L, tw, twl, e1, t);
 
   for (int stage = 0; stage < 2; ++stage) {
     const bool last = stage == 1 || !fused;
@@ -776,7 +692,7 @@ k_line_tile(LineGeom g, const PairIO* __restrict__ fio) {
       v[r].y = S[k * LS + j0 + 1];
     }
     __syncthreads();
-    line_roundtrip<T, N1, N2, true, FUSE>(v, L, tw, twl, e2, t, pc2);
+    line_roundtrip<T, N1, N2, true>(v, L, tw, twl, e2, t);
   }
 }
 

@@ -50,7 +50,7 @@ struct PairIO {
   void* out;
   const void* pre;
   const void* post;
-  const void* emul;   // T[n] spectral multiplier (includes 1/n)
+  const void* emul;   // T[2n]: spectral multiplier (includes 1/n), then its (a, b) row
   void* mid;          // fused pass: the first round trip's result
   const void* pre2;
   const void* emul2;

@@ -401,10 +401,11 @@ psx_status plane_kernel_fit(const Axis& az, int prec, const PairIO* fio, int nfi
 // (once per length and type, outside any stream capture).  FP32 at n = 64
 // and 96 picks the buffer layout by launch size (plane_kernel_fit); the
 // other kernels keep the transpose buffers behind the plane where both fit
-// in LDS, else alias it.  FP64 at 128 uses
-// the (16, 8) split on 8 waves: the lane pairs on 16 waves are held to 128
-// VGPRs and spilled 136 bytes per lane (57.6 -> 48.1 us per plane launch of
-// both sweeps).
+// in LDS, else alias it.  FP64 at 128 runs the lane pairs (8, 16) on 16
+// waves: with the fused pair operator they fit the 128 VGPRs of a
+// 1024-thread block (120, no spills; the three-step form spilled 136 bytes
+// per lane), and against the (16, 8) split on 8 waves the plane pass of a
+// dual step went from 18.3 to 16.2-16.5 us.
 template <typename T>
 psx_status launch_plane(const Axis& az, int prec, const PairIO* fio, int nfields, int nplanes,
                         hipStream_t s, bool prepare = false) {
@@ -419,8 +420,9 @@ psx_status launch_plane(const Axis& az, int prec, const PairIO* fio, int nfields
       break;
     // FP64 at 96 and 128: the first and last y round trips straight from and
     // to global memory (DIRECT; a wave's column pairs are 128-byte row
-    // segments): 128^3 plane launches 19.7 -> 18.4 us per dual step, 96^3
-    // 13.9 -> 13.1 us.  FP32 rows of 32 or 64 bytes lost (128^3 10.4 ->
+    // segments at 96, 64-byte ones at 128): measured with 128-byte segments,
+    // 128^3 plane launches 19.7 -> 18.4 us per dual step, 96^3 13.9 ->
+    // 13.1 us.  FP32 rows of 32 or 64 bytes lost (128^3 10.4 ->
     // 11.9 us; 96^3 8.9 -> 9.7 us; 64^3 4.4 -> 5.0 us), and so did the
     // (16, 8) FP32 split at 128 with them (13.2 us aliased, 12.7 us with the
     // transposes behind the plane).
@@ -430,7 +432,7 @@ psx_status launch_plane(const Axis& az, int prec, const PairIO* fio, int nfields
       break;
     case 128:
       if constexpr (F32) PSX_TRY((plane_kernel<T, 8, 16, 16, false>(az, prec, fio, nfields, nplanes, s, prepare)));
-      else PSX_TRY((plane_kernel<T, 16, 8, 8, true, true>(az, prec, fio, nfields, nplanes, s, prepare)));
+      else PSX_TRY((plane_kernel<T, 8, 16, 16, true, true>(az, prec, fio, nfields, nplanes, s, prepare)));
       break;
     default: return fail(PSX_ERR_INVALID, "no plane kernel for n = " + std::to_string(az.n));
   }
@@ -1248,6 +1250,26 @@ std::vector<double> axis_emul(const Axis& a, double cell, double ds, double b) {
   return e;
 }
 
+// The spectral rows as the kernels read them: the n factors of every row,
+// then one (a, b) row per factor row for the fused lane-pair round trips
+// (psx_fft_reg.hpp, PairAB): entry k < n/2 holds a = K(k) + K(k + n/2) and
+// b = K(k) - K(k + n/2); zeros for odd n.
+std::vector<double> spectral_rows(const std::vector<double>& K, int n) {
+  const size_t rows = K.size() / (size_t)n;
+  std::vector<double> out(K);
+  out.resize(2 * K.size(), 0.0);
+  if (n % 2 != 0) return out;
+  for (size_t r = 0; r < rows; ++r) {
+    const double* k = K.data() + r * n;
+    double* ab = out.data() + K.size() + r * n;
+    for (int i = 0; i < n / 2; ++i) {
+      ab[2 * i] = k[i] + k[i + n / 2];
+      ab[2 * i + 1] = k[i] - k[i + n / 2];
+    }
+  }
+  return out;
+}
+
 // Launch list of one sweep.  Contour step s applies the active axes in the
 // order 0..na-1 when s is even and na-1..0 when s is odd (the axis operators
 // commute), so the last pass of step s and the first of step s+1 run along
@@ -1449,12 +1471,12 @@ psx_status ensure_ckpt(psx_scft* e, int prec) {
 }
 
 // Spectral factors of a plane launch for b class c: the z factors followed
-// by the y factors (both include 1/n).
+// by the y factors (both include 1/n), then their (a, b) rows.
 std::vector<double> plane_factors(const psx_scft* e, size_t c) {
   std::vector<double> em = axis_emul(e->axes[0], e->cell[e->axes[0].axis], e->ds, e->bval[c]);
   const std::vector<double> ey = axis_emul(e->axes[1], e->cell[e->axes[1].axis], e->ds, e->bval[c]);
   em.insert(em.end(), ey.begin(), ey.end());
-  return em;
+  return spectral_rows(em, e->axes[0].n);
 }
 
 psx_status ensure_sweep(psx_scft* e, int prec) {
@@ -1474,7 +1496,7 @@ psx_status ensure_sweep(psx_scft* e, int prec) {
     PSX_TRY(axis_tw(a, prec));
     for (size_t c = 0; c < e->bval.size(); ++c) {
       const double h = a.n == 1 ? 1.0 : e->cell[a.axis];
-      std::vector<double> em = axis_emul(a, h, e->ds, e->bval[c]);
+      std::vector<double> em = spectral_rows(axis_emul(a, h, e->ds, e->bval[c]), a.n);
       if (prec == PSX_FP64) PSX_TRY(upload(&s.emul[c][i], em.data(), em.size() * 8));
       else {
         std::vector<float> f = cast_vec<float>(em);
@@ -2941,7 +2963,7 @@ static psx_status write_emul(psx_scft* e, int prec) {
     const Axis& a = e->axes[i];
     for (size_t c = 0; c < e->bval.size(); ++c) {
       const double h = a.n == 1 ? 1.0 : e->cell[a.axis];
-      std::vector<double> em = axis_emul(a, h, e->ds, e->bval[c]);
+      std::vector<double> em = spectral_rows(axis_emul(a, h, e->ds, e->bval[c]), a.n);
       if (prec == PSX_FP64) {
         PSX_HIP(hipMemcpy(s.emul[c][i], em.data(), em.size() * 8, hipMemcpyHostToDevice));
       } else {
