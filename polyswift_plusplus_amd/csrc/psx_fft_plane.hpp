@@ -54,8 +54,10 @@ struct PlaneShape {
 // columns arrive) and the last ones store theirs straight back (no final
 // barrier or staging pass).  A wave's column pairs are 2 CL elements of a
 // row, so this suits splits whose CL pairs span 64 bytes or more.
-template <typename T, int N1, int N2, int NW, bool REUSE, bool DIRECT = false>
-__global__ void __launch_bounds__(64 * NW)
+// WPE > 0: compiled for at least WPE waves per SIMD (a register budget of
+// 512 / WPE), where that lets more blocks share a CU without spills.
+template <typename T, int N1, int N2, int NW, bool REUSE, bool DIRECT = false, int WPE = 0>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, 64 * NW), amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 k_plane(LineGeom g, const PairIO* __restrict__ fio) {
   typedef typename Cx<T>::type C;
   typedef typename Vec16<T>::type V;
