@@ -2629,7 +2629,10 @@ extern "C" psx_status psx_scft_set_fields(psx_scft* e, const double* wA, const d
   PSX_HIP(hipMemcpyAsync(e->wA, wA, fb, hipMemcpyDeviceToDevice, e->stream));
   PSX_HIP(hipMemcpyAsync(e->wB, wB, fb, hipMemcpyDeviceToDevice, e->stream));
   if (p) PSX_HIP(hipMemcpyAsync(e->praw, p, fb, hipMemcpyDeviceToDevice, e->stream));
-  else PSX_HIP(hipMemsetAsync(e->praw, 0, fb, e->stream));
+  else {   // a fill kernel, not a memset node, so a caller may capture the call
+    hipLaunchKernelGGL(k_fill<double>, dim3(grid_1d(e->N)), dim3(kThreads), 0, e->stream, e->praw, e->N, 0.0);
+    PSX_HIP(hipGetLastError());
+  }
   PSX_TRY(set_pbar_zero(e));
   fields_changed(e);
   return PSX_OK;
@@ -2642,7 +2645,10 @@ extern "C" psx_status psx_scft_set_fields_host(psx_scft* e, const double* wA, co
   PSX_HIP(hipMemcpyAsync(e->wA, wA, fb, hipMemcpyHostToDevice, e->stream));
   PSX_HIP(hipMemcpyAsync(e->wB, wB, fb, hipMemcpyHostToDevice, e->stream));
   if (p) PSX_HIP(hipMemcpyAsync(e->praw, p, fb, hipMemcpyHostToDevice, e->stream));
-  else PSX_HIP(hipMemsetAsync(e->praw, 0, fb, e->stream));
+  else {   // a fill kernel, not a memset node, so a caller may capture the call
+    hipLaunchKernelGGL(k_fill<double>, dim3(grid_1d(e->N)), dim3(kThreads), 0, e->stream, e->praw, e->N, 0.0);
+    PSX_HIP(hipGetLastError());
+  }
   PSX_TRY(set_pbar_zero(e));
   fields_changed(e);
   return PSX_OK;
@@ -3075,6 +3081,14 @@ static psx_status stress_step(psx_scft* e, const T* f, const T* gg, const BlockI
   return PSX_OK;
 }
 
+// The stress accumulators cleared by a fill kernel (no memset node in a
+// caller's captured graph).
+static psx_status clear_sacc(psx_scft* e) {
+  hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(kThreads), 0, e->stream, e->sacc, 4, 0.0);
+  PSX_HIP(hipGetLastError());
+  return PSX_OK;
+}
+
 // For every contour step t of every block: q_t from the head and the
 // propagator from the tail after steps-1-t steps meet across the step
 // operator; sum their spectral product weighted by dK/dL.  Checkpointed:
@@ -3082,7 +3096,7 @@ static psx_status stress_step(psx_scft* e, const T* f, const T* gg, const BlockI
 // with backward slice Ns-1-t).
 template <typename T>
 static psx_status stress_sweep(psx_scft* e, int poly) {
-  PSX_HIP(hipMemsetAsync(e->sacc, 0, 4 * sizeof(double), e->stream));
+  PSX_TRY(clear_sacc(e));
   if (e->ckpt_K > 0) {
     SweepBufs& s = e->sw[e->prec];
     const size_t sb = (size_t)e->N * sizeof(T);
@@ -3233,7 +3247,7 @@ extern "C" psx_status psx_scft_backward(psx_scft* e, const double* gA, const dou
   for (int d = 0; d < 2; ++d) {
     // d = 0: adjoint of the forward sweep, from lam_Ns = h_Ns G_0 down to t = 0;
     // d = 1: adjoint of the backward sweep, from mu_Ns = h_0 F_0
-    if (box) PSX_HIP(hipMemsetAsync(e->sacc, 0, 4 * sizeof(double), e->stream));
+    if (box) PSX_TRY(clear_sacc(e));
     const int t0 = d == 0 ? Ns : 0;
     hipLaunchKernelGGL(k_adj_init, dim3(g), dim3(kThreads), 0, e->stream, d == 0 ? G(0) : F(0), gA, gB,
                        cA[t0], cB[t0], N, lam[0]);
