@@ -3584,21 +3584,34 @@ extern "C" psx_status psx_scft_autotune(psx_scft* e, int reps, int* h_choice) {
 // estimated costs, fixed at creation; results do not depend on it.
 constexpr int kBatchStreams = 4;
 
+// Size classes per group: a batch of more grid sizes runs its classes as
+// consecutive groups of at most this many, each group forked from and
+// joined back to the batch stream and captured as a graph of its own.  Every
+// captured graph then holds at most four class chains, the form the batch
+// runs at its 64-instance workload.
+constexpr int kGroupClasses = 4;
+
+struct BatchGroup {
+  std::vector<int> cls;                 // class indices
+  std::vector<int> post;                // the classes' instances in post-sweep enqueue order
+};
+
 struct psx_batch {
   std::vector<psx_scft*> eng;
   std::vector<BatchClass> cls;
+  std::vector<int> inst_cls;            // per instance: its class
+  std::vector<BatchGroup> groups;
   hipStream_t stream = nullptr;
   bool own_stream = false;
   int prec = PSX_FP64;
   int Ns = 0;
-  std::map<GraphKey, hipGraphExec_t> graphs;
+  std::map<std::pair<GraphKey, int>, hipGraphExec_t> graphs;   // per (mixing, group)
   hipStream_t ss[kBatchStreams] = {};   // ss[0] = stream
   std::vector<long long> graph_gen;     // engines' graph_gen when the graphs were captured
   hipEvent_t ev_fork = nullptr;
   hipEvent_t ev_join[kBatchStreams] = {};
   std::vector<hipEvent_t> ev_cls;       // per class: its sweep is done
   std::vector<int> cls_stream;          // per class
-  std::vector<int> post_order;          // instances in post-sweep enqueue order
   std::vector<int> inst_stream;         // per instance
 };
 
@@ -3616,40 +3629,42 @@ static double batch_post_cost(const psx_batch* b, const psx_scft* e) {
   return (double)e->N * (2.0 * (b->Ns + 1) * pb + 200.0) / 4.0e6 + 9 * 4.0;
 }
 
+// The list schedule of each group on its own (the groups run one after
+// another).
 static void batch_schedule(psx_batch* b) {
   const int nc = (int)b->cls.size();
-  double ready[kBatchStreams] = {};
-  std::vector<double> done(nc);
-  std::vector<int> order(nc);
-  for (int c = 0; c < nc; ++c) order[c] = c;
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-    return batch_sweep_cost(b, b->cls[x]) > batch_sweep_cost(b, b->cls[y]);
-  });
+  std::vector<double> done(nc, 0.0);
   b->cls_stream.assign(nc, 0);
-  for (int c : order) {
-    int k = 0;
-    for (int j = 1; j < kBatchStreams; ++j) if (ready[j] < ready[k]) k = j;
-    b->cls_stream[c] = k;
-    ready[k] += batch_sweep_cost(b, b->cls[c]);
-    done[c] = ready[k];
-  }
-  std::vector<int> cls_of(b->eng.size());
-  for (int c = 0; c < nc; ++c) for (int m : b->cls[c].members) cls_of[m] = c;
-  b->post_order.resize(b->eng.size());
-  for (size_t i = 0; i < b->eng.size(); ++i) b->post_order[i] = (int)i;
-  std::stable_sort(b->post_order.begin(), b->post_order.end(), [&](int x, int y) {
-    return done[cls_of[x]] < done[cls_of[y]];
-  });
   b->inst_stream.assign(b->eng.size(), 0);
-  for (int i : b->post_order) {
-    int k = 0;
-    double best = 1e300;
-    for (int j = 0; j < kBatchStreams; ++j) {
-      const double start = std::max(ready[j], done[cls_of[i]]);
-      if (start < best) { best = start; k = j; }
+  for (BatchGroup& grp : b->groups) {
+    double ready[kBatchStreams] = {};
+    std::vector<int> order = grp.cls;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+      return batch_sweep_cost(b, b->cls[x]) > batch_sweep_cost(b, b->cls[y]);
+    });
+    for (int c : order) {
+      int k = 0;
+      for (int j = 1; j < kBatchStreams; ++j) if (ready[j] < ready[k]) k = j;
+      b->cls_stream[c] = k;
+      ready[k] += batch_sweep_cost(b, b->cls[c]);
+      done[c] = ready[k];
     }
-    b->inst_stream[i] = k;
-    ready[k] = best + batch_post_cost(b, b->eng[i]);
+    grp.post.clear();
+    for (size_t i = 0; i < b->eng.size(); ++i)
+      if (std::find(grp.cls.begin(), grp.cls.end(), b->inst_cls[i]) != grp.cls.end()) grp.post.push_back((int)i);
+    std::stable_sort(grp.post.begin(), grp.post.end(), [&](int x, int y) {
+      return done[b->inst_cls[x]] < done[b->inst_cls[y]];
+    });
+    for (int i : grp.post) {
+      int k = 0;
+      double best = 1e300;
+      for (int j = 0; j < kBatchStreams; ++j) {
+        const double start = std::max(ready[j], done[b->inst_cls[i]]);
+        if (start < best) { best = start; k = j; }
+      }
+      b->inst_stream[i] = k;
+      ready[k] = best + batch_post_cost(b, b->eng[i]);
+    }
   }
 }
 
@@ -3742,6 +3757,12 @@ extern "C" psx_status psx_batch_create(int n, const psx_grid* grids, const psx_d
         for (int d = 0; d < 3; ++d) found->dims[d] = e->dims[d];
       }
       found->members.push_back(i);
+    }
+    b->inst_cls.assign(n, 0);
+    for (size_t c = 0; c < b->cls.size(); ++c) {
+      for (int m : b->cls[c].members) b->inst_cls[m] = (int)c;
+      if (c % kGroupClasses == 0) b->groups.emplace_back();
+      b->groups.back().cls.push_back((int)c);
     }
     for (auto& c : b->cls) {
       psx_scft* e0 = b->eng[c.members[0]];
@@ -3885,22 +3906,21 @@ static psx_status batch_time_class(psx_batch* b, size_t ci, int reps, float* ms)
   return st;
 }
 
-// One batch iteration on the batch streams, forked from and joined back to
-// b->stream (capturable).  An instance's post-sweep work is enqueued on its
-// batch stream by lending that stream to the engine for the call.
-static psx_status batch_enqueue(psx_batch* b, const psx_mixing* mix) {
+// One batch iteration of one group's instances on the batch streams, forked
+// from and joined back to b->stream (capturable).  An instance's post-sweep
+// work is enqueued on its batch stream by lending that stream to the engine
+// for the call.
+static psx_status batch_enqueue(psx_batch* b, const psx_mixing* mix, const BatchGroup& grp) {
   PSX_HIP(hipEventRecord(b->ev_fork, b->stream));
   for (int k = 1; k < kBatchStreams; ++k) PSX_HIP(hipStreamWaitEvent(b->ss[k], b->ev_fork, 0));
-  for (size_t ci = 0; ci < b->cls.size(); ++ci) {
-    if (b->prec == PSX_FP64) PSX_TRY(batch_sweep<double>(b, ci));
-    else PSX_TRY(batch_sweep<float>(b, ci));
+  for (int ci : grp.cls) {
+    if (b->prec == PSX_FP64) PSX_TRY(batch_sweep<double>(b, (size_t)ci));
+    else PSX_TRY(batch_sweep<float>(b, (size_t)ci));
   }
-  std::vector<int> cls_of(b->eng.size());
-  for (size_t c = 0; c < b->cls.size(); ++c) for (int m : b->cls[c].members) cls_of[m] = (int)c;
-  for (int i : b->post_order) {
+  for (int i : grp.post) {
     psx_scft* e = b->eng[i];
     hipStream_t s = b->ss[b->inst_stream[i]];
-    const int c = cls_of[i];
+    const int c = b->inst_cls[i];
     if (b->cls_stream[c] != b->inst_stream[i]) PSX_HIP(hipStreamWaitEvent(s, b->ev_cls[c], 0));
     hipStream_t own = e->stream;
     e->stream = s;
@@ -3929,27 +3949,30 @@ extern "C" psx_status psx_batch_iterate(psx_batch* b, const psx_mixing* mix, int
     batch_drop_graphs(b);
     b->graph_gen = gen;
   }
-  hipGraphExec_t exec = nullptr;
+  const size_t ng = b->groups.size();
+  std::vector<hipGraphExec_t> exec(ng, nullptr);
   if (use_graph) {
-    GraphKey key = graph_key(b->prec, mix);
-    auto it = b->graphs.find(key);
-    if (it != b->graphs.end()) exec = it->second;
-    else {
+    const GraphKey key = graph_key(b->prec, mix);
+    for (size_t gi = 0; gi < ng; ++gi) {
+      auto it = b->graphs.find({key, (int)gi});
+      if (it != b->graphs.end()) { exec[gi] = it->second; continue; }
       hipGraph_t graph;
       PSX_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-      psx_status cs = batch_enqueue(b, mix);
+      psx_status cs = batch_enqueue(b, mix, b->groups[gi]);
       hipError_t ce = hipStreamEndCapture(b->stream, &graph);
       if (cs != PSX_OK) { if (ce == hipSuccess) (void)hipGraphDestroy(graph); return cs; }
       if (ce != hipSuccess) return fail(PSX_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
-      hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      hipError_t ie = hipGraphInstantiate(&exec[gi], graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
       if (ie != hipSuccess) return fail(PSX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
-      b->graphs[key] = exec;
+      b->graphs[{key, (int)gi}] = exec[gi];
     }
   }
   for (int it = 0; it < n_iters; ++it) {
-    if (exec) PSX_HIP(hipGraphLaunch(exec, b->stream));
-    else PSX_TRY(batch_enqueue(b, mix));
+    for (size_t gi = 0; gi < ng; ++gi) {
+      if (exec[gi]) PSX_HIP(hipGraphLaunch(exec[gi], b->stream));
+      else PSX_TRY(batch_enqueue(b, mix, b->groups[gi]));
+    }
   }
   PSX_HIP(hipStreamSynchronize(b->stream));
   psx_status result = PSX_OK;

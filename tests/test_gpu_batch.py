@@ -53,6 +53,33 @@ def test_batch_matches_single_instances(prec, use_graph, monkeypatch):
     b.close()
 
 
+MANY = [dict(shape=s, cell=(0.3,) * 3, fA=0.25 + 0.05 * (i % 4), ds=0.05, chiN=12.0 + i)
+        for i, s in enumerate([(8, 8, 8), (12, 12, 12), (16, 16, 16), (10, 10, 10), (16, 8, 8),
+                               (24, 12, 6), (18, 18, 18), (6, 6, 6), (20, 20, 20)])]
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_of_nine_sizes_in_groups(use_graph, monkeypatch):
+    """Nine grid sizes are nine size classes, run as three groups of at most
+    four (one graph each): every instance still follows its single-engine
+    trajectory bit for bit."""
+    print(f"batch of 9 sizes graph={use_graph}", flush=True)
+    monkeypatch.setenv("PSX_OVERLAP_QUAD", "0")
+    b = BatchSCFT(MANY, prec="fp64")
+    assert b.nclasses == 9
+    for i, sp in enumerate(MANY):
+        b.instances[i].setFields(*_fields(i, sp))
+    h1 = b.iterate(2, MIX, use_graph=use_graph)
+    h2 = b.iterate(3, MIX, use_graph=use_graph)
+    for i, sp in enumerate(MANY):
+        ref = _single(sp, i, "fp64", 5, fields_out=True)
+        F = np.r_[h1["F"][i], h2["F"][i]]
+        assert np.max(np.abs(F - ref["F"])) <= 1e-14 * np.max(np.abs(ref["F"])), i
+        for got, want in zip(b.instances[i].getFields(), ref["fields"]):
+            assert np.array_equal(got, want), i
+    b.close()
+
+
 def test_batch_recaptures_after_engine_buffers_change(monkeypatch):
     """A deeper Anderson history on the second call reallocates every
     instance's ring: the batch graph is captured again, not replayed with the
