@@ -842,51 +842,78 @@ k_anderson_solve(int m, double beta, double lam0, double start, DevState* st) {
 }
 
 // Anderson: x_new = x + sum c_i (x_i - x) + beta (f + sum c_i (f_i - f)),
-// then p_raw, w-factors and the free-energy partials.
-template <typename T>
+// then p_raw, w-factors and the free-energy partials.  V consecutive cells
+// per thread (fields and ring slots read as aligned V-vectors; N % V == 0)
+// and the history loop unrolled to MAXM entries, so a thread has all its
+// ring loads in flight at once; each cell's operations and their order do
+// not depend on V or MAXM.
+template <typename T, int V, int MAXM>
 __global__ void __launch_bounds__(kThreads)
 k_mix(QuadArgs a) {
-  const long long N = a.N;
+  typedef CellVec<T, V> RV;
+  typedef CellVec<double, V> DV;
+  const long long N = a.N, NV = N / V;
   const int m1 = a.m1;
   const DevState* st = a.st;
   const long long cur = st->iter % m1;
   const int mk = st->mk;
   const double beta = st->beta_used;
-  const T* rx = (const T*)a.ringx;
-  const T* rf = (const T*)a.ringf;
-  const T* fc = rf + cur * 2 * N;
+  const RV* __restrict__ rx = (const RV*)a.ringx;
+  const RV* __restrict__ rf = (const RV*)a.ringf;
+  const long long slot = 2 * NV;   // one ring slot, in vectors
   double acc[kNPart] = {0.0, 0.0, 0.0, 0.0};
-  for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < N;
-       r += (long long)gridDim.x * blockDim.x) {
-    double xb[2], fb[2], old[2];
+  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < NV;
+       g += (long long)gridDim.x * blockDim.x) {
+    const long long r0 = g * V;
+    double xb[2][V], fb[2][V], old[2][V];
+#pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const long long o = h * N + r;
-      const double x = h == 0 ? a.wA[r] : a.wB[r];
-      const double f = (double)fc[o];
-      double xs = x, fs = f;
-      for (int i = 0; i < mk; ++i) {
-        const double ci = st->coef[i];
-        const long long sl = (cur - 1 - i + m1) % m1;
-        xs += ci * ((double)rx[sl * 2 * N + o] - x);
-        fs += ci * ((double)rf[sl * 2 * N + o] - f);
+      const DV xv = *(const DV*)((h == 0 ? a.wA : a.wB) + r0);
+      const RV fv = rf[cur * slot + h * NV + g];
+      double f[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        old[h][v] = xv.v[v];
+        f[v] = (double)fv.v[v];
+        xb[h][v] = old[h][v];
+        fb[h][v] = f[v];
       }
-      old[h] = x;
-      xb[h] = xs;
-      fb[h] = fs;
+#pragma unroll
+      for (int i = 0; i < MAXM; ++i) {
+        if (i < mk) {
+          const double ci = st->coef[i];
+          const long long sl = (cur - 1 - i + m1) % m1;
+          const RV xr = rx[sl * slot + h * NV + g];
+          const RV fr = rf[sl * slot + h * NV + g];
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            xb[h][v] += ci * ((double)xr.v[v] - old[h][v]);
+            fb[h][v] += ci * ((double)fr.v[v] - f[v]);
+          }
+        }
+      }
     }
-    // step with the reference's A/B coupling: dwA = beta (fA - lam1 fB)
-    double nw[2];
-    nw[0] = xb[0] + (fb[0] + (-a.lam1) * fb[1]) * beta;
-    nw[1] = xb[1] + (fb[1] + (-a.lam1) * fb[0]) * beta;
-    const double pr = (nw[0] + nw[1]) * 0.5;
-    a.wA[r] = nw[0]; a.wB[r] = nw[1]; a.praw[r] = pr;
-    ((T*)a.wfA)[r] = (T)exp(-a.halfds * nw[0]);
-    ((T*)a.wfB)[r] = (T)exp(-a.halfds * nw[1]);
-    const double phA = a.phiA[r], phB = a.phiB[r];
-    acc[0] += pr;
-    if (a.f_order == 0) acc[1] += -(phA * nw[0]) - (phB * nw[1]);
-    else acc[1] += -(phA * old[0]) - (phB * old[1]);
-    acc[2] += a.chiN * phA * phB;
+    const DV phA = *(const DV*)(a.phiA + r0), phB = *(const DV*)(a.phiB + r0);
+    DV nA, nB, pr;
+    CellVec<T, V> fA, fB;
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      // step with the reference's A/B coupling: dwA = beta (fA - lam1 fB)
+      nA.v[v] = xb[0][v] + (fb[0][v] + (-a.lam1) * fb[1][v]) * beta;
+      nB.v[v] = xb[1][v] + (fb[1][v] + (-a.lam1) * fb[0][v]) * beta;
+      pr.v[v] = (nA.v[v] + nB.v[v]) * 0.5;
+      fA.v[v] = (T)exp(-a.halfds * nA.v[v]);
+      fB.v[v] = (T)exp(-a.halfds * nB.v[v]);
+      acc[0] += pr.v[v];
+      if (a.f_order == 0) acc[1] += -(phA.v[v] * nA.v[v]) - (phB.v[v] * nB.v[v]);
+      else acc[1] += -(phA.v[v] * old[0][v]) - (phB.v[v] * old[1][v]);
+      acc[2] += a.chiN * phA.v[v] * phB.v[v];
+    }
+    *(DV*)(a.wA + r0) = nA;
+    *(DV*)(a.wB + r0) = nB;
+    *(DV*)(a.praw + r0) = pr;
+    *(CellVec<T, V>*)((T*)a.wfA + r0) = fA;
+    *(CellVec<T, V>*)((T*)a.wfB + r0) = fB;
   }
   block_sum<kNPart>(acc);
   if (threadIdx.x == 0) {

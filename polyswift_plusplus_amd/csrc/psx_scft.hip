@@ -2019,6 +2019,24 @@ void launch_gram(psx_scft* e, int m1) {
   else launch_gram_v<R, 1>(e, m1);
 }
 
+// The mixing step reads 16-byte vectors of cells when N allows, with its
+// history loop unrolled to the depth's bucket.
+template <typename R, int V>
+void launch_mix_v(psx_scft* e, const QuadArgs& a, int m) {
+  const dim3 gr(e->nb), bl(kThreads);
+  if (m <= 3) hipLaunchKernelGGL((k_mix<R, V, 3>), gr, bl, 0, e->stream, a);
+  else if (m <= 10) hipLaunchKernelGGL((k_mix<R, V, 10>), gr, bl, 0, e->stream, a);
+  else if (m <= 20) hipLaunchKernelGGL((k_mix<R, V, 20>), gr, bl, 0, e->stream, a);
+  else hipLaunchKernelGGL((k_mix<R, V, kMaxDepth>), gr, bl, 0, e->stream, a);
+}
+
+template <typename R>
+void launch_mix(psx_scft* e, const QuadArgs& a, int m) {
+  constexpr int V = 16 / (int)sizeof(R);
+  if (e->N % V == 0) launch_mix_v<R, V>(e, a, m);
+  else launch_mix_v<R, 1>(e, a, m);
+}
+
 // Q, quadrature and the field update after both sweeps (capturable).
 psx_status enqueue_post_sweep(psx_scft* e, const psx_mixing* mix) {
   PSX_TRY(launch_Q(e));
@@ -2038,10 +2056,8 @@ psx_status enqueue_post_sweep(psx_scft* e, const psx_mixing* mix) {
   hipLaunchKernelGGL(k_anderson_solve, dim3(1), dim3(64), 0, e->stream, m, mix->beta, mix->lambda0,
                      mix->anderson_start, e->st);
   QuadArgs a = quad_args(e, QM_SD, mix);
-  if (e->prec == PSX_FP64)
-    hipLaunchKernelGGL(k_mix<double>, dim3(e->nb), dim3(kThreads), 0, e->stream, a);
-  else
-    hipLaunchKernelGGL(k_mix<float>, dim3(e->nb), dim3(kThreads), 0, e->stream, a);
+  if (e->prec == PSX_FP64) launch_mix<double>(e, a, m);
+  else launch_mix<float>(e, a, m);
   PSX_HIP(hipGetLastError());
   PSX_TRY(launch_finish(e, FIN_ITER_MIX, e->hist));
   return PSX_OK;
