@@ -827,13 +827,17 @@ k_anderson_solve(int m, double beta, double lam0, double start, DevState* st) {
       for (int r = c + 1 + lane; r < mk; r += 64) V[r] -= Fr[r] * V[c];
       __syncthreads();
     }
+    // back substitution in LDS (the coefficients go to global memory once)
     if (s_ok && lane == 0) {
       for (int r = mk - 1; r >= 0; --r) {
         double sacc = V[r];
-        for (int k = r + 1; k < mk; ++k) sacc -= U[r * mk + k] * st->coef[k];
-        st->coef[r] = sacc / U[r * mk + r];
+        for (int k = r + 1; k < mk; ++k) sacc -= U[r * mk + k] * Fr[k];
+        Fr[r] = sacc / U[r * mk + r];
       }
     }
+    __syncthreads();
+    if (s_ok)
+      for (int i = lane; i < mk; i += 64) st->coef[i] = Fr[i];
   }
   if (lane == 0) {
     if (s_ok) { st->mk = mk; st->beta_used = beta; }
