@@ -459,7 +459,8 @@ psx_status psx_scft_sweep_mode(const psx_scft* e, int* h_mode, int* h_available)
 /* box, fA, chiN may all differ; the contour step count must match)        */
 /* advanced together.  Instances with the same grid share each line-pass   */
 /* launch (both sweeps of every member), the per-instance reductions and   */
-/* updates follow, and one batch iteration is one hipGraph.  No reference  */
+/* updates follow, and one batch iteration is one hipGraph per group of at */
+/* most four size classes (groups run one after another).  No reference   */
 /* counterpart (the reference runs one system per process).                */
 /* ===================================================================== */
 typedef struct psx_batch psx_batch;
