@@ -24,7 +24,8 @@ def test_mixed_protocol_reaches_fp64_free_energy():
     print(f"fp64 {h['iters']} it; mixed {r['fp32_iters']} + {r['fp64_iters']} it", flush=True)
     assert r["err"] < 1e-8
     assert abs(r["F"] - h["F"][-1]) < 1e-8
-    assert tail_mixing(mix).depth == 30 and tail_mixing(Mixing("steepest")).kind == "steepest"
+    assert tail_mixing(mix).depth == 30 and tail_mixing(mix).beta == 0.5
+    assert tail_mixing(Mixing("steepest")).kind == "steepest"
     ref.close()
     e.close()
 
